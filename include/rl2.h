@@ -1,0 +1,154 @@
+/*
+ * rl2.h -- device-resident C-ABI of librl2hip.so (MI355X / gfx950).
+ *
+ * RadioLensfit2 keeps the whole observation in host memory and hands raw pointers to
+ * every call on the per-galaxy path (data_processing.c:49-142).  On the GPU the
+ * observation must stay resident in HBM across calls, so this layer adds an observation
+ * handle (rl2_obs) that owns the device copy.  Every entry point takes plain pointers and
+ * sizes, returns an int status (RL2_OK = 0, negative on error, message via
+ * rl2_last_error()), and never falls back to the CPU: without a usable GPU the calls
+ * return RL2_ERR_NO_DEVICE.
+ *
+ * Device representation of the reference's two visibility arrays: the path only ever uses
+ * their difference visData - visSkyMod (source_extraction.c:56-61) and updates both by the
+ * same galaxy (data_processing.c:109-135), so the device holds the residual
+ * R = visData - visSkyMod.  rl2_obs_get_residual() returns R; rl2_obs_sync() writes
+ * visData = R and visSkyMod = 0, which leaves every quantity the path computes unchanged.
+ */
+#ifndef RL2_H
+#define RL2_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RL2_OK = 0,
+    RL2_ERR_NO_DEVICE = -1,
+    RL2_ERR_ARG = -2,
+    RL2_ERR_HIP = -3,
+    RL2_ERR_RANGE = -4,
+    RL2_ERR_STATE = -5,
+};
+
+/* Arithmetic of an observation's hot path.
+ * RL2_PREC_FP64: residual complex128, FP64 model, likelihood and accumulators (the
+ *   reference's arithmetic).
+ * RL2_PREC_MIXED: residual complex64; FP64 phase-argument reduction and FP64 gridding
+ *   accumulators; FP32 model and cross-correlation terms with FP64 chi-square sums;
+ *   FP64 position marginalisation, log-sum-exp and scalelength integration. */
+enum { RL2_PREC_FP64 = 0, RL2_PREC_MIXED = 1 };
+
+typedef struct rl2_obs rl2_obs;
+
+/* Per-galaxy result; the fields mirror the reference's output line
+ * (data_processing.c:102,136) plus bookkeeping. */
+typedef struct {
+    double e1, e2, var_e1, var_e2, oneDimvar, maxL;
+    int error;          /* source_fitting status of the last fit (galaxy_fitting.c:89-110) */
+    int pass;           /* 0 first pass, 1 bad-list re-fit pass (RadioLensfit2.cpp:597-606) */
+    unsigned int facet, ncoords;
+    int n_evals;        /* loglikelihood evaluations requested by the fit */
+    int subtracted;     /* 1 when the fitted model was removed from the residual */
+} rl2_gal_result;
+
+typedef struct {
+    double extraction_s, fitting_s, subtraction_s;
+    int n_bad_first, n_bad_final;
+} rl2_run_stats;
+
+/* Fit-driver options (0 = default for every field). */
+typedef struct {
+    int batch;             /* galaxies extracted/fitted per residual update (>=1); batch > 1
+                              follows data_processing-mpi.c:65-216 */
+    int do_refit;          /* 1: run the bad-list re-fit pass, as the reference always does */
+    int nm_speculate;      /* 1: evaluate reflect/expand/contract candidates in one launch */
+    int line_chunk;        /* line-search steps evaluated per launch (default 4) */
+    int lik_block;         /* threads per likelihood work-group (default 256) */
+    int verbose;
+} rl2_fit_options;
+
+const char *rl2_last_error(void);
+/* Select/initialise the device (hipSetDevice); idempotent. */
+int rl2_init(int device);
+int rl2_device_count(int *count);
+
+/* ---- observation (replaces the arrays RadioLensfit2.cpp:157-535 allocates) ---- */
+/* u,v,w: nrow metres; wavenumbers/spec: nch (RadioLensfit2.cpp:342-351);
+ * sigma2_vis: nch*nrow variances (channel-major) after defaulting
+ * (RadioLensfit2.cpp:255-270); flag: nch*nrow bytes or NULL; len: maximum uv distance in
+ * wavelengths (RadioLensfit2.cpp:408). */
+int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const double *u,
+                   const double *v, const double *w, const double *wavenumbers,
+                   const double *spec, const float *sigma2_vis, const unsigned char *flag,
+                   double len, int precision);
+int rl2_obs_destroy(rl2_obs *obs);
+/* R = visData - visSkyMod (visSkyMod may be NULL); complexd arrays of nch*nrow. */
+int rl2_obs_set_vis(rl2_obs *obs, const double *visData, const double *visSkyMod);
+int rl2_obs_get_residual(rl2_obs *obs, double *residual /* 2*nch*nrow */);
+int rl2_obs_sync(rl2_obs *obs, double *visData, double *visSkyMod);
+/* R += sign * sum_g model_g (data_galaxy_visibilities, galaxy_visibilities.c:154-190).
+ * sky_model (data_simulation.c:131-165) is sign = -1 with e = 0: visSkyMod += sum of round
+ * models.  e1/e2 may be NULL (round models). */
+int rl2_obs_add_sources(rl2_obs *obs, unsigned int ngal, const double *flux,
+                        const double *scale, const double *l, const double *m,
+                        const double *e1, const double *e2, double sign);
+/* R += complex Gaussian noise of variance sigma2 per component (counter-based stream). */
+int rl2_obs_add_noise(rl2_obs *obs, double sigma2, uint64_t seed);
+
+/* ---- one galaxy at a time (the serial path of data_processing.c) ---- */
+/* source_extraction + evaluate_facet_coords for one galaxy into facet slot `slot`.
+ * Host outputs are optional (NULL): facet_vis (2*n), facet_sigma2 (n), uu/vv (n),
+ * sum_w (facet^2 dense weights). *ncoords receives n. */
+int rl2_obs_extract(rl2_obs *obs, int slot, double l0, double m0, double flux, double R_mu,
+                    unsigned int facet, double *facet_vis, double *facet_sigma2, double *uu,
+                    double *vv, double *sum_w, unsigned int *ncoords);
+/* Upload a host facet into slot `slot` (used by the reference-signature drop-ins). */
+int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *facet_vis,
+                       const double *facet_sigma2, const double *uu, const double *vv);
+/* log-prior of the scalelength grid for slot (rfunc, distributions.c:72-87) */
+int rl2_obs_set_prior(rl2_obs *obs, int slot, const double *rprior /* numr */);
+/* loglikelihood (likelihood.c:106-149) at npts ellipticities for the facet in `slot`.
+ * L_r (npts*numr, may be NULL) receives the per-scalelength values of likelihood.c:157-187,
+ * n_iter (npts*numr, may be NULL) the Newton iterations of cross_correlation. */
+int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const double *e2,
+                   double *L_e, int *error, double *L_r, int *n_iter);
+/* source_fitting (galaxy_fitting.c:41-117) for the facet in `slot`. */
+int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_result *res);
+/* Remove a fitted galaxy: visSkyMod -= round model, visData -= fitted model
+ * (data_processing.c:109-135). */
+int rl2_obs_subtract(rl2_obs *obs, double l0, double m0, double flux, double R_mu, double e1,
+                     double e2);
+
+/* ---- the whole per-galaxy loop (data_processing.c:49-142, RadioLensfit2.cpp:597-606) ---- */
+int rl2_process(rl2_obs *obs, unsigned int ngal, const double *l, const double *m,
+                const double *flux, const double *scale, const rl2_fit_options *opt,
+                rl2_gal_result *results, rl2_run_stats *stats);
+
+/* ---- scalelength grid and the fixed constants (RadioLensfit2.cpp:370-392) ---- */
+int rl2_r_grid(double *ro /* >= 29 */);
+
+/* ---- instrumentation: per-kernel device time measured with HIP events on the stream
+ * the kernels run on, plus algorithmic work counters. ---- */
+typedef struct {
+    double ms;            /* summed kernel time */
+    long long launches;
+    double flops;         /* algorithmic FLOP-equivalents (likelihood) */
+    double bytes;         /* algorithmic HBM bytes (streaming kernels) */
+} rl2_kernel_stat;
+enum { RL2_K_EXTRACT = 0, RL2_K_COMPACT = 1, RL2_K_LIKELIHOOD = 2, RL2_K_RMARG = 3,
+       RL2_K_SUBTRACT = 4, RL2_K_SOURCES = 5, RL2_K_COUNT = 6 };
+int rl2_stats_enable(int on);
+int rl2_stats_reset(void);
+int rl2_stats_get(int kernel, rl2_kernel_stat *out);
+
+/* Block until all work queued by the library has finished. */
+int rl2_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

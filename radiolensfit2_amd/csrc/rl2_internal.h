@@ -1,0 +1,148 @@
+// rl2_internal.h -- device descriptors and kernel launchers shared by the runtime,
+// the fit driver and the reference-signature drop-ins.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "rl2_math.h"
+
+namespace rl2 {
+
+// ---------------------------------------------------------------- descriptors (device)
+// One extracted facet (the reference's par->uu/vv/data/sigma2, data_processing.c:82-84):
+// compacted non-empty cells in row-major cell order, data already divided by sigma^2.
+struct FacetDev {
+    const double *uu, *vv;     // cell centres in wavelengths
+    const double *dre, *dim;   // gridded data / variance
+    const double *s2;          // gridded variance
+    const double *mod;         // optional given model (drop-in cross_correlation); else null
+    unsigned int n;
+    unsigned int pad;
+    double rprior[kMaxR];      // log prior per scalelength index
+};
+
+// One likelihood evaluation request.
+struct LikItem {
+    int slot;
+    int pad;
+    double e1, e2;
+};
+
+struct RGrid {
+    int numr;                  // 24: ro[1..23] are the models (RadioLensfit2.cpp:370-392)
+    int pad;
+    double ro[kMaxR];
+};
+
+// Streaming-kernel arguments (one galaxy).
+struct GalaxyGeom {
+    double l0, m0, n0;         // n0 = sqrt(1 - l^2 - m^2) - 1
+    double flux;
+    double sf;                 // (scale * ARCS2RAD)^2 / detA^2
+    double e1, e2;
+};
+
+// ---------------------------------------------------------------- launchers
+// All launches go to `stream`; errors are reported through hipGetLastError by the caller.
+
+// R[c][i] += sign * sum_g model_g ; gal: ngal GalaxyGeom on the device.
+void launch_add_sources(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                        const double *u, const double *v, const double *w, const double *wn,
+                        const double *spec, const GalaxyGeom *gal, unsigned ngal, double sign,
+                        void *res);
+void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma, uint64_t seed,
+                      void *res);
+
+// Pillbox gridding into dense accumulators acc[4][facet^2] (re, im, sigma2, weight).
+// mode 0: extraction (round model of `g` added, phase-shifted to (l0,m0)), reads the
+//         residual `res` (prec selects complex128/complex64).
+// mode 1: plain gridding of complex128 `res` (drop-in gridding_visibilities).
+// mode 2: weights only (evaluate_uv_grid_size).
+struct GridLaunch {
+    int mode, prec;
+    unsigned nch;
+    unsigned long nrow;
+    const double *u, *v, *w;
+    const double *wn, *inv_lambda, *spec;
+    const void *res;
+    const float *s2f;          // sigma^2 with the sign bit set for flagged visibilities
+    GalaxyGeom g;
+    double len, inc;
+    unsigned facet;
+    double *acc;               // 4 * facet^2
+};
+void launch_grid(hipStream_t s, const GridLaunch &a);
+
+// Compaction of dense accumulators into the facet arrays; clears the accumulators.
+// n_out (device) receives the number of non-empty cells. sum_w_dense (optional, device)
+// receives the dense weights before clearing.
+void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
+                    double *vv, double *dre, double *dim, double *s2, unsigned *n_out,
+                    double *sum_w_dense, unsigned *scratch_blocks);
+size_t compact_scratch_words(unsigned facet);
+
+// Likelihood: per (item, r) work-group L_r and Newton iterations; then r-marginalisation.
+struct LikLaunch {
+    int nitems;
+    const LikItem *items;
+    const FacetDev *facets;
+    RGrid rg;
+    double *Lr;                // nitems * kMaxR
+    int *nit;                  // nitems * kMaxR
+    double *Le;                // nitems
+    int *err;                  // nitems
+    double *qag_ws;            // nitems * kQagDoubles
+    int *qag_iws;              // nitems * kQagInts
+    int block;                 // threads per work-group (64..1024)
+    int prec;
+    double *ho = nullptr;      // optional nitems * kMaxR: cross-correlation peak value
+    double *det = nullptr;     // optional nitems * kMaxR: Hessian determinant at the peak
+};
+constexpr int kQagLimit = 1000;
+constexpr int kQagDoubles = 4 * kQagLimit;
+constexpr int kQagInts = 2 * kQagLimit;
+void launch_likelihood(hipStream_t s, const LikLaunch &a);
+void launch_rmarg(hipStream_t s, const LikLaunch &a);
+
+// Residual update for one fitted galaxy: R -= model(e) - model(0).
+void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,
+                     const double *v, const double *w, const double *wn, const double *spec,
+                     const GalaxyGeom &fit, const GalaxyGeom &round, void *res);
+
+// Drop-in helpers on plain device arrays.
+void launch_model_at_zero(hipStream_t s, double e1, double e2, double scale, unsigned n,
+                          const double *uu, const double *vv, const double *s2, double *mod,
+                          double *partials);
+void launch_galaxy_vis(hipStream_t s, double spectra, double wavenumber, const GalaxyGeom &g,
+                       unsigned long n, const double *u, const double *v, const double *w,
+                       double *vis);
+void launch_phase_shift(hipStream_t s, double wavenumber, double l, double m, unsigned long n,
+                        const double *u, const double *v, const double *w, double *vis);
+void launch_marginalise_r(hipStream_t s, int num, const double *x, const double *y,
+                          double *out, double *qag_ws, int *qag_iws);
+
+// Row permutation / packing at observation set-up.
+void launch_gather_rows(hipStream_t s, unsigned long nrow, const uint32_t *perm,
+                        const double *src, double *dst);
+void launch_pack_sigma(hipStream_t s, unsigned nch, unsigned long nrow, const uint32_t *perm,
+                       const float *sigma2, const unsigned char *flag, float *s2f);
+void launch_residual_in(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                        const uint32_t *perm, const double *D, const double *S, void *res);
+void launch_residual_out(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                         const uint32_t *perm, const void *res, double *out);
+void launch_morton_keys(hipStream_t s, unsigned long nrow, const double *u, const double *v,
+                        double umax, uint32_t *keys, uint32_t *idx);
+void sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
+               uint32_t *keys_alt, uint32_t *idx_alt, void *tmp, size_t tmp_bytes,
+               uint32_t **sorted_idx);
+size_t sort_rows_tmp_bytes(unsigned long nrow);
+
+// Host-side error reporting shared by the ABI layer.
+void set_error(const std::string &msg);
+bool hip_ok(hipError_t e, const char *what);
+
+}  // namespace rl2

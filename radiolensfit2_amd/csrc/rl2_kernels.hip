@@ -1,0 +1,1127 @@
+// rl2_kernels.hip -- gfx950 kernels of the per-galaxy shape-measurement path.
+//
+// Residual layout: channel-major R[c * nrow + i] over rows i in the observation's
+// spatially sorted order (Morton order of (u, v)), complex128 (RL2_PREC_FP64) or
+// complex64 (RL2_PREC_MIXED).  sigma^2 is one float per visibility with the sign bit
+// marking a flagged visibility (sigma^2 > 0 after defaulting, RadioLensfit2.cpp:266-269).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "rl2_internal.h"
+#include "rl2_gk41.h"
+
+namespace rl2 {
+
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+// ------------------------------------------------------------------ small helpers
+__device__ inline double2 load_res(const void *res, int prec, size_t k)
+{
+    if (prec == 0) return reinterpret_cast<const double2 *>(res)[k];
+    float2 f = reinterpret_cast<const float2 *>(res)[k];
+    return make_double2((double)f.x, (double)f.y);
+}
+__device__ inline void store_res(void *res, int prec, size_t k, double2 v)
+{
+    if (prec == 0)
+        reinterpret_cast<double2 *>(res)[k] = v;
+    else
+        reinterpret_cast<float2 *>(res)[k] = make_float2((float)v.x, (float)v.y);
+}
+
+// Round-model / fitted-model visibility of one galaxy in one channel at one row
+// (galaxy_visibilities.c:154-190), returned with its phase rotor.
+__device__ inline double model_shape(const GalaxyGeom &g, double u, double v, double wn2,
+                                     double spectra)
+{
+    double k1 = (1. + g.e1) * u + g.e2 * v;
+    double k2 = g.e2 * u + (1. - g.e1) * v;
+    double shape = sersic_shape(k1, k2, g.sf, wn2);
+    return shape * (spectra * g.flux);
+}
+
+// ------------------------------------------------------------------ sources / noise
+// R += sign * sum_g flux_g spec_c shape_g exp(i k_c (u l + v m + w n)).
+// One thread per row; galaxies staged through LDS in tiles of 128.
+template <int PREC>
+__global__ __launch_bounds__(256) void add_sources_kernel(unsigned nch, unsigned long nrow,
+                                                          const double *__restrict__ u,
+                                                          const double *__restrict__ v,
+                                                          const double *__restrict__ w,
+                                                          const double *__restrict__ wn,
+                                                          const double *__restrict__ spec,
+                                                          const GalaxyGeom *__restrict__ gal,
+                                                          unsigned ngal, double sign, void *res)
+{
+    constexpr int TILE = 128;
+    constexpr int CB = 8;  // channels per register block
+    __shared__ GalaxyGeom sg[TILE];
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    bool ok = i < nrow;
+    double ui = ok ? u[i] : 0., vi = ok ? v[i] : 0., wi = ok ? w[i] : 0.;
+    for (unsigned c0 = 0; c0 < nch; c0 += CB) {
+        double are[CB], aim[CB], wnc[CB], wn2[CB], spc[CB];
+        int nc = min((int)(nch - c0), CB);
+#pragma unroll
+        for (int k = 0; k < CB; k++) {
+            are[k] = 0.;
+            aim[k] = 0.;
+            wnc[k] = (k < nc) ? wn[c0 + k] : 0.;
+            wn2[k] = wnc[k] * wnc[k];
+            spc[k] = (k < nc) ? spec[c0 + k] : 0.;
+        }
+        for (unsigned g0 = 0; g0 < ngal; g0 += TILE) {
+            __syncthreads();
+            for (unsigned t = threadIdx.x; t < TILE && g0 + t < ngal; t += blockDim.x)
+                sg[t] = gal[g0 + t];
+            __syncthreads();
+            unsigned gn = min((unsigned)TILE, ngal - g0);
+            for (unsigned t = 0; t < gn; t++) {
+                const GalaxyGeom g = sg[t];
+                double p = ui * g.l0 + vi * g.m0 + wi * g.n0;
+#pragma unroll
+                for (int k = 0; k < CB; k++) {
+                    if (k < nc) {
+                        double phase = wnc[k] * p;
+                        double shape = model_shape(g, ui, vi, wn2[k], spc[k]);
+                        double sn, cs;
+                        sincos(phase, &sn, &cs);
+                        are[k] += shape * cs;
+                        aim[k] += shape * sn;
+                    }
+                }
+            }
+        }
+        if (ok) {
+            for (int k = 0; k < nc; k++) {
+                size_t idx = (size_t)(c0 + k) * nrow + i;
+                double2 r = load_res(res, PREC, idx);
+                r.x += sign * are[k];
+                r.y += sign * aim[k];
+                store_res(res, PREC, idx, r);
+            }
+        }
+    }
+}
+
+void launch_add_sources(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                        const double *u, const double *v, const double *w, const double *wn,
+                        const double *spec, const GalaxyGeom *gal, unsigned ngal, double sign,
+                        void *res)
+{
+    dim3 grid((unsigned)((nrow + 255) / 256));
+    if (prec == 0)
+        add_sources_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, gal, ngal,
+                                                    sign, res);
+    else
+        add_sources_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, gal, ngal,
+                                                    sign, res);
+}
+
+// Philox4x32-10 counter-based generator (Salmon et al., SC'11) -> Box-Muller.
+__device__ inline uint4 philox4x32(uint4 ctr, uint2 key)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
+        uint32_t hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
+        ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+        key.x += W0;
+        key.y += W1;
+    }
+    return ctr;
+}
+
+template <int PREC>
+__global__ void add_noise_kernel(unsigned long nvis, double sigma, uint64_t seed, void *res)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nvis) return;
+    uint4 r = philox4x32(make_uint4((uint32_t)i, (uint32_t)(i >> 32), 0x5eed, 0),
+                         make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+    // 53-bit uniforms in (0, 1]
+    double u1 = ((double)(((uint64_t)r.x << 21) ^ (r.y >> 11)) + 1.0) * (1.0 / 9007199254740992.0);
+    double u2 = ((double)(((uint64_t)r.z << 21) ^ (r.w >> 11)) + 0.5) * (1.0 / 9007199254740992.0);
+    double rad = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    double2 x = load_res(res, PREC, i);
+    x.x += sigma * rad * cs;
+    x.y += sigma * rad * sn;
+    store_res(res, PREC, i, x);
+}
+
+void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma, uint64_t seed,
+                      void *res)
+{
+    dim3 grid((unsigned)((nvis + 255) / 256));
+    if (prec == 0)
+        add_noise_kernel<0><<<grid, 256, 0, s>>>(nvis, sigma, seed, res);
+    else
+        add_noise_kernel<1><<<grid, 256, 0, s>>>(nvis, sigma, seed, res);
+}
+
+// ------------------------------------------------------------------ extraction + gridding
+// One work-group owns RPT*256 consecutive (sorted) rows.  Per channel each thread grids its
+// RPT visibilities into an LDS open-addressing table keyed by cell index (at most
+// RPT*256 distinct keys, table size 2*RPT*256, so probing always terminates), then the
+// table is flushed to the global accumulators with one atomic per distinct cell.  Sorted
+// rows make neighbouring threads hit the same cells, so hot cells near the uv origin
+// receive one global atomic per work-group instead of one per visibility.
+template <int MODE, int PREC, int RPT>
+__global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
+{
+    constexpr int BS = 256;
+    constexpr int HS = 2 * RPT * BS;
+    constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
+    __shared__ uint32_t hkey[HS];
+    __shared__ double hre[HS], him[HS], hs2[HS];
+    __shared__ uint32_t hcnt[HS];
+    const int tid = threadIdx.x;
+    for (int s = tid; s < HS; s += BS) {
+        hkey[s] = kEmpty;
+        hre[s] = 0.;
+        him[s] = 0.;
+        hs2[s] = 0.;
+        hcnt[s] = 0;
+    }
+    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS);
+    double ru[RPT], rv[RPT], rp[RPT];
+    bool rok[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        unsigned long i = row0 + tid + (unsigned long)k * BS;
+        rok[k] = i < a.nrow;
+        ru[k] = rok[k] ? a.u[i] : 0.;
+        rv[k] = rok[k] ? a.v[i] : 0.;
+        double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
+        rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
+    }
+    const unsigned long ncell = (unsigned long)a.facet * a.facet;
+    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
+           *acc_w = a.acc + 3 * ncell;
+    __syncthreads();
+    for (unsigned c = 0; c < a.nch; c++) {
+        const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
+        const double wn2 = wn * wn, spectra = a.spec[c];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!rok[k]) continue;
+            const size_t idx = (size_t)c * a.nrow + row0 + tid + (size_t)k * BS;
+            const float s2 = a.s2f[idx];
+            if (signbit(s2)) continue;  // flagged
+            uint32_t key;
+            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key)) continue;
+            double xr = 0., xi = 0.;
+            if (MODE == 0) {
+                // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
+                // (source_extraction.c:51-65).
+                double phase = wn * rp[k];
+                double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
+                double sn, cs;
+                sincos(phase, &sn, &cs);
+                double2 r = load_res(a.res, PREC, idx);
+                double gre = shape * cs + r.x;
+                double gim = shape * sn + r.y;
+                xr = gre * cs - gim * (-sn);
+                xi = gre * (-sn) + gim * cs;
+            } else if (MODE == 1) {
+                double2 r = load_res(a.res, PREC, idx);
+                xr = r.x;
+                xi = r.y;
+            }
+            uint32_t h = (key * 2654435761u) >> (32 - HBITS);
+            while (true) {
+                uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                if (prev == kEmpty || prev == key) break;
+                h = (h + 1) & (HS - 1);
+            }
+            if (MODE != 2) {
+                atomicAdd(&hre[h], xr);
+                atomicAdd(&him[h], xi);
+            }
+            atomicAdd(&hs2[h], (double)s2);
+            atomicAdd(&hcnt[h], 1u);
+        }
+        __syncthreads();
+        for (int s = tid; s < HS; s += BS) {
+            uint32_t key = hkey[s];
+            if (key != kEmpty) {
+                if (MODE != 2) {
+                    unsafeAtomicAdd(&acc_re[key], hre[s]);
+                    unsafeAtomicAdd(&acc_im[key], him[s]);
+                }
+                unsafeAtomicAdd(&acc_s2[key], hs2[s]);
+                unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+                hkey[s] = kEmpty;
+                hre[s] = 0.;
+                him[s] = 0.;
+                hs2[s] = 0.;
+                hcnt[s] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+void launch_grid(hipStream_t s, const GridLaunch &a)
+{
+    constexpr int RPT = 4;
+    dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+    if (a.mode == 0) {
+        if (a.prec == 0)
+            grid_kernel<0, 0, RPT><<<grid, 256, 0, s>>>(a);
+        else
+            grid_kernel<0, 1, RPT><<<grid, 256, 0, s>>>(a);
+    } else if (a.mode == 1) {
+        grid_kernel<1, 0, RPT><<<grid, 256, 0, s>>>(a);
+    } else {
+        grid_kernel<2, 0, RPT><<<grid, 256, 0, s>>>(a);
+    }
+}
+
+// ------------------------------------------------------------------ compaction
+// Cells with weight != 0 in row-major order (evaluate_uv_grid.c:196-210 and :113-133):
+//   vis = sum_vis * w / sum_sigma2, sigma2 = sum_sigma2 / w^2, (u,v) = cell centre.
+constexpr int kCompactBS = 256, kCompactPT = 16, kCompactTile = kCompactBS * kCompactPT;
+
+size_t compact_scratch_words(unsigned facet)
+{
+    size_t ncell = (size_t)facet * facet;
+    return (ncell + kCompactTile - 1) / kCompactTile + 1;
+}
+
+__global__ __launch_bounds__(kCompactBS) void compact_count_kernel(const double *acc_w,
+                                                                   unsigned long ncell,
+                                                                   unsigned *bcount)
+{
+    __shared__ unsigned part[kCompactBS / 64];
+    unsigned long base = (unsigned long)blockIdx.x * kCompactTile + threadIdx.x * kCompactPT;
+    unsigned cnt = 0;
+    for (int k = 0; k < kCompactPT; k++) {
+        unsigned long p = base + k;
+        if (p < ncell && acc_w[p] != 0.) cnt++;
+    }
+    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = 0;
+        for (int i = 0; i < kCompactBS / 64; i++) t += part[i];
+        bcount[blockIdx.x] = t;
+    }
+}
+
+// exclusive scan of nb block counts in place (single work-group), total -> bcount[nb]
+__global__ __launch_bounds__(1024) void compact_scan_kernel(unsigned *bcount, unsigned nb,
+                                                            unsigned *n_out)
+{
+    __shared__ unsigned buf[1024];
+    __shared__ unsigned carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (unsigned base = 0; base < nb; base += 1024) {
+        unsigned i = base + threadIdx.x;
+        unsigned x = (i < nb) ? bcount[i] : 0;
+        buf[threadIdx.x] = x;
+        __syncthreads();
+        for (unsigned off = 1; off < 1024; off <<= 1) {
+            unsigned t = (threadIdx.x >= off) ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bcount[i] = carry + buf[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bcount[nb] = carry;
+        *n_out = carry;
+    }
+}
+
+__global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
+    double *acc, unsigned facet, unsigned long ncell, double len, const unsigned *boff,
+    double *uu, double *vv, double *dre, double *dim, double *s2o, double *sum_w_dense)
+{
+    __shared__ unsigned tscan[kCompactBS];
+    double *acc_re = acc, *acc_im = acc + ncell, *acc_s2 = acc + 2 * ncell, *acc_w = acc + 3 * ncell;
+    unsigned long base = (unsigned long)blockIdx.x * kCompactTile + threadIdx.x * kCompactPT;
+    unsigned cnt = 0;
+    for (int k = 0; k < kCompactPT; k++) {
+        unsigned long p = base + k;
+        if (p < ncell && acc_w[p] != 0.) cnt++;
+    }
+    tscan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < kCompactBS; off <<= 1) {
+        unsigned t = (threadIdx.x >= (unsigned)off) ? tscan[threadIdx.x - off] : 0;
+        __syncthreads();
+        tscan[threadIdx.x] += t;
+        __syncthreads();
+    }
+    unsigned n = boff[blockIdx.x] + tscan[threadIdx.x] - cnt;
+    const double inc = 2 * len / facet;
+    for (int k = 0; k < kCompactPT; k++) {
+        unsigned long p = base + k;
+        if (p >= ncell) break;
+        double wgt = acc_w[p];
+        if (sum_w_dense) sum_w_dense[p] = wgt;
+        if (wgt != 0.) {
+            double re = acc_re[p], im = acc_im[p], sg = acc_s2[p];
+            dre[n] = re * wgt / sg;
+            dim[n] = im * wgt / sg;
+            s2o[n] = sg / (wgt * wgt);
+            unsigned j = (unsigned)(p / facet), i = (unsigned)(p % facet);
+            uu[n] = (i + 0.5) * inc - len;
+            vv[n] = (j + 0.5) * inc - len;
+            n++;
+            acc_re[p] = 0.;
+            acc_im[p] = 0.;
+            acc_s2[p] = 0.;
+            acc_w[p] = 0.;
+        }
+    }
+}
+
+void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
+                    double *vv, double *dre, double *dim, double *s2, unsigned *n_out,
+                    double *sum_w_dense, unsigned *scratch)
+{
+    unsigned long ncell = (unsigned long)facet * facet;
+    unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    compact_count_kernel<<<nb, kCompactBS, 0, s>>>(acc + 3 * ncell, ncell, scratch);
+    compact_scan_kernel<<<1, 1024, 0, s>>>(scratch, nb, n_out);
+    compact_write_kernel<<<nb, kCompactBS, 0, s>>>(acc, facet, ncell, len, scratch, uu, vv, dre,
+                                                   dim, s2, sum_w_dense);
+}
+
+// ------------------------------------------------------------------ likelihood (K3)
+// One work-group per (item, scalelength): the exponential-disk model at zero
+// (galaxy_visibilities.c:59-91), the cross-correlation sums and the Newton search of its
+// peak (likelihood.c:195-358), and the position marginalisation (likelihood.c:157-187,
+// 376-421).  The model normalisation 1/sqrt(sum M^2/sigma^2) is applied at the end: the
+// Newton increments are ratios of sums of equal degree in M, so ho = h'/sqrt(S2) and
+// det = det'/S2 with h', det' computed from the unnormalised model.
+template <int BS, int N>
+__device__ inline void block_allreduce(double (&v)[N], double *lds)
+{
+#pragma unroll
+    for (int j = 0; j < N; j++)
+        for (int off = 32; off >= 1; off >>= 1) v[j] += __shfl_xor(v[j], off);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = BS / 64;
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < N; j++) lds[wave * N + j] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        double t = 0.;
+        for (int q = 0; q < NW; q++) t += lds[q * N + j];
+        v[j] = t;
+    }
+    __syncthreads();
+}
+
+template <int BS, int PPT>
+__global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
+{
+    __shared__ double red[(BS / 64) * 8];
+    const int nr = a.rg.numr - 1;
+    const int item = blockIdx.x / nr;
+    const int r = blockIdx.x % nr + 1;
+    const LikItem it = a.items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned n = F.n;
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(a.rg.ro[r]);
+    const double sf = (scale * scale) / (detA * detA);
+    const double cc2 = 4 * kPi * kPi;
+    const double a1 = 1. + e1, a2 = 1. - e1;
+    const double *__restrict__ gu = F.uu;
+    const double *__restrict__ gv = F.vv;
+    const double *__restrict__ dre = F.dre;
+    const double *__restrict__ dim = F.dim;
+    const double *__restrict__ gs2 = F.s2;
+    const double *__restrict__ gmod = F.mod;
+    const int tid = threadIdx.x;
+
+    // Point terms of the first PPT*BS cells stay in registers (static indices so the
+    // cache is not demoted to scratch memory); cells beyond are recomputed per pass.
+    double hrc[PPT > 0 ? PPT : 1], hic[PPT > 0 ? PPT : 1];
+    double s[7] = {0., 0., 0., 0., 0., 0., 0.};  // S2, value, dx, dy, dxx, dyy, dxy
+    auto model_at = [&](unsigned i, double uu, double vv) -> double {
+        if (gmod) return gmod[i];
+        const double k1 = a1 * uu + e2 * vv;
+        const double k2 = e2 * uu + a2 * vv;
+        return sersic_shape(k1, k2, sf, cc2);
+    };
+    auto pass0 = [&](unsigned i, double &hr, double &hi) {
+        const double uu = gu[i], vv = gv[i];
+        const double M = model_at(i, uu, vv);
+        s[0] += M * M / gs2[i];
+        hr = dre[i] * M;
+        hi = -dim[i] * M;
+        const double U = kTwoPi * uu, V = kTwoPi * vv;
+        s[1] += hr;
+        s[2] -= hi * U;
+        s[3] -= hi * V;
+        s[4] -= hr * U * U;
+        s[5] -= hr * V * V;
+        s[6] -= hr * U * V;
+    };
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const unsigned i = tid + k * BS;
+        hrc[k] = 0.;
+        hic[k] = 0.;
+        if (i < n) pass0(i, hrc[k], hic[k]);
+    }
+    for (unsigned i = tid + PPT * BS; i < n; i += BS) {
+        double hr, hi;
+        pass0(i, hr, hi);
+    }
+    block_allreduce<BS, 7>(s, red);
+    const double S2 = gmod ? 1.0 : s[0];
+    double value = s[1], dx = s[2], dy = s[3], dxx = s[4], dyy = s[5], dxy = s[6];
+    double det = dxx * dyy - dxy * dxy;
+    double xo = 0., yo = 0., norm = 1.;
+    int iter = 0;
+    while (det > 0 && norm > 1.e-7 && iter < 50) {
+        iter++;
+        const double incx = (dx * dyy - dy * dxy) / det;
+        const double incy = (dy * dxx - dx * dxy) / det;
+        xo -= incx;
+        yo -= incy;
+        double t[6] = {0., 0., 0., 0., 0., 0.};
+        auto newton_term = [&](unsigned i, double hr, double hi) {
+            const double U = kTwoPi * gu[i], V = kTwoPi * gv[i];
+            const double ph = xo * U + yo * V;
+            double sn, cs;
+            sincos(ph, &sn, &cs);
+            const double re = hr * cs - hi * sn;
+            const double im = hr * sn + hi * cs;
+            t[0] += re;
+            t[1] -= im * U;
+            t[2] -= im * V;
+            t[3] -= re * U * U;
+            t[4] -= re * V * V;
+            t[5] -= re * U * V;
+        };
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const unsigned i = tid + k * BS;
+            if (i < n) newton_term(i, hrc[k], hic[k]);
+        }
+        for (unsigned i = tid + PPT * BS; i < n; i += BS) {
+            const double M = model_at(i, gu[i], gv[i]);
+            newton_term(i, dre[i] * M, -dim[i] * M);
+        }
+        block_allreduce<BS, 6>(t, red);
+        value = t[0];
+        dx = t[1];
+        dy = t[2];
+        dxx = t[3];
+        dyy = t[4];
+        dxy = t[5];
+        det = dxx * dyy - dxy * dxy;
+        norm = sqrt(incx * incx + incy * incy);
+    }
+    if (tid == 0) {
+        const double ho = gmod ? value : value / sqrt(S2);
+        const double det_sigma = gmod ? det : det / S2;
+        double L;
+        if (det_sigma <= 0)
+            L = kBad;
+        else {
+            const double B = (ho * ho) * 0.5;
+            L = marg_position(B);
+            if (isnan(L))
+                L = kBad;
+            else
+                L -= 0.5 * log(det_sigma);
+        }
+        a.Lr[(size_t)item * kMaxR + r] = L;
+        a.nit[(size_t)item * kMaxR + r] = iter;
+        if (a.ho) a.ho[(size_t)item * kMaxR + r] = ho;
+        if (a.det) a.det[(size_t)item * kMaxR + r] = det_sigma;
+    }
+}
+
+void launch_likelihood(hipStream_t s, const LikLaunch &a)
+{
+    if (a.nitems <= 0) return;
+    dim3 grid((unsigned)(a.nitems * (a.rg.numr - 1)));
+    switch (a.block) {
+    // PPT: register-cached cells per thread; larger caches spill FP64 sincos state.
+    case 64: likelihood_kernel<64, 4><<<grid, 64, 0, s>>>(a); break;
+    case 128: likelihood_kernel<128, 4><<<grid, 128, 0, s>>>(a); break;
+    case 512: likelihood_kernel<512, 0><<<grid, 512, 0, s>>>(a); break;
+    case 1024: likelihood_kernel<1024, 0><<<grid, 1024, 0, s>>>(a); break;
+    default: likelihood_kernel<256, 4><<<grid, 256, 0, s>>>(a); break;
+    }
+}
+
+// ------------------------------------------------------------------ r-marginalisation (K4)
+// One thread per item: set_posterior_values (marginalise_r.c:29-64), then the spline +
+// QAG-GK41 integral of marginalise_posterior_r (marginalise_r.c:81-223).
+struct QagWs {
+    double *alist, *blist, *rlist, *elist;
+    int *order, *level;
+};
+
+__device__ inline double rescale_error(double err, double result_abs, double result_asc)
+{
+    err = fabs(err);
+    if (result_asc != 0 && err != 0) {
+        double scale = pow((200 * err / result_asc), 1.5);
+        err = (scale < 1) ? result_asc * scale : result_asc;
+    }
+    if (result_abs > kDblMin / (50 * kDblEps)) {
+        double min_err = 50 * kDblEps * result_abs;
+        if (min_err > err) err = min_err;
+    }
+    return err;
+}
+
+__device__ inline void qk41_spline(const Spline &sp, double a, double b, double *result,
+                                   double *abserr, double *resabs, double *resasc)
+{
+    const int n = 21;
+    double fv1[21], fv2[21];
+    const double center = 0.5 * (a + b);
+    const double half_length = 0.5 * (b - a);
+    const double abs_half_length = fabs(half_length);
+    const double f_center = spline_exp(sp, center);
+    double result_gauss = 0;
+    double result_kronrod = f_center * kWgk41[n - 1];
+    double result_abs = fabs(result_kronrod);
+    for (int j = 0; j < (n - 1) / 2; j++) {
+        const int jtw = j * 2 + 1;
+        const double abscissa = half_length * kXgk41[jtw];
+        const double f1 = spline_exp(sp, center - abscissa);
+        const double f2 = spline_exp(sp, center + abscissa);
+        const double fsum = f1 + f2;
+        fv1[jtw] = f1;
+        fv2[jtw] = f2;
+        result_gauss += kWg41[j] * fsum;
+        result_kronrod += kWgk41[jtw] * fsum;
+        result_abs += kWgk41[jtw] * (fabs(f1) + fabs(f2));
+    }
+    for (int j = 0; j < n / 2; j++) {
+        const int jtwm1 = j * 2;
+        const double abscissa = half_length * kXgk41[jtwm1];
+        const double f1 = spline_exp(sp, center - abscissa);
+        const double f2 = spline_exp(sp, center + abscissa);
+        fv1[jtwm1] = f1;
+        fv2[jtwm1] = f2;
+        result_kronrod += kWgk41[jtwm1] * (f1 + f2);
+        result_abs += kWgk41[jtwm1] * (fabs(f1) + fabs(f2));
+    }
+    const double mean = result_kronrod * 0.5;
+    double result_asc = kWgk41[n - 1] * fabs(f_center - mean);
+    for (int j = 0; j < n - 1; j++)
+        result_asc += kWgk41[j] * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+    const double err = (result_kronrod - result_gauss) * half_length;
+    result_kronrod *= half_length;
+    result_abs *= abs_half_length;
+    result_asc *= abs_half_length;
+    *result = result_kronrod;
+    *resabs = result_abs;
+    *resasc = result_asc;
+    *abserr = rescale_error(err, result_abs, result_asc);
+}
+
+// QUADPACK qpsrt: keep the error list descending
+__device__ inline void qpsrt(QagWs &w, int size, int limit, int &nrmax, int &imax)
+{
+    const int last = size - 1;
+    int i_nrmax = nrmax;
+    int i_maxerr = w.order[i_nrmax];
+    if (last < 2) {
+        w.order[0] = 0;
+        w.order[1] = 1;
+        imax = i_maxerr;
+        return;
+    }
+    const double errmax = w.elist[i_maxerr];
+    while (i_nrmax > 0 && errmax > w.elist[w.order[i_nrmax - 1]]) {
+        w.order[i_nrmax] = w.order[i_nrmax - 1];
+        i_nrmax--;
+    }
+    int top = (last < (limit / 2 + 2)) ? last : limit - last + 1;
+    int i = i_nrmax + 1;
+    while (i < top && errmax < w.elist[w.order[i]]) {
+        w.order[i - 1] = w.order[i];
+        i++;
+    }
+    w.order[i - 1] = i_maxerr;
+    const double errmin = w.elist[last];
+    int k = top - 1;
+    while (k > i - 2 && errmin >= w.elist[w.order[k]]) {
+        w.order[k + 1] = w.order[k];
+        k--;
+    }
+    w.order[k + 1] = last;
+    imax = w.order[i_nrmax];
+    nrmax = i_nrmax;
+}
+
+// gsl_integration_qag semantics (GSL 2.5 qag.c) with the 41-point rule; 0 = success.
+__device__ int qag41_spline(const Spline &sp, double a, double b, double epsrel, int limit,
+                            QagWs &w, double *result)
+{
+    double result0, abserr0, resabs0, resasc0;
+    int roundoff_type1 = 0, roundoff_type2 = 0, error_type = 0;
+    w.alist[0] = a;
+    w.blist[0] = b;
+    w.rlist[0] = 0.0;
+    w.elist[0] = 0.0;
+    w.order[0] = 0;
+    w.level[0] = 0;
+    int size = 0, nrmax = 0, imax = 0;
+    *result = 0;
+    qk41_spline(sp, a, b, &result0, &abserr0, &resabs0, &resasc0);
+    size = 1;
+    w.rlist[0] = result0;
+    w.elist[0] = abserr0;
+    double tolerance = fmax(0., epsrel * fabs(result0));
+    const double round_off = 50 * kDblEps * resabs0;
+    if (abserr0 <= round_off && abserr0 > tolerance) {
+        *result = result0;
+        return 14;
+    } else if ((abserr0 <= tolerance && abserr0 != resasc0) || abserr0 == 0.0) {
+        *result = result0;
+        return 0;
+    } else if (limit == 1) {
+        *result = result0;
+        return 11;
+    }
+    double area = result0, errsum = abserr0;
+    int iteration = 1;
+    do {
+        const double a_i = w.alist[imax], b_i = w.blist[imax];
+        const double r_i = w.rlist[imax], e_i = w.elist[imax];
+        const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
+        double area1, area2, error1, error2, resabs1, resabs2, resasc1, resasc2;
+        qk41_spline(sp, a1, b1, &area1, &error1, &resabs1, &resasc1);
+        qk41_spline(sp, a2, b2, &area2, &error2, &resabs2, &resasc2);
+        const double area12 = area1 + area2;
+        const double error12 = error1 + error2;
+        errsum += (error12 - e_i);
+        area += area12 - r_i;
+        if (resasc1 != error1 && resasc2 != error2) {
+            const double delta = r_i - area12;
+            if (fabs(delta) <= 1.0e-5 * fabs(area12) && error12 >= 0.99 * e_i) roundoff_type1++;
+            if (iteration >= 10 && error12 > e_i) roundoff_type2++;
+        }
+        tolerance = fmax(0., epsrel * fabs(area));
+        if (errsum > tolerance) {
+            if (roundoff_type1 >= 6 || roundoff_type2 >= 20) error_type = 2;
+            const double tmp = (1 + 100 * kDblEps) * (fabs(a2) + 1000 * kDblMin);
+            if (fabs(a1) <= tmp && fabs(b2) <= tmp) error_type = 3;
+        }
+        // update(): bisected halves, larger error first in place
+        const int i_max = imax, i_new = size;
+        const int new_level = w.level[i_max] + 1;
+        if (error2 > error1) {
+            w.alist[i_max] = a2;
+            w.rlist[i_max] = area2;
+            w.elist[i_max] = error2;
+            w.level[i_max] = new_level;
+            w.alist[i_new] = a1;
+            w.blist[i_new] = b1;
+            w.rlist[i_new] = area1;
+            w.elist[i_new] = error1;
+            w.level[i_new] = new_level;
+        } else {
+            w.blist[i_max] = b1;
+            w.rlist[i_max] = area1;
+            w.elist[i_max] = error1;
+            w.level[i_max] = new_level;
+            w.alist[i_new] = a2;
+            w.blist[i_new] = b2;
+            w.rlist[i_new] = area2;
+            w.elist[i_new] = error2;
+            w.level[i_new] = new_level;
+        }
+        size++;
+        qpsrt(w, size, limit, nrmax, imax);
+        iteration++;
+    } while (iteration < limit && !error_type && errsum > tolerance);
+    double sum = 0;
+    for (int k = 0; k < size; k++) sum += w.rlist[k];
+    *result = sum;
+    if (errsum <= tolerance) return 0;
+    if (error_type == 2) return 14;
+    if (error_type == 3) return 21;
+    if (iteration == limit) return 11;
+    return 5;
+}
+
+// marginalise_r.c:81-223 on (num, x, y); y is modified (y -= ymax).
+__device__ double marginalise_posterior(int num, const double *x, double *y, QagWs &w)
+{
+    double ymax = -10.0;
+    int ymaxpos = 0;
+    for (int ii = 0; ii < num; ii++)
+        if (y[ii] > ymax) {
+            ymax = y[ii];
+            ymaxpos = ii;
+        }
+    double xmin = x[ymaxpos], xmax = x[ymaxpos];
+    if (ymaxpos < num - 1) xmax = x[ymaxpos + 1];
+    if (ymaxpos > 0) xmin = x[ymaxpos - 1];
+    for (int ii = 0; ii < num; ii++) {
+        if (x[ii] < xmin) xmin = x[ii];
+        if (x[ii] > xmax) xmax = x[ii];
+        y[ii] -= ymax;
+    }
+    if (xmax <= xmin) return NAN;
+    Spline sp;
+    double sump = 0.;
+    if (spline_init(sp, x, y, num, 0)) return NAN;
+    int st = qag41_spline(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
+    if (st) {
+        spline_init(sp, x, y, num, 1);
+        st = qag41_spline(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
+    }
+    if (st) return kBad;
+    return log(sump) + ymax;
+}
+
+__global__ void rmarg_kernel(LikLaunch a)
+{
+    const int item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= a.nitems) return;
+    const LikItem it = a.items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const double *L_r = a.Lr + (size_t)item * kMaxR;
+    double x[kMaxR], y[kMaxR];
+    int num = 0;
+    int nRo = 1;
+    const int numR = a.rg.numr;
+    while (nRo < numR && (L_r[nRo] <= kBad)) nRo++;
+    if (nRo < numR) {
+        for (nRo = 1; nRo < numR; nRo++) {
+            const double lv = L_r[nRo];
+            if (lv > kBad) {
+                x[num] = a.rg.ro[nRo];
+                y[num] = lv + F.rprior[nRo];
+                num++;
+            }
+        }
+    }
+    int error = 0;
+    double L_e;
+    if (num > 4) {
+        QagWs w;
+        double *base = a.qag_ws + (size_t)item * kQagDoubles;
+        int *ibase = a.qag_iws + (size_t)item * kQagInts;
+        w.alist = base;
+        w.blist = base + kQagLimit;
+        w.rlist = base + 2 * kQagLimit;
+        w.elist = base + 3 * kQagLimit;
+        w.order = ibase;
+        w.level = ibase + kQagLimit;
+        L_e = marginalise_posterior(num, x, y, w);
+    } else {
+        error = 1;
+        L_e = kBad;
+    }
+    if (L_e == kBad) error = 1;
+    a.Le[item] = L_e;
+    a.err[item] = error;
+}
+
+void launch_rmarg(hipStream_t s, const LikLaunch &a)
+{
+    if (a.nitems <= 0) return;
+    rmarg_kernel<<<(a.nitems + 63) / 64, 64, 0, s>>>(a);
+}
+
+__global__ void marginalise_r_kernel(int num, const double *x, const double *y, double *out,
+                                     double *ws, int *iws)
+{
+    double xx[kMaxR], yy[kMaxR];
+    for (int i = 0; i < num; i++) {
+        xx[i] = x[i];
+        yy[i] = y[i];
+    }
+    QagWs w{ws, ws + kQagLimit, ws + 2 * kQagLimit, ws + 3 * kQagLimit, iws, iws + kQagLimit};
+    out[0] = marginalise_posterior(num, xx, yy, w);
+    for (int i = 0; i < num; i++) out[1 + i] = yy[i];
+}
+
+void launch_marginalise_r(hipStream_t s, int num, const double *x, const double *y,
+                          double *out, double *qag_ws, int *qag_iws)
+{
+    marginalise_r_kernel<<<1, 1, 0, s>>>(num, x, y, out, qag_ws, qag_iws);
+}
+
+// ------------------------------------------------------------------ subtraction (K2)
+// data_processing.c:109-135: visSkyMod -= round model, visData -= fitted model, i.e.
+// R = (R - fit) + round with one phase rotor per visibility.
+template <int PREC>
+__global__ __launch_bounds__(256) void subtract_kernel(unsigned nch, unsigned long nrow,
+                                                       const double *__restrict__ u,
+                                                       const double *__restrict__ v,
+                                                       const double *__restrict__ w,
+                                                       const double *__restrict__ wn,
+                                                       const double *__restrict__ spec,
+                                                       GalaxyGeom fit, GalaxyGeom rnd, void *res)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    const double ui = u[i], vi = v[i], wi = w[i];
+    const double p = ui * fit.l0 + vi * fit.m0 + wi * fit.n0;
+    for (unsigned c = 0; c < nch; c++) {
+        const double k = wn[c], k2 = k * k, spectra = spec[c];
+        const double phase = k * p;
+        double sn, cs;
+        sincos(phase, &sn, &cs);
+        const double sr = model_shape(rnd, ui, vi, k2, spectra);
+        const double sfit = model_shape(fit, ui, vi, k2, spectra);
+        const size_t idx = (size_t)c * nrow + i;
+        double2 r = load_res(res, PREC, idx);
+        r.x = (r.x - sfit * cs) + sr * cs;
+        r.y = (r.y - sfit * sn) + sr * sn;
+        store_res(res, PREC, idx, r);
+    }
+}
+
+void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,
+                     const double *v, const double *w, const double *wn, const double *spec,
+                     const GalaxyGeom &fit, const GalaxyGeom &round, void *res)
+{
+    dim3 grid((unsigned)((nrow + 255) / 256));
+    if (prec == 0)
+        subtract_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, fit, round, res);
+    else
+        subtract_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, fit, round, res);
+}
+
+// ------------------------------------------------------------------ drop-in helpers
+// galaxy_visibilities.c:59-91 in two steps: model + per-block partial sums of M^2/sigma^2,
+// then normalisation by sqrt of the fixed-order total.
+__global__ __launch_bounds__(256) void model_zero_kernel(double e1, double e2, double sf, unsigned n,
+                                                         const double *uu, const double *vv,
+                                                         const double *s2, double *mod,
+                                                         double *partials)
+{
+    __shared__ double red[4];
+    unsigned i = blockIdx.x * 256 + threadIdx.x;
+    double t = 0.;
+    if (i < n) {
+        double k1 = (1. + e1) * uu[i] + e2 * vv[i];
+        double k2 = e2 * uu[i] + (1. - e1) * vv[i];
+        double M = sersic_shape(k1, k2, sf, 4 * kPi * kPi);
+        mod[i] = M;
+        t = M * M / s2[i];
+    }
+    double v[1] = {t};
+    block_allreduce<256, 1>(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v[0];
+}
+__global__ __launch_bounds__(256) void model_norm_kernel(unsigned n, unsigned nb,
+                                                         const double *partials, double *mod)
+{
+    __shared__ double tot;
+    if (threadIdx.x == 0) {
+        double s = 0.;
+        for (unsigned b = 0; b < nb; b++) s += partials[b];
+        tot = sqrt(s);
+    }
+    __syncthreads();
+    for (unsigned i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) mod[i] /= tot;
+}
+void launch_model_at_zero(hipStream_t s, double e1, double e2, double scale, unsigned n,
+                          const double *uu, const double *vv, const double *s2, double *mod,
+                          double *partials)
+{
+    double detA = 1. - e1 * e1 - e2 * e2;
+    double sc = arcs2rad_scale(scale);
+    double sf = (sc * sc) / (detA * detA);
+    unsigned nb = (n + 255) / 256;
+    if (nb == 0) return;
+    model_zero_kernel<<<nb, 256, 0, s>>>(e1, e2, sf, n, uu, vv, s2, mod, partials);
+    model_norm_kernel<<<min(nb, 1024u), 256, 0, s>>>(n, nb, partials, mod);
+}
+
+__global__ void galaxy_vis_kernel(double spectra, double wn, GalaxyGeom g, unsigned long n,
+                                  const double *u, const double *v, const double *w, double *vis)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double ui = u[i], vi = v[i], wi = w[i];
+    double phase = ui * g.l0 + vi * g.m0 + wi * g.n0;
+    phase = wn * phase;
+    double shape = model_shape(g, ui, vi, wn * wn, spectra);
+    double sn, cs;
+    sincos(phase, &sn, &cs);
+    vis[2 * i] = shape * cs;
+    vis[2 * i + 1] = shape * sn;
+}
+void launch_galaxy_vis(hipStream_t s, double spectra, double wavenumber, const GalaxyGeom &g,
+                       unsigned long n, const double *u, const double *v, const double *w,
+                       double *vis)
+{
+    if (!n) return;
+    galaxy_vis_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(spectra, wavenumber, g, n, u, v,
+                                                                  w, vis);
+}
+
+__global__ void phase_shift_kernel(double wn, double l, double m, double nn, unsigned long n,
+                                   const double *u, const double *v, const double *w, double *vis)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double phase = u[i] * l + v[i] * m + w[i] * nn;
+    phase *= -wn;
+    double sp, cp;
+    sincos(phase, &sp, &cp);
+    double re = vis[2 * i], im = vis[2 * i + 1];
+    vis[2 * i] = re * cp - im * sp;
+    vis[2 * i + 1] = re * sp + im * cp;
+}
+void launch_phase_shift(hipStream_t s, double wavenumber, double l, double m, unsigned long n,
+                        const double *u, const double *v, const double *w, double *vis)
+{
+    if (!n) return;
+    double nn = sqrt(1. - l * l - m * m) - 1.;
+    phase_shift_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(wavenumber, l, m, nn, n, u, v,
+                                                                   w, vis);
+}
+
+// ------------------------------------------------------------------ observation set-up
+__global__ void gather_rows_kernel(unsigned long nrow, const uint32_t *perm, const double *src,
+                                   double *dst)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i < nrow) dst[i] = src[perm[i]];
+}
+void launch_gather_rows(hipStream_t s, unsigned long nrow, const uint32_t *perm,
+                        const double *src, double *dst)
+{
+    gather_rows_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(nrow, perm, src, dst);
+}
+
+__global__ void pack_sigma_kernel(unsigned nch, unsigned long nrow, const uint32_t *perm,
+                                  const float *sigma2, const unsigned char *flag, float *s2f)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    unsigned long src = perm[i];
+    for (unsigned c = 0; c < nch; c++) {
+        float sg = sigma2[(size_t)c * nrow + src];
+        bool fl = flag && flag[(size_t)c * nrow + src];
+        s2f[(size_t)c * nrow + i] = fl ? -fabsf(sg) - 0.0f : sg;
+    }
+}
+void launch_pack_sigma(hipStream_t s, unsigned nch, unsigned long nrow, const uint32_t *perm,
+                       const float *sigma2, const unsigned char *flag, float *s2f)
+{
+    pack_sigma_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(nch, nrow, perm, sigma2, flag,
+                                                                     s2f);
+}
+
+template <int PREC>
+__global__ void residual_in_kernel(unsigned nch, unsigned long nrow, const uint32_t *perm,
+                                   const double *D, const double *S, void *res)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    unsigned long src = perm[i];
+    for (unsigned c = 0; c < nch; c++) {
+        size_t k = (size_t)c * nrow + src;
+        double re = D[2 * k], im = D[2 * k + 1];
+        if (S) {
+            re = re - S[2 * k];
+            im = im - S[2 * k + 1];
+        }
+        store_res(res, PREC, (size_t)c * nrow + i, make_double2(re, im));
+    }
+}
+void launch_residual_in(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                        const uint32_t *perm, const double *D, const double *S, void *res)
+{
+    dim3 grid((unsigned)((nrow + 255) / 256));
+    if (prec == 0)
+        residual_in_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, perm, D, S, res);
+    else
+        residual_in_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, perm, D, S, res);
+}
+
+template <int PREC>
+__global__ void residual_out_kernel(unsigned nch, unsigned long nrow, const uint32_t *perm,
+                                    const void *res, double *out)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    unsigned long dst = perm[i];
+    for (unsigned c = 0; c < nch; c++) {
+        double2 r = load_res(res, PREC, (size_t)c * nrow + i);
+        size_t k = (size_t)c * nrow + dst;
+        out[2 * k] = r.x;
+        out[2 * k + 1] = r.y;
+    }
+}
+void launch_residual_out(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                         const uint32_t *perm, const void *res, double *out)
+{
+    dim3 grid((unsigned)((nrow + 255) / 256));
+    if (prec == 0)
+        residual_out_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, perm, res, out);
+    else
+        residual_out_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, perm, res, out);
+}
+
+// Morton (Z-order) key of (u, v) quantised to 16 bits per axis over [-umax, umax].
+__device__ inline uint32_t spread16(uint32_t x)
+{
+    x &= 0xffffu;
+    x = (x | (x << 8)) & 0x00ff00ffu;
+    x = (x | (x << 4)) & 0x0f0f0f0fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+__global__ void morton_kernel(unsigned long nrow, const double *u, const double *v, double umax,
+                              uint32_t *keys, uint32_t *idx)
+{
+    unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    double s = 65535.0 / (2.0 * umax);
+    double qu = fmin(fmax((u[i] + umax) * s, 0.0), 65535.0);
+    double qv = fmin(fmax((v[i] + umax) * s, 0.0), 65535.0);
+    keys[i] = spread16((uint32_t)qu) | (spread16((uint32_t)qv) << 1);
+    idx[i] = (uint32_t)i;
+}
+void launch_morton_keys(hipStream_t s, unsigned long nrow, const double *u, const double *v,
+                        double umax, uint32_t *keys, uint32_t *idx)
+{
+    morton_kernel<<<(unsigned)((nrow + 255) / 256), 256, 0, s>>>(nrow, u, v, umax, keys, idx);
+}
+
+size_t sort_rows_tmp_bytes(unsigned long nrow)
+{
+    size_t bytes = 0;
+    hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
+    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)nrow);
+    return bytes;
+}
+
+void sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
+               uint32_t *keys_alt, uint32_t *idx_alt, void *tmp, size_t tmp_bytes,
+               uint32_t **sorted_idx)
+{
+    hipcub::DoubleBuffer<uint32_t> k(keys, keys_alt), v(idx, idx_alt);
+    hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k, v, (int)nrow, 0, 32, s);
+    *sorted_idx = v.Current();
+}
+
+}  // namespace rl2

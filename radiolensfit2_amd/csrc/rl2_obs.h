@@ -1,0 +1,106 @@
+// rl2_obs.h -- the device-resident observation behind the rl2_obs handle (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <vector>
+
+#include "rl2.h"
+#include "rl2_internal.h"
+
+namespace rl2 {
+
+// One extraction target: dense accumulators and the compacted facet (the reference's
+// sum_w / facet_visData / facet_sigma2 / facet_u / facet_v buffers,
+// RadioLensfit2.cpp:425-501, sized for facet_size(RMAX)).
+struct FacetSlot {
+    double *acc = nullptr;   // 4 * fmax^2
+    double *uu = nullptr, *vv = nullptr, *dre = nullptr, *dim = nullptr, *s2 = nullptr;
+    double *sumw = nullptr;  // dense weights, allocated on first request
+    unsigned *scan = nullptr;
+    unsigned facet = 0;
+    unsigned n = 0;          // host copy of the compacted size (valid after a sync)
+    double rprior[kMaxR] = {};
+};
+
+// Buffers of one batched likelihood launch (items -> L_e).
+struct LikBuffers {
+    int cap = 0;
+    LikItem *d_items = nullptr, *h_items = nullptr;
+    double *d_Lr = nullptr, *h_Lr = nullptr;
+    int *d_nit = nullptr, *h_nit = nullptr;
+    double *d_Le = nullptr, *h_Le = nullptr;
+    int *d_err = nullptr, *h_err = nullptr;
+    double *d_ws = nullptr;
+    int *d_iws = nullptr;
+    bool ensure(int n);
+    void release();
+};
+
+struct Obs {
+    int prec = 0;
+    unsigned nch = 0;
+    unsigned long nrow = 0, nvis = 0;
+    double len = 0.;
+    unsigned fmax = 0;
+    double *d_u = nullptr, *d_v = nullptr, *d_w = nullptr;
+    uint32_t *d_perm = nullptr;  // sorted row -> original row
+    void *d_res = nullptr;
+    float *d_s2f = nullptr;
+    double *d_wn = nullptr, *d_invlam = nullptr, *d_spec = nullptr;
+    std::vector<double> wn, spec, invlam;
+    std::vector<FacetSlot> slots;
+    FacetDev *d_facets = nullptr;
+    std::vector<FacetDev> h_facets;
+    LikBuffers lik;
+    RGrid rg;
+    ~Obs();
+    bool ensure_slots(int n);
+    void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device
+};
+
+// Batched likelihood evaluation for items over the observation's facet slots.
+// Fills Le/err (and L_r, n_iter when non-null: nitems * kMaxR). Synchronous.
+bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, double *Lr,
+              int *nit);
+
+// Stream and instrumentation shared by all translation units.
+hipStream_t stream();
+struct KTimer {
+    int kernel;
+    hipEvent_t a = nullptr, b = nullptr;
+    KTimer(int k);
+    ~KTimer();
+};
+void stats_add_work(int kernel, double flops, double bytes);
+
+GalaxyGeom make_geom(double l0, double m0, double flux, double scale, double e1, double e2);
+bool ensure_device();
+bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
+                 unsigned facet, bool want_sumw, double e1 = 0., double e2 = 0.);
+bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2);
+bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const double *l,
+                  const double *m, const double *flux, const double *scale,
+                  const rl2_fit_options &opt, rl2_gal_result *results,
+                  std::vector<unsigned> *new_bad, double *t_ext, double *t_fit, double *t_sub,
+                  const std::function<void(unsigned)> &on_done);
+
+// Fit driver (rl2_fit.cpp).
+struct FitRequest {
+    int slot;
+    rl2_gal_result *res;
+    // likelihood_sampling only (likelihood.c:428-554): start from (e1, e2, maxL) in *res
+    bool sampling_only = false;
+    int np_max = kNpMax;
+};
+using BatchEval = std::function<bool(const std::vector<LikItem> &, std::vector<double> &,
+                                     std::vector<int> &)>;
+bool fit_batch(const std::vector<FitRequest> &reqs, const rl2_fit_options &opt,
+               const BatchEval &eval);
+
+}  // namespace rl2
+
+struct rl2_obs {
+    rl2::Obs o;
+};

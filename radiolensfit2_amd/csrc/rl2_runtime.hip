@@ -1,0 +1,883 @@
+// rl2_runtime.hip -- observation lifecycle, batched likelihood launches, the per-galaxy
+// driver and the rl2_* C-ABI (include/rl2.h).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <math.h>
+#include <mutex>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "rl2_obs.h"
+
+namespace rl2 {
+
+// ------------------------------------------------------------------ global state
+namespace {
+thread_local std::string g_err;
+int g_device = -1;
+hipStream_t g_stream = nullptr;
+bool g_stats_on = false;
+struct KStat {
+    double ms = 0.;
+    long long launches = 0;
+    double flops = 0., bytes = 0.;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+KStat g_stats[RL2_K_COUNT];
+std::vector<hipEvent_t> g_event_pool;
+
+hipEvent_t take_event()
+{
+    if (!g_event_pool.empty()) {
+        hipEvent_t e = g_event_pool.back();
+        g_event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+void drain_stats()
+{
+    for (auto &k : g_stats) {
+        for (auto &p : k.pending) {
+            float ms = 0.f;
+            hipEventSynchronize(p.second);
+            hipEventElapsedTime(&ms, p.first, p.second);
+            k.ms += ms;
+            g_event_pool.push_back(p.first);
+            g_event_pool.push_back(p.second);
+        }
+        k.pending.clear();
+    }
+}
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+}  // namespace
+
+void set_error(const std::string &msg) { g_err = msg; }
+bool hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+hipStream_t stream() { return g_stream; }
+
+KTimer::KTimer(int k) : kernel(k)
+{
+    if (g_stats_on) {
+        a = take_event();
+        b = take_event();
+        hipEventRecord(a, g_stream);
+    }
+}
+KTimer::~KTimer()
+{
+    if (g_stats_on && a) {
+        hipEventRecord(b, g_stream);
+        g_stats[kernel].pending.push_back({a, b});
+        g_stats[kernel].launches++;
+    }
+}
+void stats_add_work(int kernel, double flops, double bytes)
+{
+    if (!g_stats_on) return;
+    g_stats[kernel].flops += flops;
+    g_stats[kernel].bytes += bytes;
+}
+
+static bool ensure_init()
+{
+    if (g_stream) return true;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_error("no HIP device available (librl2hip has no CPU path)");
+        return false;
+    }
+    if (g_device < 0) g_device = 0;
+    if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return false;
+    return hip_ok(hipStreamCreate(&g_stream), "hipStreamCreate");
+}
+
+template <class T>
+static bool dmalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    return hip_ok(hipMalloc((void **)p, count * sizeof(T)), "hipMalloc");
+}
+template <class T>
+static void dfree(T *&p)
+{
+    if (p) hipFree((void *)p);
+    p = nullptr;
+}
+
+// ------------------------------------------------------------------ likelihood buffers
+bool LikBuffers::ensure(int n)
+{
+    if (n <= cap) return true;
+    release();
+    int c = 256;
+    while (c < n) c *= 2;
+    bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
+              dmalloc(&d_nit, (size_t)c * kMaxR) && dmalloc(&d_Le, c) && dmalloc(&d_err, c) &&
+              dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts);
+    ok = ok && hip_ok(hipHostMalloc((void **)&h_items, c * sizeof(LikItem)), "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_Lr, (size_t)c * kMaxR * sizeof(double)), "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_nit, (size_t)c * kMaxR * sizeof(int)), "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_Le, c * sizeof(double)), "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_err, c * sizeof(int)), "hipHostMalloc");
+    if (ok) cap = c;
+    return ok;
+}
+void LikBuffers::release()
+{
+    dfree(d_items);
+    dfree(d_Lr);
+    dfree(d_nit);
+    dfree(d_Le);
+    dfree(d_err);
+    dfree(d_ws);
+    dfree(d_iws);
+    if (h_items) hipHostFree(h_items);
+    if (h_Lr) hipHostFree(h_Lr);
+    if (h_nit) hipHostFree(h_nit);
+    if (h_Le) hipHostFree(h_Le);
+    if (h_err) hipHostFree(h_err);
+    h_items = nullptr;
+    h_Lr = nullptr;
+    h_nit = nullptr;
+    h_Le = nullptr;
+    h_err = nullptr;
+    cap = 0;
+}
+
+// ------------------------------------------------------------------ observation
+Obs::~Obs()
+{
+    hipStreamSynchronize(g_stream);
+    dfree(d_u);
+    dfree(d_v);
+    dfree(d_w);
+    dfree(d_perm);
+    dfree(d_res);
+    dfree(d_s2f);
+    dfree(d_wn);
+    dfree(d_invlam);
+    dfree(d_spec);
+    for (auto &s : slots) {
+        dfree(s.acc);
+        dfree(s.uu);
+        dfree(s.vv);
+        dfree(s.dre);
+        dfree(s.dim);
+        dfree(s.s2);
+        dfree(s.sumw);
+        dfree(s.scan);
+    }
+    dfree(d_facets);
+    lik.release();
+}
+
+bool Obs::ensure_slots(int n)
+{
+    if ((int)slots.size() >= n) return true;
+    size_t ncell = (size_t)fmax * fmax;
+    int old = (int)slots.size();
+    slots.resize(n);
+    for (int i = old; i < n; i++) {
+        FacetSlot &s = slots[i];
+        if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
+              dmalloc(&s.dre, ncell) && dmalloc(&s.dim, ncell) && dmalloc(&s.s2, ncell) &&
+              dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
+            return false;
+        if (!hip_ok(hipMemsetAsync(s.acc, 0, 4 * ncell * sizeof(double), g_stream), "memset"))
+            return false;
+    }
+    FacetDev *nd = nullptr;
+    if (!dmalloc(&nd, n)) return false;
+    if (d_facets) {
+        hipMemcpyAsync(nd, d_facets, old * sizeof(FacetDev), hipMemcpyDeviceToDevice, g_stream);
+        hipStreamSynchronize(g_stream);
+        hipFree(d_facets);
+    }
+    d_facets = nd;
+    h_facets.resize(n);
+    for (int i = old; i < n; i++) {
+        FacetDev &f = h_facets[i];
+        memset(&f, 0, sizeof f);
+        f.uu = slots[i].uu;
+        f.vv = slots[i].vv;
+        f.dre = slots[i].dre;
+        f.dim = slots[i].dim;
+        f.s2 = slots[i].s2;
+        f.mod = nullptr;
+        f.n = 0;
+        // full descriptor once (n = 0 until a compaction or load writes it)
+        if (!hip_ok(hipMemcpyAsync(d_facets + i, &f, sizeof(FacetDev), hipMemcpyHostToDevice,
+                                   g_stream),
+                    "facet descriptor"))
+            return false;
+    }
+    return hip_ok(hipStreamSynchronize(g_stream), "ensure_slots");
+}
+
+// Pushes the slot descriptor except the device-written count n (updated by compaction).
+void Obs::facet_upload(int i)
+{
+    FacetDev &f = h_facets[i];
+    for (int k = 0; k < kMaxR; k++) f.rprior[k] = slots[i].rprior[k];
+    const size_t off = offsetof(FacetDev, rprior);
+    hipMemcpyAsync((char *)(d_facets + i) + off, (char *)&f + off, sizeof(FacetDev) - off,
+                   hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(d_facets + i, &f, offsetof(FacetDev, n), hipMemcpyHostToDevice, g_stream);
+}
+
+GalaxyGeom make_geom(double l0, double m0, double flux, double scale, double e1, double e2)
+{
+    GalaxyGeom g;
+    double detA = 1. - e1 * e1 - e2 * e2;
+    double sc = arcs2rad_scale(scale);
+    g.sf = (sc * sc) / (detA * detA);
+    g.l0 = l0;
+    g.m0 = m0;
+    g.n0 = sqrt(1. - l0 * l0 - m0 * m0) - 1.;
+    g.flux = flux;
+    g.e1 = e1;
+    g.e2 = e2;
+    return g;
+}
+
+static double bytes_per_vis(const Obs &o, int kernel)
+{
+    const double res = o.prec == 0 ? 16. : 8.;
+    if (kernel == RL2_K_EXTRACT) return res + 4.;  // residual read + sigma^2/flag
+    return 2. * res;                                // read-modify-write
+}
+
+bool ensure_device() { return ensure_init(); }
+
+bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
+                 unsigned facet, bool want_sumw, double e1, double e2)
+{
+    if (facet == 0 || facet > o.fmax) {
+        set_error("facet size " + std::to_string(facet) + " outside the allocated facet (max " +
+                  std::to_string(o.fmax) + ")");
+        return false;
+    }
+    if (!o.ensure_slots(slot + 1)) return false;
+    FacetSlot &s = o.slots[slot];
+    s.facet = facet;
+    GridLaunch a;
+    a.mode = 0;
+    a.prec = o.prec;
+    a.nch = o.nch;
+    a.nrow = o.nrow;
+    a.u = o.d_u;
+    a.v = o.d_v;
+    a.w = o.d_w;
+    a.wn = o.d_wn;
+    a.inv_lambda = o.d_invlam;
+    a.spec = o.d_spec;
+    a.res = o.d_res;
+    a.s2f = o.d_s2f;
+    a.g = make_geom(l0, m0, flux, R_mu, e1, e2);
+    a.len = o.len;
+    a.inc = 2 * o.len / facet;
+    a.facet = facet;
+    a.acc = s.acc;
+    {
+        KTimer t(RL2_K_EXTRACT);
+        launch_grid(g_stream, a);
+    }
+    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_EXTRACT) + o.nrow * 24.);
+    if (want_sumw && !s.sumw && !dmalloc(&s.sumw, (size_t)o.fmax * o.fmax)) return false;
+    {
+        KTimer t(RL2_K_COMPACT);
+        launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2,
+                       &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
+    }
+    return hip_ok(hipGetLastError(), "extract launch");
+}
+
+bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2)
+{
+    GalaxyGeom fit = make_geom(l0, m0, flux, R_mu, e1, e2);
+    GalaxyGeom rnd = make_geom(l0, m0, flux, R_mu, 0., 0.);
+    {
+        KTimer t(RL2_K_SUBTRACT);
+        launch_subtract(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn, o.d_spec, fit,
+                        rnd, o.d_res);
+    }
+    stats_add_work(RL2_K_SUBTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_SUBTRACT) + o.nrow * 24.);
+    return hip_ok(hipGetLastError(), "subtract launch");
+}
+
+bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, double *Lr,
+              int *nit)
+{
+    const int n = (int)items.size();
+    if (n == 0) return true;
+    if (!o.lik.ensure(n)) return false;
+    LikBuffers &b = o.lik;
+    memcpy(b.h_items, items.data(), n * sizeof(LikItem));
+    hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice, g_stream);
+    LikLaunch a;
+    a.nitems = n;
+    a.items = b.d_items;
+    a.facets = o.d_facets;
+    a.rg = o.rg;
+    a.Lr = b.d_Lr;
+    a.nit = b.d_nit;
+    a.Le = b.d_Le;
+    a.err = b.d_err;
+    a.qag_ws = b.d_ws;
+    a.qag_iws = b.d_iws;
+    a.block = 256;
+    a.prec = o.prec;
+    {
+        KTimer t(RL2_K_LIKELIHOOD);
+        launch_likelihood(g_stream, a);
+    }
+    {
+        KTimer t(RL2_K_RMARG);
+        launch_rmarg(g_stream, a);
+    }
+    if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+    hipMemcpyAsync(b.h_Le, b.d_Le, n * sizeof(double), hipMemcpyDeviceToHost, g_stream);
+    hipMemcpyAsync(b.h_err, b.d_err, n * sizeof(int), hipMemcpyDeviceToHost, g_stream);
+    const bool want_detail = Lr || nit || g_stats_on;
+    if (want_detail) {
+        hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double), hipMemcpyDeviceToHost,
+                       g_stream);
+        hipMemcpyAsync(b.h_nit, b.d_nit, (size_t)n * kMaxR * sizeof(int), hipMemcpyDeviceToHost,
+                       g_stream);
+    }
+    if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+    memcpy(Le, b.h_Le, n * sizeof(double));
+    memcpy(err, b.h_err, n * sizeof(int));
+    if (Lr) memcpy(Lr, b.h_Lr, (size_t)n * kMaxR * sizeof(double));
+    if (nit) memcpy(nit, b.h_nit, (size_t)n * kMaxR * sizeof(int));
+    if (g_stats_on) {
+        // FLOP-equivalents per (item, r): n * (19 + 18 + 25 * N_it)
+        double fl = 0.;
+        for (int i = 0; i < n; i++) {
+            const double nn = o.slots[items[i].slot].n;
+            for (int r = 1; r < o.rg.numr; r++) fl += nn * (37. + 25. * b.h_nit[(size_t)i * kMaxR + r]);
+        }
+        stats_add_work(RL2_K_LIKELIHOOD, fl, 0.);
+    }
+    return true;
+}
+
+static bool read_slot_n(Obs &o, int slot, unsigned *n)
+{
+    return hip_ok(hipMemcpyAsync(n, &o.d_facets[slot].n, sizeof *n, hipMemcpyDeviceToHost, g_stream),
+                  "copy n") &&
+           hip_ok(hipStreamSynchronize(g_stream), "sync");
+}
+
+static void prior_for(Obs &o, int slot, double flux)
+{
+    const double mu = scale_mean(flux);
+    FacetSlot &s = o.slots[slot];
+    for (int k = 0; k < kMaxR; k++) s.rprior[k] = 0.;
+    for (int nRo = 1; nRo < o.rg.numr; nRo++) s.rprior[nRo] = rfunc(mu, kRstd, o.rg.ro[nRo]);
+    o.facet_upload(slot);
+}
+
+// One pass of data_processing.c:49-142 over `todo` (batch > 1: data_processing-mpi.c:65-216).
+// pass 0 collects failed fits in `new_bad`; pass 1 (re-fit) reports them with e = 0.
+bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const double *l,
+                  const double *m, const double *flux, const double *scale,
+                  const rl2_fit_options &opt, rl2_gal_result *results,
+                  std::vector<unsigned> *new_bad, double *t_ext, double *t_fit, double *t_sub,
+                  const std::function<void(unsigned)> &on_done)
+{
+    const int B = opt.batch > 0 ? opt.batch : 1;
+    if (!o.ensure_slots(B)) return false;
+    BatchEval eval = [&](const std::vector<LikItem> &items, std::vector<double> &Le,
+                         std::vector<int> &err) {
+        return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);
+    };
+    for (size_t b0 = 0; b0 < todo.size(); b0 += B) {
+        const size_t nb = std::min((size_t)B, todo.size() - b0);
+        double t0 = now_s();
+        for (size_t k = 0; k < nb; k++) {
+            const unsigned gal = todo[b0 + k];
+            prior_for(o, (int)k, flux[gal]);
+            const unsigned facet = facet_size(scale[gal], o.len);
+            if (!obs_extract(o, (int)k, l[gal], m[gal], flux[gal], scale[gal], facet, false))
+                return false;
+            results[gal].facet = facet;
+        }
+        for (size_t k = 0; k < nb; k++) {
+            unsigned nn = 0;
+            if (!read_slot_n(o, (int)k, &nn)) return false;
+            o.slots[k].n = nn;
+            rl2_gal_result &r = results[todo[b0 + k]];
+            r.ncoords = nn;
+            r.pass = pass;
+            r.subtracted = 0;
+        }
+        double t1 = now_s();
+        *t_ext += t1 - t0;
+        std::vector<FitRequest> reqs;
+        for (size_t k = 0; k < nb; k++) reqs.push_back({(int)k, &results[todo[b0 + k]]});
+        if (!fit_batch(reqs, opt, eval)) return false;
+        double t2 = now_s();
+        *t_fit += t2 - t1;
+        for (size_t k = 0; k < nb; k++) {
+            const unsigned gal = todo[b0 + k];
+            rl2_gal_result &r = results[gal];
+            if (r.error) {
+                if (pass == 0) {
+                    if (new_bad) new_bad->push_back(gal);
+                } else {
+                    r.e1 = 0.;
+                    r.e2 = 0.;
+                }
+            } else {
+                if (!obs_subtract(o, l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2)) return false;
+                r.subtracted = 1;
+            }
+            if (on_done) on_done(gal);
+        }
+        if (!hip_ok(hipStreamSynchronize(g_stream), "subtract sync")) return false;
+        *t_sub += now_s() - t2;
+    }
+    return true;
+}
+
+bool process(Obs &o, unsigned ngal, const double *l, const double *m, const double *flux,
+             const double *scale, const rl2_fit_options &opt, rl2_gal_result *results,
+             rl2_run_stats *stats)
+{
+    double t_ext = 0., t_fit = 0., t_sub = 0.;
+    std::vector<unsigned> todo(ngal), bad;
+    for (unsigned g = 0; g < ngal; g++) todo[g] = g;
+    if (!process_list(o, todo, 0, l, m, flux, scale, opt, results, &bad, &t_ext, &t_fit, &t_sub,
+                      nullptr))
+        return false;
+    const int n_bad_first = (int)bad.size();
+    int n_bad_final = n_bad_first;
+    if (opt.do_refit && !bad.empty()) {
+        if (!process_list(o, bad, 1, l, m, flux, scale, opt, results, nullptr, &t_ext, &t_fit,
+                          &t_sub, nullptr))
+            return false;
+        n_bad_final = 0;
+        for (unsigned g : bad)
+            if (results[g].error) n_bad_final++;
+    }
+    if (stats) {
+        stats->extraction_s = t_ext;
+        stats->fitting_s = t_fit;
+        stats->subtraction_s = t_sub;
+        stats->n_bad_first = n_bad_first;
+        stats->n_bad_final = n_bad_final;
+    }
+    return true;
+}
+
+}  // namespace rl2
+
+using namespace rl2;
+
+// ====================================================================== C-ABI
+extern "C" {
+
+const char *rl2_last_error(void) { return g_err.c_str(); }
+
+int rl2_init(int device)
+{
+    if (g_stream && device == g_device) return RL2_OK;
+    if (g_stream) {
+        set_error("rl2_init: already initialised on another device");
+        return RL2_ERR_STATE;
+    }
+    g_device = device;
+    return ensure_init() ? RL2_OK : RL2_ERR_NO_DEVICE;
+}
+
+int rl2_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return RL2_OK;
+}
+
+int rl2_r_grid(double *ro)
+{
+    // RadioLensfit2.cpp:370-392
+    const double Rmin = kRmin, Rmax = kRmax;
+    ro[0] = 0.;
+    ro[1] = Rmin;
+    int nRo = 2;
+    while (nRo < kNumR && ro[nRo - 1] < Rmax) {
+        double Rinterval = 0.08 + 0.4 * pow(((ro[nRo - 1] - Rmin) / (Rmax - Rmin)), 2);
+        ro[nRo] = ro[nRo - 1] + Rinterval;
+        nRo++;
+    }
+    return nRo;
+}
+
+int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const double *u,
+                   const double *v, const double *w, const double *wavenumbers,
+                   const double *spec, const float *sigma2_vis, const unsigned char *flag,
+                   double len, int precision)
+{
+    *out = nullptr;
+    if (!ensure_init()) return RL2_ERR_NO_DEVICE;
+    if (!u || !v || !w || !wavenumbers || !spec || !sigma2_vis || nch == 0 || nrow == 0 ||
+        !(len > 0.) || (precision != RL2_PREC_FP64 && precision != RL2_PREC_MIXED)) {
+        set_error("rl2_obs_create: invalid arguments");
+        return RL2_ERR_ARG;
+    }
+    if (nrow >= (1ul << 32) || (unsigned long long)nch * nrow >= (1ull << 40)) {
+        set_error("rl2_obs_create: observation too large");
+        return RL2_ERR_RANGE;
+    }
+    rl2_obs *h = new rl2_obs;
+    Obs &o = h->o;
+    o.prec = precision;
+    o.nch = nch;
+    o.nrow = nrow;
+    o.nvis = (unsigned long)nch * nrow;
+    o.len = len;
+    o.fmax = facet_size(kRmax, len);
+    o.rg.numr = rl2_r_grid(o.rg.ro);
+    for (int k = o.rg.numr; k < kMaxR; k++) o.rg.ro[k] = 0.;
+    o.wn.assign(wavenumbers, wavenumbers + nch);
+    o.spec.assign(spec, spec + nch);
+    o.invlam.resize(nch);
+    for (unsigned c = 0; c < nch; c++) o.invlam[c] = wavenumbers[c] / (2. * kPi);
+    hipStream_t s = g_stream;
+    double *tu = nullptr, *tv = nullptr, *tw = nullptr;
+    uint32_t *keys = nullptr, *idx = nullptr, *keys2 = nullptr, *idx2 = nullptr;
+    float *ts2 = nullptr;
+    unsigned char *tfl = nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = sort_rows_tmp_bytes(nrow);
+    bool ok = dmalloc(&o.d_u, nrow) && dmalloc(&o.d_v, nrow) && dmalloc(&o.d_w, nrow) &&
+              dmalloc(&o.d_perm, nrow) && dmalloc(&o.d_s2f, o.nvis) && dmalloc(&o.d_wn, nch) &&
+              dmalloc(&o.d_invlam, nch) && dmalloc(&o.d_spec, nch) && dmalloc(&tu, nrow) &&
+              dmalloc(&tv, nrow) && dmalloc(&tw, nrow) && dmalloc(&keys, nrow) &&
+              dmalloc(&idx, nrow) && dmalloc(&keys2, nrow) && dmalloc(&idx2, nrow) &&
+              dmalloc(&ts2, o.nvis) && (!flag || dmalloc(&tfl, o.nvis)) &&
+              hip_ok(hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1), "hipMalloc");
+    size_t res_bytes = o.nvis * (precision == RL2_PREC_FP64 ? 16 : 8);
+    ok = ok && hip_ok(hipMalloc(&o.d_res, res_bytes), "hipMalloc residual");
+    if (ok) {
+        hipMemcpyAsync(tu, u, nrow * 8, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(tv, v, nrow * 8, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(tw, w, nrow * 8, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(ts2, sigma2_vis, o.nvis * 4, hipMemcpyHostToDevice, s);
+        if (flag) hipMemcpyAsync(tfl, flag, o.nvis, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(o.d_wn, o.wn.data(), nch * 8, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(o.d_invlam, o.invlam.data(), nch * 8, hipMemcpyHostToDevice, s);
+        hipMemcpyAsync(o.d_spec, o.spec.data(), nch * 8, hipMemcpyHostToDevice, s);
+        // spatial sort of the rows (Morton order of (u, v) over [-len_m, len_m])
+        double umax = 0.;
+        for (unsigned c = 0; c < nch; c++) umax = fmax(umax, len * 2. * kPi / wavenumbers[c]);
+        umax *= 1.0000001;
+        launch_morton_keys(s, nrow, tu, tv, umax, keys, idx);
+        uint32_t *sorted = nullptr;
+        sort_rows(s, nrow, keys, idx, keys2, idx2, tmp, tmp_bytes, &sorted);
+        hipMemcpyAsync(o.d_perm, sorted, nrow * 4, hipMemcpyDeviceToDevice, s);
+        launch_gather_rows(s, nrow, o.d_perm, tu, o.d_u);
+        launch_gather_rows(s, nrow, o.d_perm, tv, o.d_v);
+        launch_gather_rows(s, nrow, o.d_perm, tw, o.d_w);
+        launch_pack_sigma(s, nch, nrow, o.d_perm, ts2, tfl, o.d_s2f);
+        hipMemsetAsync(o.d_res, 0, res_bytes, s);
+        ok = hip_ok(hipGetLastError(), "obs set-up") &&
+             hip_ok(hipStreamSynchronize(s), "obs set-up sync");
+    }
+    dfree(tu);
+    dfree(tv);
+    dfree(tw);
+    dfree(keys);
+    dfree(idx);
+    dfree(keys2);
+    dfree(idx2);
+    dfree(ts2);
+    dfree(tfl);
+    if (tmp) hipFree(tmp);
+    if (!ok) {
+        delete h;
+        return RL2_ERR_HIP;
+    }
+    *out = h;
+    return RL2_OK;
+}
+
+int rl2_obs_destroy(rl2_obs *obs)
+{
+    delete obs;
+    return RL2_OK;
+}
+
+int rl2_obs_set_vis(rl2_obs *obs, const double *visData, const double *visSkyMod)
+{
+    if (!obs || !visData) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    double *dD = nullptr, *dS = nullptr;
+    bool ok = dmalloc(&dD, 2 * o.nvis) && (!visSkyMod || dmalloc(&dS, 2 * o.nvis));
+    if (ok) {
+        hipMemcpyAsync(dD, visData, o.nvis * 16, hipMemcpyHostToDevice, g_stream);
+        if (dS) hipMemcpyAsync(dS, visSkyMod, o.nvis * 16, hipMemcpyHostToDevice, g_stream);
+        launch_residual_in(g_stream, o.prec, o.nch, o.nrow, o.d_perm, dD, dS, o.d_res);
+        ok = hip_ok(hipGetLastError(), "set_vis") && hip_ok(hipStreamSynchronize(g_stream), "sync");
+    }
+    dfree(dD);
+    dfree(dS);
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_get_residual(rl2_obs *obs, double *residual)
+{
+    if (!obs || !residual) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    double *d = nullptr;
+    bool ok = dmalloc(&d, 2 * o.nvis);
+    if (ok) {
+        launch_residual_out(g_stream, o.prec, o.nch, o.nrow, o.d_perm, o.d_res, d);
+        hipMemcpyAsync(residual, d, o.nvis * 16, hipMemcpyDeviceToHost, g_stream);
+        ok = hip_ok(hipGetLastError(), "get_residual") &&
+             hip_ok(hipStreamSynchronize(g_stream), "sync");
+    }
+    dfree(d);
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_sync(rl2_obs *obs, double *visData, double *visSkyMod)
+{
+    int st = rl2_obs_get_residual(obs, visData);
+    if (st == RL2_OK && visSkyMod) memset(visSkyMod, 0, obs->o.nvis * 16);
+    return st;
+}
+
+int rl2_obs_add_sources(rl2_obs *obs, unsigned int ngal, const double *flux, const double *scale,
+                        const double *l, const double *m, const double *e1, const double *e2,
+                        double sign)
+{
+    if (!obs || !flux || !scale || !l || !m) return RL2_ERR_ARG;
+    if (ngal == 0) return RL2_OK;
+    Obs &o = obs->o;
+    std::vector<GalaxyGeom> g(ngal);
+    for (unsigned i = 0; i < ngal; i++)
+        g[i] = make_geom(l[i], m[i], flux[i], scale[i], e1 ? e1[i] : 0., e2 ? e2[i] : 0.);
+    GalaxyGeom *dg = nullptr;
+    if (!dmalloc(&dg, ngal)) return RL2_ERR_HIP;
+    hipMemcpyAsync(dg, g.data(), ngal * sizeof(GalaxyGeom), hipMemcpyHostToDevice, g_stream);
+    {
+        KTimer t(RL2_K_SOURCES);
+        launch_add_sources(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn, o.d_spec,
+                           dg, ngal, sign, o.d_res);
+    }
+    bool ok = hip_ok(hipGetLastError(), "add_sources") &&
+              hip_ok(hipStreamSynchronize(g_stream), "sync");
+    dfree(dg);
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_add_noise(rl2_obs *obs, double sigma2, uint64_t seed)
+{
+    if (!obs || !(sigma2 >= 0.)) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    launch_add_noise(g_stream, o.prec, o.nvis, sqrt(sigma2), seed, o.d_res);
+    return hip_ok(hipGetLastError(), "add_noise") && hip_ok(hipStreamSynchronize(g_stream), "sync")
+               ? RL2_OK
+               : RL2_ERR_HIP;
+}
+
+int rl2_obs_extract(rl2_obs *obs, int slot, double l0, double m0, double flux, double R_mu,
+                    unsigned int facet, double *facet_vis, double *facet_sigma2, double *uu,
+                    double *vv, double *sum_w, unsigned int *ncoords)
+{
+    if (!obs || slot < 0) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if (!obs_extract(o, slot, l0, m0, flux, R_mu, facet, sum_w != nullptr)) return RL2_ERR_HIP;
+    unsigned n = 0;
+    if (!read_slot_n(o, slot, &n)) return RL2_ERR_HIP;
+    FacetSlot &s = o.slots[slot];
+    s.n = n;
+    if (ncoords) *ncoords = n;
+    std::vector<double> re, im;
+    if (facet_vis) {
+        re.resize(n);
+        im.resize(n);
+        hipMemcpyAsync(re.data(), s.dre, n * 8, hipMemcpyDeviceToHost, g_stream);
+        hipMemcpyAsync(im.data(), s.dim, n * 8, hipMemcpyDeviceToHost, g_stream);
+    }
+    if (facet_sigma2) hipMemcpyAsync(facet_sigma2, s.s2, n * 8, hipMemcpyDeviceToHost, g_stream);
+    if (uu) hipMemcpyAsync(uu, s.uu, n * 8, hipMemcpyDeviceToHost, g_stream);
+    if (vv) hipMemcpyAsync(vv, s.vv, n * 8, hipMemcpyDeviceToHost, g_stream);
+    if (sum_w)
+        hipMemcpyAsync(sum_w, s.sumw, (size_t)facet * facet * 8, hipMemcpyDeviceToHost, g_stream);
+    if (!hip_ok(hipStreamSynchronize(g_stream), "extract copy")) return RL2_ERR_HIP;
+    if (facet_vis)
+        for (unsigned i = 0; i < n; i++) {
+            facet_vis[2 * i] = re[i];
+            facet_vis[2 * i + 1] = im[i];
+        }
+    return RL2_OK;
+}
+
+int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *facet_vis,
+                       const double *facet_sigma2, const double *uu, const double *vv)
+{
+    if (!obs || slot < 0 || !facet_vis || !facet_sigma2 || !uu || !vv) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if (!o.ensure_slots(slot + 1)) return RL2_ERR_HIP;
+    if ((size_t)n > (size_t)o.fmax * o.fmax) {
+        set_error("facet larger than the allocated facet buffers");
+        return RL2_ERR_RANGE;
+    }
+    FacetSlot &s = o.slots[slot];
+    std::vector<double> re(n), im(n);
+    for (unsigned i = 0; i < n; i++) {
+        re[i] = facet_vis[2 * i];
+        im[i] = facet_vis[2 * i + 1];
+    }
+    hipMemcpyAsync(s.dre, re.data(), n * 8, hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(s.dim, im.data(), n * 8, hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(s.s2, facet_sigma2, n * 8, hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(s.uu, uu, n * 8, hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(s.vv, vv, n * 8, hipMemcpyHostToDevice, g_stream);
+    hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
+    s.n = n;
+    return hip_ok(hipStreamSynchronize(g_stream), "load_facet") ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_set_prior(rl2_obs *obs, int slot, const double *rprior)
+{
+    if (!obs || slot < 0 || !rprior) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if (!o.ensure_slots(slot + 1)) return RL2_ERR_HIP;
+    for (int k = 0; k < kMaxR; k++) o.slots[slot].rprior[k] = (k < o.rg.numr) ? rprior[k] : 0.;
+    o.facet_upload(slot);
+    return hip_ok(hipStreamSynchronize(g_stream), "set_prior") ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const double *e2,
+                   double *L_e, int *error, double *L_r, int *n_iter)
+{
+    if (!obs || slot < 0 || npts < 0 || (npts && (!e1 || !e2 || !L_e || !error)))
+        return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if ((int)o.slots.size() <= slot) return RL2_ERR_STATE;
+    std::vector<LikItem> items(npts);
+    for (int i = 0; i < npts; i++) items[i] = {slot, 0, e1[i], e2[i]};
+    std::vector<double> Lr;
+    std::vector<int> nit;
+    if (L_r) Lr.resize((size_t)npts * kMaxR);
+    if (n_iter) nit.resize((size_t)npts * kMaxR);
+    if (!lik_eval(o, items, L_e, error, L_r ? Lr.data() : nullptr, n_iter ? nit.data() : nullptr))
+        return RL2_ERR_HIP;
+    const int nr = o.rg.numr;
+    for (int i = 0; i < npts; i++)
+        for (int r = 0; r < nr; r++) {
+            if (L_r) L_r[(size_t)i * nr + r] = (r == 0) ? 0. : Lr[(size_t)i * kMaxR + r];
+            if (n_iter) n_iter[(size_t)i * nr + r] = (r == 0) ? 0 : nit[(size_t)i * kMaxR + r];
+        }
+    return RL2_OK;
+}
+
+int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_result *res)
+{
+    if (!obs || slot < 0 || !res) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if ((int)o.slots.size() <= slot) return RL2_ERR_STATE;
+    rl2_fit_options op = {};
+    if (opt) op = *opt;
+    if (!opt) {
+        op.nm_speculate = 1;
+        op.line_chunk = 4;
+    }
+    std::vector<FitRequest> reqs{{slot, res}};
+    bool ok = fit_batch(reqs, op, [&](const std::vector<LikItem> &items, std::vector<double> &Le,
+                                      std::vector<int> &err) {
+        return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);
+    });
+    unsigned n = 0;
+    if (ok && read_slot_n(o, slot, &n)) res->ncoords = n;
+    res->facet = o.slots[slot].facet;
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_obs_subtract(rl2_obs *obs, double l0, double m0, double flux, double R_mu, double e1,
+                     double e2)
+{
+    if (!obs) return RL2_ERR_ARG;
+    if (!obs_subtract(obs->o, l0, m0, flux, R_mu, e1, e2)) return RL2_ERR_HIP;
+    return hip_ok(hipStreamSynchronize(g_stream), "subtract") ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_process(rl2_obs *obs, unsigned int ngal, const double *l, const double *m,
+                const double *flux, const double *scale, const rl2_fit_options *opt,
+                rl2_gal_result *results, rl2_run_stats *stats)
+{
+    if (!obs || !l || !m || !flux || !scale || !results) return RL2_ERR_ARG;
+    rl2_fit_options op = {};
+    if (opt)
+        op = *opt;
+    else {
+        op.batch = 1;
+        op.do_refit = 1;
+        op.nm_speculate = 1;
+        op.line_chunk = 4;
+    }
+    for (unsigned g = 0; g < ngal; g++) memset(&results[g], 0, sizeof(rl2_gal_result));
+    return process(obs->o, ngal, l, m, flux, scale, op, results, stats) ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_stats_enable(int on)
+{
+    if (!ensure_init()) return RL2_ERR_NO_DEVICE;
+    hipStreamSynchronize(g_stream);
+    drain_stats();
+    g_stats_on = on != 0;
+    return RL2_OK;
+}
+
+int rl2_stats_reset(void)
+{
+    if (g_stream) hipStreamSynchronize(g_stream);
+    drain_stats();
+    for (auto &k : g_stats) {
+        k.ms = 0.;
+        k.launches = 0;
+        k.flops = 0.;
+        k.bytes = 0.;
+    }
+    return RL2_OK;
+}
+
+int rl2_stats_get(int kernel, rl2_kernel_stat *out)
+{
+    if (kernel < 0 || kernel >= RL2_K_COUNT || !out) return RL2_ERR_ARG;
+    if (g_stream) hipStreamSynchronize(g_stream);
+    drain_stats();
+    out->ms = g_stats[kernel].ms;
+    out->launches = g_stats[kernel].launches;
+    out->flops = g_stats[kernel].flops;
+    out->bytes = g_stats[kernel].bytes;
+    return RL2_OK;
+}
+
+int rl2_synchronize(void)
+{
+    if (!g_stream) return RL2_OK;
+    return hip_ok(hipStreamSynchronize(g_stream), "synchronize") ? RL2_OK : RL2_ERR_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+"""Shared fixtures: seeded synthetic observations and the oracle's view of them."""
+from __future__ import annotations
+
+import ctypes as C
+import functools
+
+import numpy as np
+
+from oracle import oracle as O
+from radiolensfit2_amd import synth
+
+
+@functools.lru_cache(maxsize=8)
+def small_case(n_dish=24, n_times=60, n_chan=2, n_gal=20, seed=1, r_max=5e3, dt_s=480.0,
+               g=(0.02, -0.01), noise=True):
+    """Seeded observation + catalog + oracle sky model. Returns (obs, cat, vis_sky)."""
+    cfg = synth.small_array(n_dish=n_dish, n_times=n_times, n_chan=n_chan, r_max=r_max,
+                            dt_s=dt_s)
+    cfg.seed = seed
+    cat = synth.make_catalog(n_gal, seed=seed, g=g)
+    obs = synth.make_observation(cfg, cat, noise=noise)
+    vis_sky = oracle_sky_model(obs, cat)
+    return obs, cat, vis_sky
+
+
+def oracle_sky_model(obs, cat):
+    vs = np.zeros(obs.nvis, np.complex128)
+    gal = np.zeros(obs.nvis, np.complex128)
+    O.lib().orl2_sky_model(O.ptr(obs.wn), O.ptr(obs.spec), obs.nch, len(cat), O.ptr(cat.flux),
+                           O.ptr(cat.scale), O.ptr(cat.l), O.ptr(cat.m), obs.nrow, O.ptr(obs.u),
+                           O.ptr(obs.v), O.ptr(obs.w), O.cptr(gal), O.cptr(vs))
+    return vs
+
+
+def oracle_extract(obs, cat, g, vis_data, vis_sky):
+    L = O.lib()
+    facet = L.orl2_facet_size(cat.scale[g], obs.len)
+    fv, fs, uu, vv, sw = O.extract(obs, cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet,
+                                   vis_data, vis_sky)
+    return facet, fv, fs, uu, vv, sw
+
+
+def oracle_params(fv, fs, uu, vv, flux):
+    ro, numr = O.r_grid()
+    rp = O.rprior_for(flux, ro, numr)
+    ro40 = np.zeros(40)
+    ro40[:numr] = ro
+    return O.make_params(fv, fs, uu, vv, ro40, rp, numr)
+
+
+def oracle_fit(params):
+    out = [C.c_double() for _ in range(6)]
+    err = O.lib().orl2_source_fitting(C.byref(params), *[C.byref(x) for x in out])
+    return err, [x.value for x in out]
+
+
+def oracle_process(obs, cat, vis_sky, batch=1, refit=1):
+    L = O.lib()
+    vd = obs.vis.copy()
+    vs = vis_sky.copy()
+    res = (O.GalResult * len(cat))()
+    st = O.RunStats()
+    L.orl2_data_processing(len(cat), batch, obs.len, obs.nch, O.ptr(obs.wn), O.ptr(obs.spec),
+                           obs.nrow, O.ptr(obs.u), O.ptr(obs.v), O.ptr(obs.w),
+                           O.ptr(obs.sigma2, O.c_float_p), O.ptr(obs.flag, O.c_bool_p),
+                           O.cptr(vd), O.cptr(vs), O.ptr(cat.l), O.ptr(cat.m), O.ptr(cat.flux),
+                           O.ptr(cat.scale), res, refit, C.byref(st))
+    return res, st, vd - vs

@@ -1,0 +1,110 @@
+"""HIP path vs the FP64 oracle on seeded synthetic observations (FP64 precision mode).
+
+Tolerances: cell indices, cell order, weights and facet coordinates are bit-exact; gridded
+sums agree to ~1e-12 relative (atomic summation order and last-ulp sin/cos differences);
+log-likelihoods to 1e-8 absolute; fitted ellipticities to 1e-8 (identical simplex
+trajectory); the whole per-galaxy loop to 1e-6 in e (residual differences accumulate
+through the subtractions).
+"""
+import numpy as np
+import pytest
+
+from radiolensfit2_amd import _native as N
+from radiolensfit2_amd import pipeline as P
+from tests import _common as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case():
+    return T.small_case()
+
+
+@pytest.fixture(scope="module")
+def dev(gpu, case):
+    obs, cat, vs = case
+    d = P.DeviceObservation(obs, vis_sky=vs)
+    yield d
+    d.close()
+
+
+def test_residual_roundtrip(dev, case):
+    obs, cat, vs = case
+    r = dev.residual()
+    np.testing.assert_array_equal(r, obs.vis - vs)
+
+
+@pytest.mark.parametrize("g", [0, 5, 10, 15])
+def test_extraction_matches_oracle(dev, case, g):
+    obs, cat, vs = case
+    facet, fv, fs, uu, vv, sw = T.oracle_extract(obs, cat, g, obs.vis, vs)
+    gv, gs, guu, gvv, gsw = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet)
+    assert len(gv) == len(fv)
+    np.testing.assert_array_equal(gsw, sw)       # per-cell counts
+    np.testing.assert_array_equal(guu, uu)       # cell centres and compacted order
+    np.testing.assert_array_equal(gvv, vv)
+    np.testing.assert_allclose(gs, fs, rtol=1e-13, atol=0)
+    scale = np.abs(fv).max()
+    assert np.abs(gv - fv).max() <= 1e-11 * scale
+
+
+def test_likelihood_matches_oracle(dev, case):
+    obs, cat, vs = case
+    g = 0
+    facet, fv, fs, uu, vv, sw = T.oracle_extract(obs, cat, g, obs.vis, vs)
+    params, keep = T.oracle_params(fv, fs, uu, vv, cat.flux[g])
+    ro = P.r_grid()
+    dev.load_facet(fv, fs, uu, vv, slot=1)
+    dev.set_prior(P.rprior_for(cat.flux[g], ro), slot=1)
+    rng = np.random.default_rng(3)
+    e = rng.uniform(-0.6, 0.6, size=(24, 2))
+    e = e[(e ** 2).sum(1) < 0.65]
+    Le, err, Lr, nit = dev.loglik(e[:, 0], e[:, 1], slot=1, detail=True)
+    from oracle import oracle as O
+    import ctypes as C
+    for k in range(len(e)):
+        oLr = np.zeros(40)
+        onit = np.zeros(40, np.int32)
+        oerr = C.c_int()
+        oLe = O.lib().orl2_loglikelihood_detail(C.byref(params), e[k, 0], e[k, 1], C.byref(oerr),
+                                                O.ptr(oLr), onit.ctypes.data_as(O.c_int_p))
+        assert err[k] == oerr.value
+        np.testing.assert_array_equal(nit[k, 1:24], onit[1:24])
+        np.testing.assert_allclose(Lr[k, 1:24], oLr[1:24], rtol=0, atol=1e-8)
+        assert abs(Le[k] - oLe) <= 1e-8, (k, Le[k], oLe)
+
+
+@pytest.mark.parametrize("g", [0, 3, 7])
+def test_fit_matches_oracle(dev, case, g):
+    obs, cat, vs = case
+    facet, fv, fs, uu, vv, sw = T.oracle_extract(obs, cat, g, obs.vis, vs)
+    params, keep = T.oracle_params(fv, fs, uu, vv, cat.flux[g])
+    oerr, (e1, e2, v1, v2, od, maxL) = T.oracle_fit(params)
+    ro = P.r_grid()
+    dev.load_facet(fv, fs, uu, vv, slot=1)
+    dev.set_prior(P.rprior_for(cat.flux[g], ro), slot=1)
+    for spec in (0, 1):
+        r = dev.fit(slot=1, nm_speculate=spec, line_chunk=1 + 3 * spec)
+        assert r.error == oerr
+        assert abs(r.e1 - e1) <= 1e-8 and abs(r.e2 - e2) <= 1e-8, (r.e1, e1, r.e2, e2)
+        assert abs(r.maxL - maxL) <= 1e-8
+        np.testing.assert_allclose([r.var_e1, r.var_e2, r.oneDimvar], [v1, v2, od], rtol=1e-6,
+                                   atol=1e-12)
+
+
+@pytest.mark.parametrize("batch", [1, 4])
+def test_process_matches_oracle(gpu, case, batch):
+    obs, cat, vs = case
+    ores, ost, ores_vis = T.oracle_process(obs, cat, vs, batch=batch)
+    d = P.DeviceObservation(obs, vis_sky=vs)
+    res, st = d.process(cat, batch=batch)
+    resid = d.residual()
+    d.close()
+    assert st.n_bad_first == ost.n_bad_first and st.n_bad_final == ost.n_bad_final
+    for g in range(len(cat)):
+        a, b = res[g], ores[g]
+        assert (a.error, a.pass_, a.subtracted) == (b.error, b.pass_, b.subtracted), g
+        assert abs(a.e1 - b.e1) <= 1e-6 and abs(a.e2 - b.e2) <= 1e-6, (g, a.e1, b.e1, a.e2, b.e2)
+    scale = np.abs(ores_vis).max()
+    assert np.abs(resid - ores_vis).max() <= 1e-10 * scale
