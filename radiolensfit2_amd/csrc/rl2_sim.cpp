@@ -154,10 +154,18 @@ void inverse_cdf_draw(MT19937 &gen, int nr, double *data, double min_value, doub
 void random_ellipticity(MT19937 &gen, int ne, int NP, double *e1, double *e2)
 {
     const int N = kNPoints;
-    std::vector<double> F(N + 1);
-    double h = kEmax / N;
-    F[0] = 0.;
-    for (int i = 1; i <= N; i++) F[i] = cdf(e_pdf, i * h);
+    const double h = kEmax / N;
+    // The table depends on constants only.  The reference's trapezoid refinement adds
+    // it-1 midpoints per level (distributions.c:117), so it converges slowly and often
+    // runs all JMAX levels: build it once per process, in parallel as the reference does
+    // (generate_random_values.c:85).
+    static const std::vector<double> F = [] {
+        std::vector<double> t(kNPoints + 1, 0.);
+        const double hh = kEmax / kNPoints;
+#pragma omp parallel for schedule(dynamic, 4)
+        for (int i = 1; i <= kNPoints; i++) t[i] = cdf(e_pdf, i * hh);
+        return t;
+    }();
     double inc = (double)kPi / NP;
     int ind = 0;
     for (int i = 0; i < ne; i++) {

@@ -1,0 +1,173 @@
+"""The C-ABI libraries load and export every declared symbol; generators and the RLMS
+container (no GPU compute)."""
+import ctypes as C
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from radiolensfit2_amd import _native as N
+from radiolensfit2_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = "\n".join(ln for ln in text.splitlines() if not ln.lstrip().startswith("#"))
+    names = set()
+    for stmt in text.split(";"):
+        s = " ".join(stmt.split())
+        if not s or s.startswith(("typedef", "#")) or "{" in s or "}" in s:
+            s = s.split("}")[-1].strip()
+        m = re.match(r"^(?:extern\s+\"C\"\s*)?(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", s)
+        if m and not s.startswith("typedef"):
+            names.add(m.group(1))
+    return names
+
+
+@pytest.mark.parametrize("header,lib", [("rl2.h", "hip"), ("rl2_reference.h", "hip"),
+                                        ("rl2_sim.h", "sim")])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = _declared(header)
+    assert len(names) >= 10
+    L = N.hip() if lib == "hip" else N.sim()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_reference_surface_is_complete():
+    """Every function of the reference's per-galaxy headers (FACET build) is declared."""
+    want = {  # galaxy_visibilities.h:31-55, evaluate_uv_grid.h:29-33, source_extraction.h:31,
+        # likelihood.h:32-54, marginalise_r.h:24-27, galaxy_fitting.h:30, data_processing.h:32
+        "gaussian_shape", "sersic_shape", "model_galaxy_visibilities_at_zero",
+        "data_galaxy_visibilities", "data_visibilities_phase_shift", "facet_size",
+        "evaluate_uv_grid_size", "evaluate_facet_coords", "gridding_visibilities",
+        "average_facets", "source_extraction", "f_likelihood", "loglikelihood",
+        "loglikelihood_r", "cross_correlation", "marginalise_over_position_shift",
+        "likelihood_sampling", "set_posterior_values", "marginalise_posterior_r", "marf",
+        "source_fitting", "data_processing", "sky_model", "scale_mean", "rfunc"}
+    assert want <= _declared("rl2_reference.h")
+
+
+def test_no_device_calls_fail_cleanly_without_gpu():
+    n = C.c_int(-1)
+    assert N.hip().rl2_device_count(C.byref(n)) == 0
+    if n.value == 0:
+        h = C.c_void_p()
+        z = np.zeros(4)
+        st = N.hip().rl2_obs_create(C.byref(h), 1, 4, N.dptr(z), N.dptr(z), N.dptr(z),
+                                    N.dptr(np.ones(1)), N.dptr(np.ones(1)),
+                                    N.fptr(np.ones(4, np.float32)), None, 1.0, 0)
+        assert st == -1 and b"no HIP device" in N.hip().rl2_last_error()
+
+
+def test_mt19937_known_answers():
+    S = N.sim()
+    g = S.rl2sim_mt_new(5489)
+    for _ in range(9999):
+        S.rl2sim_mt_get(g)
+    assert S.rl2sim_mt_get(g) == 4123659995  # C++11 [rand.predef] 10000th output
+    S.rl2sim_mt_free(g)
+    g = S.rl2sim_mt_new(1)
+    bg = np.random.MT19937()
+    bg._legacy_seeding(1)
+    assert [S.rl2sim_mt_get(g) for _ in range(1000)] == list(bg.random_raw(1000))
+    assert S.rl2sim_mt_uniform(g) == bg.random_raw(1)[0] / 2 ** 32  # gsl_rng_uniform
+    S.rl2sim_mt_free(g)
+    g0, g1 = S.rl2sim_mt_new(0), S.rl2sim_mt_new(4357)  # GSL maps seed 0 to 4357
+    assert [S.rl2sim_mt_get(g0) for _ in range(5)] == [S.rl2sim_mt_get(g1) for _ in range(5)]
+
+
+def test_catalog_properties():
+    cat = synth.make_catalog(200, seed=3, fov_arcmin=20.0, fmin=10.0, fmax=200.0)
+    assert len(cat) == 200
+    f = cat.flux.reshape(20, 10)
+    assert (f == f[:, :1]).all() and (np.diff(f[:, 0]) <= 0).all()  # groups of 10, descending
+    assert (cat.flux >= 10).all() and (cat.flux <= 200).all()
+    s = cat.scale.reshape(20, 10)
+    assert (s == s[:, :1]).all() and (cat.scale >= 0.3).all() and (cat.scale <= 3.5).all()
+    e1, e2 = cat.e1.reshape(100, 2), cat.e2.reshape(100, 2)  # +/- pairs (no shear applied)
+    np.testing.assert_allclose(e1[:, 0], -e1[:, 1], atol=1.5e-6)
+    np.testing.assert_allclose(e2[:, 0], -e2[:, 1], atol=1.5e-6)
+    mod = np.hypot(cat.e1, cat.e2).reshape(20, 10)
+    np.testing.assert_allclose(mod, np.broadcast_to(mod[:, :1], mod.shape), atol=3e-6)  # one modulus per group
+    assert mod.max() < 0.804
+    r = np.hypot(cat.l, cat.m)
+    assert r.max() <= 10 * 60 * math.pi / 648000 + 1e-6  # inside the 20' disk
+    assert np.allclose(cat.l * 1e6, np.round(cat.l * 1e6), rtol=0, atol=1e-6)  # %f rounding
+
+
+def test_shear_closed_form():
+    e1 = np.array([0.0, 0.3, -0.2])
+    e2 = np.array([0.0, 0.1, 0.4])
+    a, b = e1.copy(), e2.copy()
+    N.sim().rl2sim_shear(3, 0.02, -0.01, N.dptr(a), N.dptr(b))
+    z = (e1 + 1j * e2 + (0.02 - 0.01j)) / (1 + np.conj(0.02 - 0.01j) * (e1 + 1j * e2))
+    np.testing.assert_allclose(a, z.real, rtol=1e-14, atol=1e-16)
+    np.testing.assert_allclose(b, z.imag, rtol=1e-14, atol=1e-16)
+
+
+def test_uvw_geometry():
+    cfg = synth.small_array(n_dish=10, n_times=5)
+    u, v, w, wn, spec, s2, flag, L = synth.make_geometry(cfg)
+    nbl = 45
+    assert len(u) == nbl * 5
+    # baseline length is invariant under earth rotation
+    b = np.sqrt(u * u + v * v + w * w).reshape(5, nbl)
+    np.testing.assert_allclose(b, np.broadcast_to(b[:1], b.shape), rtol=1e-12)
+    assert L == math.ceil(max(abs(u).max(), abs(v).max())) * wn[-1] / (2 * math.pi)
+    # sigma^2 = SEFD^2 / (2 t dnu eta^2) as float (RadioLensfit2.cpp:255)
+    assert s2[0] == np.float32(400e6 ** 2 / (2 * 360 * 60e6 * 0.81))
+
+
+def test_simulation_is_linear_and_noise_has_sigma():
+    cfg = synth.small_array(n_dish=8, n_times=10)
+    cat = synth.make_catalog(20, seed=2)
+    a = synth.make_observation(cfg, synth.Catalog(*(getattr(cat, f)[:10].copy() for f in (
+        "flux", "scale", "e1", "e2", "l", "m", "snr"))), noise=False, ms_roundtrip=False)
+    b = synth.make_observation(cfg, synth.Catalog(*(getattr(cat, f)[10:].copy() for f in (
+        "flux", "scale", "e1", "e2", "l", "m", "snr"))), noise=False, ms_roundtrip=False)
+    ab = synth.make_observation(cfg, cat, noise=False, ms_roundtrip=False)
+    np.testing.assert_allclose(ab.vis, a.vis + b.vis, rtol=1e-12, atol=1e-9)
+    cfg2 = synth.small_array(n_dish=30, n_times=40)
+    n = synth.make_observation(cfg2, None, noise=True, ms_roundtrip=False)
+    sd = math.sqrt(n.sigma2[0])
+    assert np.std(n.vis.real) == pytest.approx(sd, rel=0.03)
+    assert np.std(n.vis.imag) == pytest.approx(sd, rel=0.03)
+
+
+def test_rlms_roundtrip(tmp_path):
+    cfg = synth.small_array(n_dish=6, n_times=4, n_chan=3)
+    cat = synth.make_catalog(10, seed=5)
+    obs = synth.make_observation(cfg, cat, ms_roundtrip=False)
+    nrow, nch = obs.nrow, obs.nch
+    vis_rc = obs.vis.reshape(nch, nrow).T  # row-major (row, channel) like the MS DATA column
+    data = np.empty((nrow, nch, 2), np.float32)
+    data[..., 0] = (vis_rc.real * 1e-6).astype(np.float32)
+    data[..., 1] = (vis_rc.imag * 1e-6).astype(np.float32)
+    flag_rc = np.zeros((nrow, nch), np.uint8)
+    flag_rc[::7, 1] = 1
+    sig = np.full(nrow, 2.5, np.float32)
+    p = str(tmp_path / "obs.rlms").encode()
+    S = N.sim()
+    assert S.rl2ms_write(p, 6, nch, nrow, 950e6, 60e6, 360.0, N.dptr(obs.u), N.dptr(obs.v),
+                         N.dptr(obs.w), flag_rc.ctypes.data_as(N.c_ubyte_p),
+                         data.ctypes.data_as(N.c_float_p), N.fptr(sig)) == 0
+    u, v, w = np.zeros(nrow), np.zeros(nrow), np.zeros(nrow)
+    fl = np.zeros(nrow * nch, np.uint8)
+    vis = np.zeros(nrow * nch, np.complex128)
+    s2 = np.zeros(nrow * nch, np.float32)
+    st = C.c_int()
+    umax = S.rl2ms_read(p, N.dptr(u), N.dptr(v), N.dptr(w), fl.ctypes.data_as(N.c_ubyte_p),
+                        N.cplx(vis), N.fptr(s2), C.byref(st))
+    assert st.value == 0 and umax == math.ceil(max(abs(obs.u).max(), abs(obs.v).max()))
+    np.testing.assert_array_equal(u, obs.u)
+    np.testing.assert_array_equal(fl.reshape(nch, nrow).T, flag_rc)
+    want = (data[..., 0].astype(np.float64) * 1e6 + 1j * data[..., 1].astype(np.float64) * 1e6)
+    np.testing.assert_array_equal(vis.reshape(nch, nrow).T, want)
+    assert (s2[:nrow] == np.float32(6.25)).all() and (s2[nrow:] == 0).all()
