@@ -3,23 +3,25 @@
 
 Workload (default): 197 dishes (dense 1 km core, log-radial to 150 km), 19,306 baselines x
 480 times x 16 channels of 15 MHz from 950 MHz = 148.3M visibilities with float64 uvw;
-1,000 exponential galaxies (seeded catalog, shear g = (0.03, 0)), simulated on the device
+1,000 exponential galaxies (seeded catalog, shear g = (0.03, 0)) simulated on the device
 (sheared models + Gaussian noise), sky model of round models subtracted.  The per-galaxy
 path is RadioLensfit2's: extract -> fit (simplex + sampling) -> subtract, galaxies in
 decreasing flux order, failed fits re-fitted after the others.
 
-A step is one batch of --batch galaxies through that path.  --warmup steps run first
-(untimed), then exactly --steps are timed (their bad-list re-fit included), inputs
-resident in HBM.  One JSON line is printed.  A CPU baseline (the FP64 oracle, OpenMP over
-channels / scalelengths, on the host cores) processes the first --cpu-sample galaxies of the
-timed set from the same residual; their ellipticities also give a full-size parity figure.
+A step is one batch of --batch galaxies through that path.  The residual is checkpointed
+after set-up; --warmup steps run and the checkpoint is restored, so every configuration
+times exactly --steps steps over the same first steps*batch galaxies (their bad-list
+re-fit included), inputs resident in HBM.  One JSON line is printed.
+
+Correctness figures: --cpu-sample galaxies re-run on the FP64 CPU oracle from the same
+residual (CPU baseline + parity); --compare-fp64 re-runs the timed galaxies with the FP64
+path on the GPU and reports the fraction within |de| <= 1e-4.
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
-import math
 import os
 import sys
 import time
@@ -33,7 +35,7 @@ from radiolensfit2_amd import _native as N  # noqa: E402
 from radiolensfit2_amd import pipeline as P  # noqa: E402
 from radiolensfit2_amd import synth  # noqa: E402
 
-METRIC = ("galaxies/s @|Δe|≤1e-4 vs FP64: learned+batched vs plain GPU; HBM GB/s")
+METRIC = "galaxies/s @|Δe|≤1e-4 vs FP64: learned+batched vs plain GPU; HBM GB/s"
 PEAK_HBM_GBS = 8000.0
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
@@ -48,63 +50,107 @@ def subset(cat, lo, hi):
                            ("flux", "scale", "e1", "e2", "l", "m", "snr")))
 
 
-def make_workload(cfg, ngal, precision, seed=1, shear=(0.03, 0.0)):
+def make_geometry_and_catalog(cfg, ngal, seed=1, shear=(0.03, 0.0)):
     t0 = time.time()
     u, v, w, wn, spec, sigma2, flag, len_lambda = synth.make_geometry(cfg)
     obs = synth.Observation(cfg, u, v, w, wn, spec, sigma2, flag, len_lambda,
                             np.zeros(0, np.complex128))
     cat = synth.make_catalog(ngal, seed=seed, g=shear)
     log(f"[bench] geometry + catalog: {time.time() - t0:.1f}s nrow={obs.nrow} nvis={obs.nvis}")
+    return obs, cat
+
+
+def make_device_observation(obs, cat, res_prec):
+    """Observation in HBM; data = sheared models + noise, residual = data - round models."""
     t0 = time.time()
     H = N.hip()
     N.check(H.rl2_init(0), "rl2_init")
     h = C.c_void_p()
-    N.check(H.rl2_obs_create(C.byref(h), obs.nch, obs.nrow, N.dptr(u), N.dptr(v), N.dptr(w),
-                             N.dptr(wn), N.dptr(spec), N.fptr(sigma2), None, len_lambda,
-                             precision), "rl2_obs_create")
+    N.check(H.rl2_obs_create(C.byref(h), obs.nch, obs.nrow, N.dptr(obs.u), N.dptr(obs.v),
+                             N.dptr(obs.w), N.dptr(obs.wn), N.dptr(obs.spec),
+                             N.fptr(obs.sigma2), None, obs.len, res_prec), "rl2_obs_create")
     dev = P.DeviceObservation.__new__(P.DeviceObservation)
-    dev.obs, dev.precision, dev.h = obs, precision, h
-    # data = sum of sheared models + noise; residual = data - sum of round models
+    dev.obs, dev.precision, dev.h = obs, res_prec, h
     dev.add_sources(cat, 1.0)
-    dev.add_noise(float(sigma2[0]), 12345)
+    dev.add_noise(float(obs.sigma2[0]), 12345)
     dev.add_sources(cat, -1.0, round_models=True)
-    N.check(H.rl2_synchronize(), "sync")
+    dev.checkpoint()
     log(f"[bench] device simulation + sky model: {time.time() - t0:.1f}s")
-    return obs, cat, dev
+    return dev
 
 
-def cpu_baseline(obs, cat, resid, lo, n):
-    """FP64 oracle on galaxies [lo, lo+n) from the residual the GPU started from."""
+def run_path(dev, cat, n_gal, warmup_gal, opts, timed=True):
+    if warmup_gal:
+        t0 = time.time()
+        dev.process(subset(cat, 0, warmup_gal), **opts)
+        dev.restore()
+        log(f"[bench] warmup ({warmup_gal} galaxies): {time.time() - t0:.2f}s")
+    sub = subset(cat, 0, n_gal)
+    if timed:
+        P.stats_enable(True)
+        P.stats_reset()
+    N.check(N.hip().rl2_synchronize(), "sync")
+    t0 = time.perf_counter()
+    res, st = dev.process(sub, **opts)
+    N.check(N.hip().rl2_synchronize(), "sync")
+    elapsed = time.perf_counter() - t0
+    kstats = P.stats() if timed else None
+    if timed:
+        P.stats_enable(False)
+    return res, st, elapsed, kstats
+
+
+def cpu_baseline(obs, cat, resid, n):
+    """FP64 oracle on galaxies [0, n) from the same initial residual."""
     from oracle import oracle as O
     L = O.lib()
-    sub = subset(cat, lo, lo + n)
-    vd = resid
-    vs = np.zeros_like(vd)
+    sub = subset(cat, 0, n)
+    vs = np.zeros_like(resid)
     res = (O.GalResult * n)()
     st = O.RunStats()
     t0 = time.time()
     L.orl2_data_processing(n, 1, obs.len, obs.nch, O.ptr(obs.wn), O.ptr(obs.spec), obs.nrow,
                            O.ptr(obs.u), O.ptr(obs.v), O.ptr(obs.w),
-                           O.ptr(obs.sigma2, O.c_float_p), None, O.cptr(vd), O.cptr(vs),
+                           O.ptr(obs.sigma2, O.c_float_p), None, O.cptr(resid), O.cptr(vs),
                            O.ptr(sub.l), O.ptr(sub.m), O.ptr(sub.flux), O.ptr(sub.scale), res, 1,
                            C.byref(st))
-    dt = time.time() - t0
-    return dt, [(r.e1, r.e2, r.error) for r in res]
+    return time.time() - t0, [(r.e1, r.e2, r.error) for r in res]
+
+
+def roofline(kstats, lik_peak):
+    lik, ext = kstats["likelihood"], kstats["extract"]
+    lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
+    ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
+    r_lik = {"kernel": "likelihood_kernel", "bound": "valu", "achieved": round(lik_tf, 4),
+             "peak": lik_peak, "unit": "TFLOP/s", "frac": round(lik_tf / lik_peak, 5),
+             "traffic": None, "launches": lik["launches"],
+             "avg_launch_us": round(1e3 * lik["ms"] / max(1, lik["launches"]), 3)}
+    r_ext = {"kernel": "grid_kernel (extraction)", "bound": "hbm", "achieved": round(ext_gbs, 2),
+             "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ext_gbs / PEAK_HBM_GBS, 5),
+             "traffic": None, "launches": ext["launches"],
+             "avg_launch_us": round(1e3 * ext["ms"] / max(1, ext["launches"]), 3)}
+    return (r_lik if lik["ms"] >= ext["ms"] else r_ext), r_lik, r_ext
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--residual", choices=["c128", "c64"], default="c128",
+                    help="residual visibilities in HBM: complex128 or complex64")
+    ap.add_argument("--lik-precision", type=int, choices=[0, 1, 2], default=0,
+                    help="0 FP64; 1 mixed throughout; 2 mixed simplex + FP64 sampling")
     ap.add_argument("--config", choices=["197dish", "64dish"], default="197dish")
     ap.add_argument("--ngal", type=int, default=1000)
     ap.add_argument("--cpu-sample", type=int, default=1,
-                    help="galaxies of the timed set re-run on the CPU oracle (0: skip)")
+                    help="galaxies re-run on the FP64 CPU oracle (0: skip)")
+    ap.add_argument("--compare-fp64", action="store_true",
+                    help="re-run the timed galaxies with the FP64 GPU path (untimed)")
     ap.add_argument("--no-speculate", action="store_true")
     ap.add_argument("--line-chunk", type=int, default=4)
+    ap.add_argument("--lik-block", type=int, default=512)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -113,63 +159,59 @@ def main():
         sys.exit(2)
 
     cfg = synth.ARRAY_197 if args.config == "197dish" else synth.ARRAY_64
-    prec = N.RL2_PREC_FP64 if args.precision == "fp64" else N.RL2_PREC_MIXED
+    res_prec = N.RL2_PREC_FP64 if args.residual == "c128" else N.RL2_PREC_MIXED
     B = args.batch
-    need = (args.warmup + args.steps) * B
-    if need > args.ngal:
-        sys.exit(f"warmup+steps need {need} galaxies, catalog has {args.ngal}")
-    obs, cat, dev = make_workload(cfg, args.ngal, prec)
+    n_gal = args.steps * B
+    if max(n_gal, args.warmup * B) > args.ngal:
+        sys.exit(f"steps*batch = {n_gal} galaxies, catalog has {args.ngal}")
+    obs, cat = make_geometry_and_catalog(cfg, args.ngal)
+    dev = make_device_observation(obs, cat, res_prec)
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
-                line_chunk=args.line_chunk)
-
-    if args.warmup:
-        t0 = time.time()
-        dev.process(subset(cat, 0, args.warmup * B), **opts)
-        log(f"[bench] warmup {args.warmup} steps: {time.time() - t0:.2f}s")
-    lo, hi = args.warmup * B, need
-    resid = dev.residual() if args.cpu_sample > 0 else None
-    timed = subset(cat, lo, hi)
-    P.stats_enable(True)
-    P.stats_reset()
-    N.check(N.hip().rl2_synchronize(), "sync")
-    t0 = time.perf_counter()
-    res, st = dev.process(timed, **opts)
-    N.check(N.hip().rl2_synchronize(), "sync")
-    elapsed = time.perf_counter() - t0
-    kstats = P.stats()
-    P.stats_enable(False)
-    n_gal = hi - lo
+                line_chunk=args.line_chunk, lik_block=args.lik_block,
+                lik_precision=args.lik_precision)
+    resid0 = dev.residual() if args.cpu_sample > 0 else None
+    res, st, elapsed, kstats = run_path(dev, cat, n_gal, args.warmup * B, opts)
     value = n_gal / elapsed
-    log(f"[bench] timed {args.steps} steps ({n_gal} galaxies): {elapsed:.3f}s -> {value:.2f} gal/s;"
-        f" bad first/final {st.n_bad_first}/{st.n_bad_final}")
+    log(f"[bench] timed {args.steps} steps ({n_gal} galaxies): {elapsed:.3f}s -> "
+        f"{value:.2f} gal/s; bad first/final {st.n_bad_first}/{st.n_bad_final}")
     for k, s in kstats.items():
         log(f"[bench]   {k:10s} {s['ms']:10.2f} ms  {s['launches']:7d} launches")
+    mixed = args.lik_precision != 0 or res_prec != N.RL2_PREC_FP64
+    peak = PEAK_FP64_TFLOPS if args.lik_precision == 0 else PEAK_FP32_TFLOPS
+    dominant, r_lik, r_ext = roofline(kstats, peak)
 
-    # rooflines from HIP-event kernel times measured on the library's stream
-    lik = kstats["likelihood"]
-    ext = kstats["extract"]
-    peak_valu = PEAK_FP64_TFLOPS if prec == N.RL2_PREC_FP64 else PEAK_FP32_TFLOPS
-    lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
-    ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
-    roof_lik = {"kernel": "likelihood_kernel", "bound": "valu", "achieved": round(lik_tf, 4),
-                "peak": peak_valu, "unit": "TFLOP/s", "frac": round(lik_tf / peak_valu, 5),
-                "traffic": None, "launches": lik["launches"],
-                "avg_launch_us": round(1e3 * lik["ms"] / max(1, lik["launches"]), 3)}
-    roof_ext = {"kernel": "grid_kernel (extraction)", "bound": "hbm", "achieved": round(ext_gbs, 2),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ext_gbs / PEAK_HBM_GBS, 5),
-                "traffic": None, "launches": ext["launches"],
-                "avg_launch_us": round(1e3 * ext["ms"] / max(1, ext["launches"]), 3)}
-    dominant = roof_lik if lik["ms"] >= ext["ms"] else roof_ext
+    accept = None
+    if args.compare_fp64:
+        if mixed:
+            if res_prec != N.RL2_PREC_FP64:
+                dev.close()
+                dev64 = make_device_observation(obs, cat, N.RL2_PREC_FP64)
+            else:
+                dev64 = dev
+                dev64.restore()
+            ref, _, t64, _ = run_path(dev64, cat, n_gal, 0, dict(opts, lik_precision=0),
+                                      timed=False)
+            de = np.array([max(abs(res[k].e1 - ref[k].e1), abs(res[k].e2 - ref[k].e2))
+                           for k in range(n_gal)])
+            same_flag = np.array([res[k].error == ref[k].error for k in range(n_gal)])
+            ok = (de <= 1e-4) & same_flag
+            accept = {"reference": "FP64 GPU path (= FP64 oracle to 1e-12), same batch",
+                      "galaxies": n_gal, "within_1e-4": int(ok.sum()),
+                      "fraction": round(float(ok.mean()), 4),
+                      "max_abs_de": float(de.max()), "median_abs_de": float(np.median(de)),
+                      "fp64_gpu_galaxies_per_s": round(n_gal / t64, 4)}
+            log(f"[bench] vs FP64: {accept}")
+        else:
+            accept = {"reference": "this run is the FP64 path", "fraction": 1.0}
 
-    cpu = None
-    parity = None
+    cpu = parity = None
     if args.cpu_sample > 0:
         ncpu = min(args.cpu_sample, n_gal)
-        dt, ores = cpu_baseline(obs, cat, resid, lo, ncpu)
+        dt, ores = cpu_baseline(obs, cat, resid0, ncpu)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = {"value": round(ncpu / dt, 5), "unit": "galaxies/s", "cores": cores, "kind": "port",
-               "sample": f"{ncpu} galaxies (the first of the timed set: extraction, fit, "
-                         f"subtraction, bad-list re-fit) on the FP64 oracle, {dt:.1f}s"}
+               "sample": f"first {ncpu} galaxies of the timed set (extraction, fit, subtraction, "
+                         f"bad-list re-fit) on the FP64 oracle, {dt:.1f}s"}
         de = [max(abs(res[k].e1 - ores[k][0]), abs(res[k].e2 - ores[k][1])) for k in range(ncpu)
               if res[k].error == ores[k][2] == 0]
         parity = {"galaxies": ncpu, "max_abs_de_vs_fp64_oracle": (max(de) if de else None),
@@ -187,25 +229,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == N.RL2_PREC_FP64 else "f32+f64",
+        "dtype": "f64" if not mixed else "f32+f64",
         "data": "synthetic",
         "config": {
             "workload": (f"{cfg.n_dish} dishes, {obs.nrow // cfg.n_times} baselines x "
                          f"{cfg.n_times} times x {cfg.n_chan} ch ({obs.nvis / 1e6:.1f}M vis), "
-                         f"{args.ngal}-galaxy catalog; timed galaxies {lo}..{hi - 1}"),
+                         f"{args.ngal}-galaxy catalog; timed galaxies 0..{n_gal - 1}"),
             "batch": B,
-            "path": ("plain" if B == 1 else "batched") + "-" + args.precision,
+            "path": ("plain" if B == 1 else "batched") + ("-mixed" if mixed else "-fp64"),
+            "residual": args.residual,
+            "lik_precision": args.lik_precision,
             "parallelism": "1 GPU",
         },
         "roofline": dominant,
-        "roofline_likelihood": roof_lik,
-        "roofline_extraction": roof_ext,
+        "roofline_likelihood": r_lik,
+        "roofline_extraction": r_ext,
         "cpu_baseline": cpu,
         "parity": parity,
+        "acceptance_vs_fp64": accept,
         "bad_fits": {"first_pass": st.n_bad_first, "after_refit": st.n_bad_final},
     }
     print(json.dumps(out), flush=True)
-    dev.close()
 
 
 if __name__ == "__main__":

@@ -69,6 +69,8 @@ typedef struct {
     int line_chunk;        /* line-search steps evaluated per launch (default 4) */
     int lik_block;         /* threads per likelihood work-group (default 256) */
     int verbose;
+    int lik_precision;     /* 0: FP64 likelihood throughout; 1: mixed-precision likelihood
+                              throughout; 2: mixed for the simplex, FP64 for the sampling */
 } rl2_fit_options;
 
 const char *rl2_last_error(void);
@@ -98,6 +100,9 @@ int rl2_obs_add_sources(rl2_obs *obs, unsigned int ngal, const double *flux,
                         const double *e1, const double *e2, double sign);
 /* R += complex Gaussian noise of variance sigma2 per component (counter-based stream). */
 int rl2_obs_add_noise(rl2_obs *obs, double sigma2, uint64_t seed);
+/* Keep a device copy of the residual / put it back (repeatable runs over the same state). */
+int rl2_obs_checkpoint(rl2_obs *obs);
+int rl2_obs_restore(rl2_obs *obs);
 
 /* ---- one galaxy at a time (the serial path of data_processing.c) ---- */
 /* source_extraction + evaluate_facet_coords for one galaxy into facet slot `slot`.
@@ -116,6 +121,8 @@ int rl2_obs_set_prior(rl2_obs *obs, int slot, const double *rprior /* numr */);
  * n_iter (npts*numr, may be NULL) the Newton iterations of cross_correlation. */
 int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const double *e2,
                    double *L_e, int *error, double *L_r, int *n_iter);
+/* Precision of rl2_obs_loglik evaluations: 0 FP64 (default), 1 mixed. */
+int rl2_obs_loglik_precision(rl2_obs *obs, int prec);
 /* source_fitting (galaxy_fitting.c:41-117) for the facet in `slot`. */
 int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_result *res);
 /* Remove a fitted galaxy: visSkyMod -= round model, visData -= fitted model

@@ -45,7 +45,8 @@ class RunStats(C.Structure):
 
 class FitOptions(C.Structure):
     _fields_ = [("batch", C.c_int), ("do_refit", C.c_int), ("nm_speculate", C.c_int),
-                ("line_chunk", C.c_int), ("lik_block", C.c_int), ("verbose", C.c_int)]
+                ("line_chunk", C.c_int), ("lik_block", C.c_int), ("verbose", C.c_int),
+                ("lik_precision", C.c_int)]
 
 
 class KernelStat(C.Structure):
@@ -90,6 +91,9 @@ _HIP_SIGS = {
     "rl2_obs_add_sources": (C.c_int, [C.c_void_p, C.c_uint, c_double_p, c_double_p, c_double_p,
                                       c_double_p, c_double_p, c_double_p, C.c_double]),
     "rl2_obs_add_noise": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64]),
+    "rl2_obs_checkpoint": (C.c_int, [C.c_void_p]),
+    "rl2_obs_restore": (C.c_int, [C.c_void_p]),
+    "rl2_obs_loglik_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "rl2_obs_extract": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
                                   C.c_double, C.c_uint, c_double_p, c_double_p, c_double_p,
                                   c_double_p, c_double_p, c_uint_p]),

@@ -57,6 +57,8 @@ struct Ctx {
     bool sampling_only = false;
     int np_max = kNpMax;
     int n_evals = 0;
+    int prec_nm = 0, prec_samp = 0;  // likelihood precision of the simplex / sampling phase
+    int cur_prec = 0;                // precision of the pending requests
 };
 
 struct Await {
@@ -164,6 +166,7 @@ Task fit_task(Ctx *c)
         maxL = R.maxL;
         goto sampling;
     }
+    c->cur_prec = c->prec_nm;
     {
     // ---- gsl_multimin_fminimizer_set (nmsimplex_set) ----
     Simplex s;
@@ -303,6 +306,7 @@ Task fit_task(Ctx *c)
     if (mes_e1 * mes_e1 + mes_e2 * mes_e2 > 0.65) maxL = -1e+10;
     }
 sampling:
+    c->cur_prec = c->prec_samp;
     double var_e1 = 0., var_e2 = 0., oneDimvar = 0.;
     int error = 0;
     if (maxL > -1.e+10 || c->sampling_only) {
@@ -466,6 +470,8 @@ bool fit_batch(const std::vector<FitRequest> &reqs, const rl2_fit_options &opt,
         ctx[i].line_chunk = opt.line_chunk > 0 ? opt.line_chunk : 4;
         ctx[i].sampling_only = reqs[i].sampling_only;
         ctx[i].np_max = reqs[i].np_max;
+        ctx[i].prec_nm = (opt.lik_precision == 1 || opt.lik_precision == 2) ? 1 : 0;
+        ctx[i].prec_samp = (opt.lik_precision == 1) ? 1 : 0;
         tasks.push_back(fit_task(&ctx[i]));
     }
     for (auto &t : tasks) t.h.resume();
@@ -476,7 +482,8 @@ bool fit_batch(const std::vector<FitRequest> &reqs, const rl2_fit_options &opt,
         items.clear();
         for (size_t i = 0; i < nb; i++)
             if (!tasks[i].h.done() && ctx[i].pending)
-                for (const Pt &p : *ctx[i].pending) items.push_back({ctx[i].slot, 0, p.e1, p.e2});
+                for (const Pt &p : *ctx[i].pending)
+                    items.push_back({ctx[i].slot, ctx[i].cur_prec, p.e1, p.e2});
         if (items.empty()) break;
         Le.assign(items.size(), 0.);
         err.assign(items.size(), 0);

@@ -20,6 +20,8 @@ struct FacetDev {
     const double *dre, *dim;   // gridded data / variance
     const double *s2;          // gridded variance
     const double *mod;         // optional given model (drop-in cross_correlation); else null
+    const float4 *pk;          // FP32 copy {u, v, D_re, D_im} for the mixed-precision kernel
+    const float *invs2;        // FP32 1 / sigma^2
     unsigned int n;
     unsigned int pad;
     double rprior[kMaxR];      // log prior per scalelength index
@@ -28,7 +30,7 @@ struct FacetDev {
 // One likelihood evaluation request.
 struct LikItem {
     int slot;
-    int pad;
+    int prec;                  // 0: FP64 kernel, 1: mixed-precision kernel
     double e1, e2;
 };
 
@@ -84,6 +86,10 @@ void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, doub
                     double *vv, double *dre, double *dim, double *s2, unsigned *n_out,
                     double *sum_w_dense, unsigned *scratch_blocks);
 size_t compact_scratch_words(unsigned facet);
+// FP32 packing of a compacted facet (count read from the device descriptor).
+void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
+                       const double *vv, const double *dre, const double *dim, const double *s2,
+                       float4 *pk, float *invs2);
 
 // Likelihood: per (item, r) work-group L_r and Newton iterations; then r-marginalisation.
 struct LikLaunch {
@@ -101,6 +107,7 @@ struct LikLaunch {
     int prec;
     double *ho = nullptr;      // optional nitems * kMaxR: cross-correlation peak value
     double *det = nullptr;     // optional nitems * kMaxR: Hessian determinant at the peak
+    const int *host_prec = nullptr;  // host copy of items[].prec (selects the kernels)
 };
 constexpr int kQagLimit = 1000;
 constexpr int kQagDoubles = 4 * kQagLimit;

@@ -426,6 +426,28 @@ __device__ inline void block_allreduce(double (&v)[N], double *lds)
     __syncthreads();
 }
 
+// likelihood.c:174-183: L_r = log of the position-marginalised likelihood minus
+// 1/2 log det of the cross-correlation Hessian; -1e10 when the peak search failed.
+__device__ inline void finish_lr(const LikLaunch &a, int item, int r, double ho,
+                                 double det_sigma, int iter)
+{
+    double L;
+    if (det_sigma <= 0)
+        L = kBad;
+    else {
+        const double B = (ho * ho) * 0.5;
+        L = marg_position(B);
+        if (isnan(L))
+            L = kBad;
+        else
+            L -= 0.5 * log(det_sigma);
+    }
+    a.Lr[(size_t)item * kMaxR + r] = L;
+    a.nit[(size_t)item * kMaxR + r] = iter;
+    if (a.ho) a.ho[(size_t)item * kMaxR + r] = ho;
+    if (a.det) a.det[(size_t)item * kMaxR + r] = det_sigma;
+}
+
 template <int BS, int PPT>
 __global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
 {
@@ -434,6 +456,7 @@ __global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
     const int item = blockIdx.x / nr;
     const int r = blockIdx.x % nr + 1;
     const LikItem it = a.items[item];
+    if (it.prec != 0) return;  // evaluated by the mixed-precision kernel
     const FacetDev &F = a.facets[it.slot];
     const unsigned n = F.n;
     const double e1 = it.e1, e2 = it.e2;
@@ -441,6 +464,7 @@ __global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
     const double scale = arcs2rad_scale(a.rg.ro[r]);
     const double sf = (scale * scale) / (detA * detA);
     const double cc2 = 4 * kPi * kPi;
+    const double sw = sf * cc2;
     const double a1 = 1. + e1, a2 = 1. - e1;
     const double *__restrict__ gu = F.uu;
     const double *__restrict__ gv = F.vv;
@@ -454,11 +478,14 @@ __global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
     // cache is not demoted to scratch memory); cells beyond are recomputed per pass.
     double hrc[PPT > 0 ? PPT : 1], hic[PPT > 0 ? PPT : 1];
     double s[7] = {0., 0., 0., 0., 0., 0., 0.};  // S2, value, dx, dy, dxx, dyy, dxy
+    // (1 + s w |k|^2)^(-3/2) as rsqrt(den)^3: one reciprocal square root instead of a
+    // square root and a division (a few ulp from the reference's 1/(den*sqrt(den))).
     auto model_at = [&](unsigned i, double uu, double vv) -> double {
         if (gmod) return gmod[i];
         const double k1 = a1 * uu + e2 * vv;
         const double k2 = e2 * uu + a2 * vv;
-        return sersic_shape(k1, k2, sf, cc2);
+        const double rr = rsqrt(1. + sw * (k1 * k1 + k2 * k2));
+        return rr * rr * rr;
     };
     auto pass0 = [&](unsigned i, double &hr, double &hi) {
         const double uu = gu[i], vv = gv[i];
@@ -531,39 +558,211 @@ __global__ __launch_bounds__(BS) void likelihood_kernel(LikLaunch a)
         det = dxx * dyy - dxy * dxy;
         norm = sqrt(incx * incx + incy * incy);
     }
-    if (tid == 0) {
-        const double ho = gmod ? value : value / sqrt(S2);
-        const double det_sigma = gmod ? det : det / S2;
-        double L;
-        if (det_sigma <= 0)
-            L = kBad;
-        else {
-            const double B = (ho * ho) * 0.5;
-            L = marg_position(B);
-            if (isnan(L))
-                L = kBad;
-            else
-                L -= 0.5 * log(det_sigma);
+    if (tid == 0)
+        finish_lr(a, item, r, gmod ? value : value / sqrt(S2), gmod ? det : det / S2, iter);
+}
+
+// ------------------------------------------------------------------ likelihood, mixed
+// FP32 model and cross-correlation terms from the packed facet {u, v, D_re, D_im}; the
+// Newton phase x.u + y.v is formed and reduced in FP64 (revolutions), sin/cos evaluated
+// in FP32 on the reduced octant; per-thread partial sums are flushed to FP64 every
+// kFlush cells; the Newton update, position and scalelength marginalisation run in FP64.
+constexpr int kFlush = 8;
+
+// sin(2 pi t), cos(2 pi t): t reduced to [-1/2, 1/2] in FP64, then to an octant
+// |y| <= 1/8 in FP32; Taylor series of sin / cos of 2 pi y to degree 9 / 10
+// (truncation < 3e-9 on the octant).
+__device__ inline void sincos_rev(double t, float &sn, float &cs)
+{
+    const float x = (float)(t - rint(t));
+    const float q = rintf(4.f * x);
+    const float y = x - 0.25f * q;
+    const float th = 6.28318530717958648f * y;
+    const float z = th * th;
+    const float s = th * (1.f + z * (-1.f / 6 + z * (1.f / 120 + z * (-1.f / 5040 +
+                                                                  z * (1.f / 362880)))));
+    const float c = 1.f + z * (-0.5f + z * (1.f / 24 + z * (-1.f / 720 + z * (1.f / 40320 +
+                                                                          z * (-1.f / 3628800)))));
+    const int k = ((int)q) & 3;
+    sn = (k == 0) ? s : (k == 1) ? c : (k == 2) ? -s : -c;
+    cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
+}
+
+template <int BS, int PPT>
+__global__ __launch_bounds__(BS) void likelihood32_kernel(LikLaunch a)
+{
+    __shared__ double red[(BS / 64) * 8];
+    const int nr = a.rg.numr - 1;
+    const int item = blockIdx.x / nr;
+    const int r = blockIdx.x % nr + 1;
+    const LikItem it = a.items[item];
+    if (it.prec != 1) return;
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned n = F.n;
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(a.rg.ro[r]);
+    const double sf = (scale * scale) / (detA * detA);
+    const float sw = (float)(sf * (4 * kPi * kPi));
+    const float a1 = (float)(1. + e1), a2 = (float)(1. - e1), fe2 = (float)e2;
+    const float tp = 6.28318530717958648f;
+    const float4 *__restrict__ pk = F.pk;
+    const float *__restrict__ is2 = F.invs2;
+    const int tid = threadIdx.x;
+
+    float hrc[PPT > 0 ? PPT : 1], hic[PPT > 0 ? PPT : 1];
+    double s[7] = {0., 0., 0., 0., 0., 0., 0.};
+    float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int cnt = 0;
+    auto model = [&](const float4 p) -> float {
+        const float k1 = a1 * p.x + fe2 * p.y;
+        const float k2 = fe2 * p.x + a2 * p.y;
+        const float rr = rsqrtf(1.f + sw * (k1 * k1 + k2 * k2));
+        return rr * rr * rr;
+    };
+    auto flush7 = [&]() {
+#pragma unroll
+        for (int j = 0; j < 7; j++) {
+            s[j] += (double)f[j];
+            f[j] = 0.f;
         }
-        a.Lr[(size_t)item * kMaxR + r] = L;
-        a.nit[(size_t)item * kMaxR + r] = iter;
-        if (a.ho) a.ho[(size_t)item * kMaxR + r] = ho;
-        if (a.det) a.det[(size_t)item * kMaxR + r] = det_sigma;
+    };
+    auto pass0 = [&](unsigned i, float &hr, float &hi) {
+        const float4 p = pk[i];
+        const float M = model(p);
+        hr = p.z * M;
+        hi = -p.w * M;
+        const float U = tp * p.x, V = tp * p.y;
+        f[0] += M * M * is2[i];
+        f[1] += hr;
+        f[2] -= hi * U;
+        f[3] -= hi * V;
+        f[4] -= hr * U * U;
+        f[5] -= hr * V * V;
+        f[6] -= hr * U * V;
+        if (++cnt == kFlush) {
+            flush7();
+            cnt = 0;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < PPT; k++) {
+        const unsigned i = tid + k * BS;
+        hrc[k] = 0.f;
+        hic[k] = 0.f;
+        if (i < n) pass0(i, hrc[k], hic[k]);
     }
+    for (unsigned i = tid + PPT * BS; i < n; i += BS) {
+        float hr, hi;
+        pass0(i, hr, hi);
+    }
+    flush7();
+    block_allreduce<BS, 7>(s, red);
+    const double S2 = s[0];
+    double value = s[1], dx = s[2], dy = s[3], dxx = s[4], dyy = s[5], dxy = s[6];
+    double det = dxx * dyy - dxy * dxy;
+    double xo = 0., yo = 0., norm = 1.;
+    int iter = 0;
+    while (det > 0 && norm > 1.e-7 && iter < 50) {
+        iter++;
+        const double incx = (dx * dyy - dy * dxy) / det;
+        const double incy = (dy * dxx - dx * dxy) / det;
+        xo -= incx;
+        yo -= incy;
+        double t[6] = {0., 0., 0., 0., 0., 0.};
+        float g[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        cnt = 0;
+        auto newton_term = [&](const float4 p, float hr, float hi) {
+            float sn, cs;
+            sincos_rev(xo * (double)p.x + yo * (double)p.y, sn, cs);
+            const float U = tp * p.x, V = tp * p.y;
+            const float re = hr * cs - hi * sn;
+            const float im = hr * sn + hi * cs;
+            g[0] += re;
+            g[1] -= im * U;
+            g[2] -= im * V;
+            g[3] -= re * U * U;
+            g[4] -= re * V * V;
+            g[5] -= re * U * V;
+            if (++cnt == kFlush) {
+#pragma unroll
+                for (int j = 0; j < 6; j++) {
+                    t[j] += (double)g[j];
+                    g[j] = 0.f;
+                }
+                cnt = 0;
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < PPT; k++) {
+            const unsigned i = tid + k * BS;
+            if (i < n) newton_term(pk[i], hrc[k], hic[k]);
+        }
+        for (unsigned i = tid + PPT * BS; i < n; i += BS) {
+            const float4 p = pk[i];
+            const float M = model(p);
+            newton_term(p, p.z * M, -p.w * M);
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++) t[j] += (double)g[j];
+        block_allreduce<BS, 6>(t, red);
+        value = t[0];
+        dx = t[1];
+        dy = t[2];
+        dxx = t[3];
+        dyy = t[4];
+        dxy = t[5];
+        det = dxx * dyy - dxy * dxy;
+        norm = sqrt(incx * incx + incy * incy);
+    }
+    if (tid == 0) finish_lr(a, item, r, value / sqrt(S2), det / S2, iter);
 }
 
 void launch_likelihood(hipStream_t s, const LikLaunch &a)
 {
     if (a.nitems <= 0) return;
     dim3 grid((unsigned)(a.nitems * (a.rg.numr - 1)));
-    switch (a.block) {
-    // PPT: register-cached cells per thread; larger caches spill FP64 sincos state.
-    case 64: likelihood_kernel<64, 4><<<grid, 64, 0, s>>>(a); break;
-    case 128: likelihood_kernel<128, 4><<<grid, 128, 0, s>>>(a); break;
-    case 512: likelihood_kernel<512, 0><<<grid, 512, 0, s>>>(a); break;
-    case 1024: likelihood_kernel<1024, 0><<<grid, 1024, 0, s>>>(a); break;
-    default: likelihood_kernel<256, 4><<<grid, 256, 0, s>>>(a); break;
+    bool any64 = false, any32 = false;
+    for (int i = 0; i < a.nitems; i++) {  // host copy of the request precisions
+        if (a.host_prec && a.host_prec[i]) any32 = true; else any64 = true;
     }
+    if (any64) {
+        switch (a.block) {
+        // PPT: register-cached cells per thread; larger caches spill FP64 sincos state.
+        case 64: likelihood_kernel<64, 4><<<grid, 64, 0, s>>>(a); break;
+        case 128: likelihood_kernel<128, 4><<<grid, 128, 0, s>>>(a); break;
+        case 512: likelihood_kernel<512, 0><<<grid, 512, 0, s>>>(a); break;
+        case 1024: likelihood_kernel<1024, 0><<<grid, 1024, 0, s>>>(a); break;
+        default: likelihood_kernel<256, 4><<<grid, 256, 0, s>>>(a); break;
+        }
+    }
+    if (any32) {
+        switch (a.block) {
+        case 128: likelihood32_kernel<128, 8><<<grid, 128, 0, s>>>(a); break;
+        case 512: likelihood32_kernel<512, 4><<<grid, 512, 0, s>>>(a); break;
+        case 1024: likelihood32_kernel<1024, 2><<<grid, 1024, 0, s>>>(a); break;
+        default: likelihood32_kernel<256, 8><<<grid, 256, 0, s>>>(a); break;
+        }
+    }
+}
+
+__global__ void pack_facet_kernel(const FacetDev *dfacet, const double *uu, const double *vv,
+                                  const double *dre, const double *dim, const double *s2,
+                                  float4 *pk, float *invs2)
+{
+    const unsigned n = dfacet->n;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        pk[i] = make_float4((float)uu[i], (float)vv[i], (float)dre[i], (float)dim[i]);
+        invs2[i] = (float)(1.0 / s2[i]);
+    }
+}
+
+void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
+                       const double *vv, const double *dre, const double *dim, const double *s2,
+                       float4 *pk, float *invs2)
+{
+    unsigned blocks = std::min(2048u, (cap + 255) / 256 + 1);
+    pack_facet_kernel<<<blocks, 256, 0, s>>>(dfacet, uu, vv, dre, dim, s2, pk, invs2);
 }
 
 // ------------------------------------------------------------------ r-marginalisation (K4)
@@ -794,14 +993,222 @@ __device__ double marginalise_posterior(int num, const double *x, double *y, Qag
     return log(sump) + ymax;
 }
 
-__global__ void rmarg_kernel(LikLaunch a)
+// Wave-cooperative variants: one 64-lane work-group per item.  Lanes 0..40 evaluate the
+// integrand at the 41 Gauss-Kronrod nodes; every lane then forms the QUADPACK sums from
+// the shared values in the sequential order, so all lanes follow the same control flow and
+// the result is bit-identical to the one-thread version above.
+__device__ inline void qk41_wave(const Spline &sp, double a, double b, double *fvs,
+                                 double *result, double *abserr, double *resabs,
+                                 double *resasc)
 {
-    const int item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= a.nitems) return;
+    const int n = 21;
+    const int lane = threadIdx.x;
+    const double center = 0.5 * (a + b);
+    const double half_length = 0.5 * (b - a);
+    const double abs_half_length = fabs(half_length);
+    if (lane < 41) {
+        double x;
+        if (lane < 20)
+            x = center - half_length * kXgk41[lane];
+        else if (lane < 40)
+            x = center + half_length * kXgk41[lane - 20];
+        else
+            x = center;
+        fvs[lane] = spline_exp(sp, x);
+    }
+    __syncthreads();
+    const double f_center = fvs[40];
+    double result_gauss = 0;
+    double result_kronrod = f_center * kWgk41[n - 1];
+    double result_abs = fabs(result_kronrod);
+    for (int j = 0; j < (n - 1) / 2; j++) {
+        const int jtw = j * 2 + 1;
+        const double f1 = fvs[jtw], f2 = fvs[20 + jtw];
+        const double fsum = f1 + f2;
+        result_gauss += kWg41[j] * fsum;
+        result_kronrod += kWgk41[jtw] * fsum;
+        result_abs += kWgk41[jtw] * (fabs(f1) + fabs(f2));
+    }
+    for (int j = 0; j < n / 2; j++) {
+        const int jtwm1 = j * 2;
+        const double f1 = fvs[jtwm1], f2 = fvs[20 + jtwm1];
+        result_kronrod += kWgk41[jtwm1] * (f1 + f2);
+        result_abs += kWgk41[jtwm1] * (fabs(f1) + fabs(f2));
+    }
+    const double mean = result_kronrod * 0.5;
+    double result_asc = kWgk41[n - 1] * fabs(f_center - mean);
+    for (int j = 0; j < n - 1; j++)
+        result_asc += kWgk41[j] * (fabs(fvs[j] - mean) + fabs(fvs[20 + j] - mean));
+    __syncthreads();  // fvs is reused by the next call
+    const double err = (result_kronrod - result_gauss) * half_length;
+    result_kronrod *= half_length;
+    result_abs *= abs_half_length;
+    result_asc *= abs_half_length;
+    *result = result_kronrod;
+    *resabs = result_abs;
+    *resasc = result_asc;
+    *abserr = rescale_error(err, result_abs, result_asc);
+}
+
+__device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, int limit,
+                          QagWs &w, double *fvs, double *result)
+{
+    double result0, abserr0, resabs0, resasc0;
+    int roundoff_type1 = 0, roundoff_type2 = 0, error_type = 0;
+    const bool writer = threadIdx.x == 0;
+    if (writer) {
+        w.alist[0] = a;
+        w.blist[0] = b;
+        w.rlist[0] = 0.0;
+        w.elist[0] = 0.0;
+        w.order[0] = 0;
+        w.level[0] = 0;
+    }
+    int size = 0, nrmax = 0, imax = 0;
+    *result = 0;
+    qk41_wave(sp, a, b, fvs, &result0, &abserr0, &resabs0, &resasc0);
+    size = 1;
+    if (writer) {
+        w.rlist[0] = result0;
+        w.elist[0] = abserr0;
+    }
+    double tolerance = fmax(0., epsrel * fabs(result0));
+    const double round_off = 50 * kDblEps * resabs0;
+    if (abserr0 <= round_off && abserr0 > tolerance) {
+        *result = result0;
+        return 14;
+    } else if ((abserr0 <= tolerance && abserr0 != resasc0) || abserr0 == 0.0) {
+        *result = result0;
+        return 0;
+    } else if (limit == 1) {
+        *result = result0;
+        return 11;
+    }
+    double area = result0, errsum = abserr0;
+    int iteration = 1;
+    __syncthreads();
+    do {
+        const double a_i = w.alist[imax], b_i = w.blist[imax];
+        const double r_i = w.rlist[imax], e_i = w.elist[imax];
+        const int lev = w.level[imax];
+        const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
+        double area1, area2, error1, error2, resabs1, resabs2, resasc1, resasc2;
+        qk41_wave(sp, a1, b1, fvs, &area1, &error1, &resabs1, &resasc1);
+        qk41_wave(sp, a2, b2, fvs, &area2, &error2, &resabs2, &resasc2);
+        const double area12 = area1 + area2;
+        const double error12 = error1 + error2;
+        errsum += (error12 - e_i);
+        area += area12 - r_i;
+        if (resasc1 != error1 && resasc2 != error2) {
+            const double delta = r_i - area12;
+            if (fabs(delta) <= 1.0e-5 * fabs(area12) && error12 >= 0.99 * e_i) roundoff_type1++;
+            if (iteration >= 10 && error12 > e_i) roundoff_type2++;
+        }
+        tolerance = fmax(0., epsrel * fabs(area));
+        if (errsum > tolerance) {
+            if (roundoff_type1 >= 6 || roundoff_type2 >= 20) error_type = 2;
+            const double tmp = (1 + 100 * kDblEps) * (fabs(a2) + 1000 * kDblMin);
+            if (fabs(a1) <= tmp && fabs(b2) <= tmp) error_type = 3;
+        }
+        const int i_max = imax, i_new = size;
+        const int new_level = lev + 1;
+        if (writer) {
+            if (error2 > error1) {
+                w.alist[i_max] = a2;
+                w.rlist[i_max] = area2;
+                w.elist[i_max] = error2;
+                w.level[i_max] = new_level;
+                w.alist[i_new] = a1;
+                w.blist[i_new] = b1;
+                w.rlist[i_new] = area1;
+                w.elist[i_new] = error1;
+                w.level[i_new] = new_level;
+            } else {
+                w.blist[i_max] = b1;
+                w.rlist[i_max] = area1;
+                w.elist[i_max] = error1;
+                w.level[i_max] = new_level;
+                w.alist[i_new] = a2;
+                w.blist[i_new] = b2;
+                w.rlist[i_new] = area2;
+                w.elist[i_new] = error2;
+                w.level[i_new] = new_level;
+            }
+        }
+        size++;
+        __syncthreads();
+        if (writer) qpsrt(w, size, limit, nrmax, imax);
+        __syncthreads();
+        // broadcast the list state lane 0 maintains
+        __shared__ int sh_imax, sh_nrmax;
+        if (writer) {
+            sh_imax = imax;
+            sh_nrmax = nrmax;
+        }
+        __syncthreads();
+        imax = sh_imax;
+        nrmax = sh_nrmax;
+        __syncthreads();
+        iteration++;
+    } while (iteration < limit && !error_type && errsum > tolerance);
+    double sum = 0;
+    for (int k = 0; k < size; k++) sum += w.rlist[k];
+    *result = sum;
+    if (errsum <= tolerance) return 0;
+    if (error_type == 2) return 14;
+    if (error_type == 3) return 21;
+    if (iteration == limit) return 11;
+    return 5;
+}
+
+// marginalise_r.c:81-223 for the wave: (x, y) in LDS, y shifted in place.
+__device__ double marginalise_posterior_wave(int num, const double *x, double *y, QagWs &w,
+                                             Spline &sp, double *fvs)
+{
+    double ymax = -10.0;
+    int ymaxpos = 0;
+    for (int ii = 0; ii < num; ii++)
+        if (y[ii] > ymax) {
+            ymax = y[ii];
+            ymaxpos = ii;
+        }
+    double xmin = x[ymaxpos], xmax = x[ymaxpos];
+    if (ymaxpos < num - 1) xmax = x[ymaxpos + 1];
+    if (ymaxpos > 0) xmin = x[ymaxpos - 1];
+    for (int ii = 0; ii < num; ii++) {
+        if (x[ii] < xmin) xmin = x[ii];
+        if (x[ii] > xmax) xmax = x[ii];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int ii = 0; ii < num; ii++) y[ii] -= ymax;
+    __syncthreads();
+    if (xmax <= xmin) return NAN;
+    __shared__ int st_init;
+    if (threadIdx.x == 0) st_init = spline_init(sp, x, y, num, 0);
+    __syncthreads();
+    if (st_init) return NAN;
+    double sump = 0.;
+    int st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
+    if (st) {
+        __syncthreads();
+        if (threadIdx.x == 0) spline_init(sp, x, y, num, 1);
+        __syncthreads();
+        st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
+    }
+    if (st) return kBad;
+    return log(sump) + ymax;
+}
+
+__global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
+{
+    __shared__ Spline sp;
+    __shared__ double fvs[64];
+    __shared__ double x[kMaxR], y[kMaxR];
+    const int item = blockIdx.x;
     const LikItem it = a.items[item];
     const FacetDev &F = a.facets[it.slot];
     const double *L_r = a.Lr + (size_t)item * kMaxR;
-    double x[kMaxR], y[kMaxR];
     int num = 0;
     int nRo = 1;
     const int numR = a.rg.numr;
@@ -810,12 +1217,15 @@ __global__ void rmarg_kernel(LikLaunch a)
         for (nRo = 1; nRo < numR; nRo++) {
             const double lv = L_r[nRo];
             if (lv > kBad) {
-                x[num] = a.rg.ro[nRo];
-                y[num] = lv + F.rprior[nRo];
+                if (threadIdx.x == 0) {
+                    x[num] = a.rg.ro[nRo];
+                    y[num] = lv + F.rprior[nRo];
+                }
                 num++;
             }
         }
     }
+    __syncthreads();
     int error = 0;
     double L_e;
     if (num > 4) {
@@ -828,20 +1238,22 @@ __global__ void rmarg_kernel(LikLaunch a)
         w.elist = base + 3 * kQagLimit;
         w.order = ibase;
         w.level = ibase + kQagLimit;
-        L_e = marginalise_posterior(num, x, y, w);
+        L_e = marginalise_posterior_wave(num, x, y, w, sp, fvs);
     } else {
         error = 1;
         L_e = kBad;
     }
     if (L_e == kBad) error = 1;
-    a.Le[item] = L_e;
-    a.err[item] = error;
+    if (threadIdx.x == 0) {
+        a.Le[item] = L_e;
+        a.err[item] = error;
+    }
 }
 
 void launch_rmarg(hipStream_t s, const LikLaunch &a)
 {
     if (a.nitems <= 0) return;
-    rmarg_kernel<<<(a.nitems + 63) / 64, 64, 0, s>>>(a);
+    rmarg_kernel<<<a.nitems, 64, 0, s>>>(a);
 }
 
 __global__ void marginalise_r_kernel(int num, const double *x, const double *y, double *out,

@@ -18,6 +18,8 @@ struct FacetSlot {
     double *acc = nullptr;   // 4 * fmax^2
     double *uu = nullptr, *vv = nullptr, *dre = nullptr, *dim = nullptr, *s2 = nullptr;
     double *sumw = nullptr;  // dense weights, allocated on first request
+    float4 *pk = nullptr;    // FP32 {u, v, D_re, D_im} for the mixed-precision likelihood
+    float *invs2 = nullptr;
     unsigned *scan = nullptr;
     unsigned facet = 0;
     unsigned n = 0;          // host copy of the compacted size (valid after a sync)
@@ -47,6 +49,7 @@ struct Obs {
     double *d_u = nullptr, *d_v = nullptr, *d_w = nullptr;
     uint32_t *d_perm = nullptr;  // sorted row -> original row
     void *d_res = nullptr;
+    void *d_res_saved = nullptr;  // rl2_obs_checkpoint copy
     float *d_s2f = nullptr;
     double *d_wn = nullptr, *d_invlam = nullptr, *d_spec = nullptr;
     std::vector<double> wn, spec, invlam;
@@ -55,6 +58,8 @@ struct Obs {
     std::vector<FacetDev> h_facets;
     LikBuffers lik;
     RGrid rg;
+    int lik_block = 256;     // threads per likelihood work-group
+    int loglik_prec = 0;     // precision of rl2_obs_loglik requests (0 FP64, 1 mixed)
     ~Obs();
     bool ensure_slots(int n);
     void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device

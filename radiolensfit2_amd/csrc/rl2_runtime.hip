@@ -169,6 +169,7 @@ Obs::~Obs()
     dfree(d_w);
     dfree(d_perm);
     dfree(d_res);
+    dfree(d_res_saved);
     dfree(d_s2f);
     dfree(d_wn);
     dfree(d_invlam);
@@ -181,6 +182,8 @@ Obs::~Obs()
         dfree(s.dim);
         dfree(s.s2);
         dfree(s.sumw);
+        dfree(s.pk);
+        dfree(s.invs2);
         dfree(s.scan);
     }
     dfree(d_facets);
@@ -197,6 +200,7 @@ bool Obs::ensure_slots(int n)
         FacetSlot &s = slots[i];
         if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
               dmalloc(&s.dre, ncell) && dmalloc(&s.dim, ncell) && dmalloc(&s.s2, ncell) &&
+              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
         if (!hip_ok(hipMemsetAsync(s.acc, 0, 4 * ncell * sizeof(double), g_stream), "memset"))
@@ -220,6 +224,8 @@ bool Obs::ensure_slots(int n)
         f.dim = slots[i].dim;
         f.s2 = slots[i].s2;
         f.mod = nullptr;
+        f.pk = slots[i].pk;
+        f.invs2 = slots[i].invs2;
         f.n = 0;
         // full descriptor once (n = 0 until a compaction or load writes it)
         if (!hip_ok(hipMemcpyAsync(d_facets + i, &f, sizeof(FacetDev), hipMemcpyHostToDevice,
@@ -304,6 +310,8 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
         KTimer t(RL2_K_COMPACT);
         launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2,
                        &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
+        launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
+                          s.s2, s.pk, s.invs2);
     }
     return hip_ok(hipGetLastError(), "extract launch");
 }
@@ -341,8 +349,11 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     a.err = b.d_err;
     a.qag_ws = b.d_ws;
     a.qag_iws = b.d_iws;
-    a.block = 256;
+    a.block = o.lik_block;
     a.prec = o.prec;
+    std::vector<int> hp(n);
+    for (int i = 0; i < n; i++) hp[i] = items[i].prec;
+    a.host_prec = hp.data();
     {
         KTimer t(RL2_K_LIKELIHOOD);
         launch_likelihood(g_stream, a);
@@ -404,6 +415,7 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
 {
     const int B = opt.batch > 0 ? opt.batch : 1;
     if (!o.ensure_slots(B)) return false;
+    o.lik_block = opt.lik_block > 0 ? opt.lik_block : 256;
     BatchEval eval = [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                          std::vector<int> &err) {
         return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);
@@ -699,6 +711,34 @@ int rl2_obs_add_noise(rl2_obs *obs, double sigma2, uint64_t seed)
                : RL2_ERR_HIP;
 }
 
+int rl2_obs_checkpoint(rl2_obs *obs)
+{
+    if (!obs) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    const size_t bytes = o.nvis * (o.prec == RL2_PREC_FP64 ? 16 : 8);
+    if (!o.d_res_saved && !hip_ok(hipMalloc(&o.d_res_saved, bytes), "hipMalloc checkpoint"))
+        return RL2_ERR_HIP;
+    return hip_ok(hipMemcpyAsync(o.d_res_saved, o.d_res, bytes, hipMemcpyDeviceToDevice, g_stream),
+                  "checkpoint") && hip_ok(hipStreamSynchronize(g_stream), "sync")
+               ? RL2_OK
+               : RL2_ERR_HIP;
+}
+
+int rl2_obs_restore(rl2_obs *obs)
+{
+    if (!obs) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    if (!o.d_res_saved) {
+        set_error("rl2_obs_restore: no checkpoint");
+        return RL2_ERR_STATE;
+    }
+    const size_t bytes = o.nvis * (o.prec == RL2_PREC_FP64 ? 16 : 8);
+    return hip_ok(hipMemcpyAsync(o.d_res, o.d_res_saved, bytes, hipMemcpyDeviceToDevice, g_stream),
+                  "restore") && hip_ok(hipStreamSynchronize(g_stream), "sync")
+               ? RL2_OK
+               : RL2_ERR_HIP;
+}
+
 int rl2_obs_extract(rl2_obs *obs, int slot, double l0, double m0, double flux, double R_mu,
                     unsigned int facet, double *facet_vis, double *facet_sigma2, double *uu,
                     double *vv, double *sum_w, unsigned int *ncoords)
@@ -754,6 +794,8 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
     hipMemcpyAsync(s.uu, uu, n * 8, hipMemcpyHostToDevice, g_stream);
     hipMemcpyAsync(s.vv, vv, n * 8, hipMemcpyHostToDevice, g_stream);
     hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
+    launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
+                      s.invs2);
     s.n = n;
     return hip_ok(hipStreamSynchronize(g_stream), "load_facet") ? RL2_OK : RL2_ERR_HIP;
 }
@@ -776,7 +818,7 @@ int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const dou
     Obs &o = obs->o;
     if ((int)o.slots.size() <= slot) return RL2_ERR_STATE;
     std::vector<LikItem> items(npts);
-    for (int i = 0; i < npts; i++) items[i] = {slot, 0, e1[i], e2[i]};
+    for (int i = 0; i < npts; i++) items[i] = {slot, o.loglik_prec, e1[i], e2[i]};
     std::vector<double> Lr;
     std::vector<int> nit;
     if (L_r) Lr.resize((size_t)npts * kMaxR);
@@ -792,6 +834,13 @@ int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const dou
     return RL2_OK;
 }
 
+int rl2_obs_loglik_precision(rl2_obs *obs, int prec)
+{
+    if (!obs || (prec != 0 && prec != 1)) return RL2_ERR_ARG;
+    obs->o.loglik_prec = prec;
+    return RL2_OK;
+}
+
 int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_result *res)
 {
     if (!obs || slot < 0 || !res) return RL2_ERR_ARG;
@@ -804,6 +853,7 @@ int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_resu
         op.line_chunk = 4;
     }
     std::vector<FitRequest> reqs{{slot, res}};
+    o.lik_block = op.lik_block > 0 ? op.lik_block : 256;
     bool ok = fit_batch(reqs, op, [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                                       std::vector<int> &err) {
         return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);

@@ -30,8 +30,11 @@ def rprior_for(flux, ro):
     return rp
 
 
-def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=256):
-    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0)
+def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=512,
+                lik_precision=0):
+    """rl2_fit_options; lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex +
+    FP64 sampling."""
+    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0, lik_precision)
 
 
 class DeviceObservation:
@@ -77,6 +80,15 @@ class DeviceObservation:
             self.h, len(cat), N.dptr(cat.flux), N.dptr(cat.scale), N.dptr(cat.l), N.dptr(cat.m),
             None if round_models else N.dptr(cat.e1), None if round_models else N.dptr(cat.e2),
             sign), "rl2_obs_add_sources")
+
+    def checkpoint(self):
+        N.check(N.hip().rl2_obs_checkpoint(self.h), "rl2_obs_checkpoint")
+
+    def restore(self):
+        N.check(N.hip().rl2_obs_restore(self.h), "rl2_obs_restore")
+
+    def loglik_precision(self, prec):
+        N.check(N.hip().rl2_obs_loglik_precision(self.h, prec), "rl2_obs_loglik_precision")
 
     def add_noise(self, sigma2, seed):
         N.check(N.hip().rl2_obs_add_noise(self.h, float(sigma2), int(seed)), "rl2_obs_add_noise")

@@ -108,3 +108,29 @@ def test_process_matches_oracle(gpu, case, batch):
         assert abs(a.e1 - b.e1) <= 1e-6 and abs(a.e2 - b.e2) <= 1e-6, (g, a.e1, b.e1, a.e2, b.e2)
     scale = np.abs(ores_vis).max()
     assert np.abs(resid - ores_vis).max() <= 1e-10 * scale
+
+
+def test_mixed_likelihood_tracks_fp64(dev, case):
+    """FP32 model/cross-correlation terms with FP64 phase reduction and sums: the
+    log-likelihood stays within a few parts in 1e7 of the FP64 value (relative to its size)."""
+    obs, cat, vs = case
+    g = 0
+    facet, fv, fs, uu, vv, sw = T.oracle_extract(obs, cat, g, obs.vis, vs)
+    dev.load_facet(fv, fs, uu, vv, slot=2)
+    dev.set_prior(P.rprior_for(cat.flux[g], P.r_grid()), slot=2)
+    rng = np.random.default_rng(7)
+    e = rng.uniform(-0.6, 0.6, size=(40, 2))
+    e = e[(e ** 2).sum(1) < 0.65]
+    L64, err64, Lr64, nit64 = dev.loglik(e[:, 0], e[:, 1], slot=2, detail=True)
+    dev.loglik_precision(1)
+    try:
+        L32, err32, Lr32, nit32 = dev.loglik(e[:, 0], e[:, 1], slot=2, detail=True)
+    finally:
+        dev.loglik_precision(0)
+    assert (err32 == err64).all()
+    ok = err64 == 0
+    scale = np.maximum(1.0, np.abs(L64[ok]))
+    assert (np.abs(L32[ok] - L64[ok]) / scale).max() < 5e-6
+    valid = Lr64[:, 1:] > -1e9
+    assert (np.abs(Lr32[:, 1:][valid] - Lr64[:, 1:][valid])
+            / np.maximum(1, np.abs(Lr64[:, 1:][valid]))).max() < 5e-6
