@@ -150,13 +150,27 @@ def main():
                     help="re-run the timed galaxies with the FP64 GPU path (untimed)")
     ap.add_argument("--no-speculate", action="store_true")
     ap.add_argument("--line-chunk", type=int, default=4)
-    ap.add_argument("--lik-block", type=int, default=512)
+    ap.add_argument("--lik-block", type=int, default=0,
+                    help="0: tiled likelihood; 64..1024: one work-group per (e, r) task")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (the reference kernel "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE "
               f"{world}); the multi-GPU sharding is a design, not a build.", file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes}), flush=True)
+        return
 
     cfg = synth.ARRAY_197 if args.config == "197dish" else synth.ARRAY_64
     res_prec = N.RL2_PREC_FP64 if args.residual == "c128" else N.RL2_PREC_MIXED

@@ -67,7 +67,9 @@ typedef struct {
     int do_refit;          /* 1: run the bad-list re-fit pass, as the reference always does */
     int nm_speculate;      /* 1: evaluate reflect/expand/contract candidates in one launch */
     int line_chunk;        /* line-search steps evaluated per launch (default 4) */
-    int lik_block;         /* threads per likelihood work-group (default 256) */
+    int lik_block;         /* 0 (default): tiled likelihood, every (e, r) task split over
+                              work-groups of 2048 facet cells; 64..1024: one work-group
+                              of that many threads per (e, r) task */
     int verbose;
     int lik_precision;     /* 0: FP64 likelihood throughout; 1: mixed-precision likelihood
                               throughout; 2: mixed for the simplex, FP64 for the sampling */

@@ -115,6 +115,33 @@ constexpr int kQagInts = 2 * kQagLimit;
 void launch_likelihood(hipStream_t s, const LikLaunch &a);
 void launch_rmarg(hipStream_t s, const LikLaunch &a);
 
+// Tiled likelihood.  Every (item, scalelength) task is split into chunks of kTileCells
+// facet cells with one work-group per chunk.  A pass (the zero-offset sums, or one Newton
+// iteration) is one launch: each work-group writes its chunk's sums, and the last
+// work-group of a task to arrive adds the chunk sums in chunk order and takes the Newton
+// step or finishes L_r.  The r-marginalisation follows as its own launch.
+constexpr int kTileBS = 256, kTileCPT = 8, kTileCells = kTileBS * kTileCPT;
+struct TileTask {
+    double xo, yo;             // position offset of the current Newton iterate
+    double norm;               // |increment| of the last step (1 before the first)
+    double S2;                 // sum M^2 / sigma^2 of the unnormalised model
+    int iter;                  // Newton steps taken
+    int active;                // 1 while another Newton pass is due
+    unsigned arrivals;         // work-groups of the current pass that have finished
+    unsigned pad;
+};
+struct TileLaunch {
+    LikLaunch lk;              // items, facets, r grid, outputs and QAG workspace
+    const unsigned *wg_off;    // nitems + 1: first work-group of each item (nr * chunks)
+    TileTask *tasks;           // nitems * kMaxR
+    double *partials;          // 8 doubles per work-group
+    unsigned *status;         // [0] tasks still active, [1] largest Newton step count
+    unsigned nwg;
+};
+void launch_tile_init(hipStream_t s, const TileLaunch &t);
+void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
+void launch_tile_status(hipStream_t s, const TileLaunch &t);
+
 // Residual update for one fitted galaxy: R -= model(e) - model(0).
 void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,
                      const double *v, const double *w, const double *wn, const double *spec,

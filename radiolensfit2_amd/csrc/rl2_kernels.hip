@@ -162,22 +162,31 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 }
 
 // ------------------------------------------------------------------ extraction + gridding
-// One work-group owns RPT*256 consecutive (sorted) rows.  Per channel each thread grids its
-// RPT visibilities into an LDS open-addressing table keyed by cell index (at most
-// RPT*256 distinct keys, table size 2*RPT*256, so probing always terminates), then the
-// table is flushed to the global accumulators with one atomic per distinct cell.  Sorted
-// rows make neighbouring threads hit the same cells, so hot cells near the uv origin
-// receive one global atomic per work-group instead of one per visibility.
+// One work-group owns RPT*256 consecutive (sorted) rows, each thread RPT consecutive rows.
+// Sorted rows are neighbours in uv, so most visibilities of a wave fall in a few cells.
+// Per channel:
+//  1. each thread merges runs of equal cells among its own RPT visibilities;
+//  2. per row slot, the wave sums the values of lanes sharing a cell (one pass per
+//     distinct cell: the lowest pending lane's cell is broadcast, matching lanes are summed
+//     with cross-lane adds) and that lane alone inserts the sum into an LDS open-addressing
+//     table keyed by cell (HS >= RPT*256 slots, so a free slot always exists);
+//  3. the table's occupied slots are flushed to the global accumulators with one atomic
+//     per cell and component.
+// LDS atomics are therefore issued once per distinct cell per wave instead of once per
+// visibility (same-address LDS atomics serialise across the 64 lanes).
 template <int MODE, int PREC, int RPT>
 __global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
 {
     constexpr int BS = 256;
-    constexpr int HS = 2 * RPT * BS;
+    constexpr int HS = RPT * BS;
     constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
+    static_assert((1 << HBITS) == HS, "hash size must be a power of two");
     __shared__ uint32_t hkey[HS];
     __shared__ double hre[HS], him[HS], hs2[HS];
     __shared__ uint32_t hcnt[HS];
-    const int tid = threadIdx.x;
+    __shared__ uint32_t hlist[HS];
+    __shared__ uint32_t hn;
+    const int tid = threadIdx.x, lane = tid & 63;
     for (int s = tid; s < HS; s += BS) {
         hkey[s] = kEmpty;
         hre[s] = 0.;
@@ -185,16 +194,17 @@ __global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
         hs2[s] = 0.;
         hcnt[s] = 0;
     }
-    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS);
+    if (tid == 0) hn = 0;
+    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS) + (unsigned long)tid * RPT;
     double ru[RPT], rv[RPT], rp[RPT];
     bool rok[RPT];
 #pragma unroll
     for (int k = 0; k < RPT; k++) {
-        unsigned long i = row0 + tid + (unsigned long)k * BS;
+        const unsigned long i = row0 + k;
         rok[k] = i < a.nrow;
         ru[k] = rok[k] ? a.u[i] : 0.;
         rv[k] = rok[k] ? a.v[i] : 0.;
-        double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
+        const double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
         rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
     }
     const unsigned long ncell = (unsigned long)a.facet * a.facet;
@@ -204,62 +214,113 @@ __global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
     for (unsigned c = 0; c < a.nch; c++) {
         const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
         const double wn2 = wn * wn, spectra = a.spec[c];
+        uint32_t key[RPT], cnt[RPT];
+        double xr[RPT], xi[RPT], xs[RPT];
+        bool ok[RPT];
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
+            ok[k] = false;
+            key[k] = kEmpty;
+            cnt[k] = 1;
+            xr[k] = xi[k] = xs[k] = 0.;
             if (!rok[k]) continue;
-            const size_t idx = (size_t)c * a.nrow + row0 + tid + (size_t)k * BS;
+            const size_t idx = (size_t)c * a.nrow + row0 + k;
             const float s2 = a.s2f[idx];
             if (signbit(s2)) continue;  // flagged
-            uint32_t key;
-            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key)) continue;
-            double xr = 0., xi = 0.;
+            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key[k])) continue;
+            ok[k] = true;
+            xs[k] = (double)s2;
             if (MODE == 0) {
                 // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
                 // (source_extraction.c:51-65).
-                double phase = wn * rp[k];
-                double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
+                const double phase = wn * rp[k];
+                const double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
                 double sn, cs;
                 sincos(phase, &sn, &cs);
-                double2 r = load_res(a.res, PREC, idx);
-                double gre = shape * cs + r.x;
-                double gim = shape * sn + r.y;
-                xr = gre * cs - gim * (-sn);
-                xi = gre * (-sn) + gim * cs;
+                const double2 r = load_res(a.res, PREC, idx);
+                const double gre = shape * cs + r.x;
+                const double gim = shape * sn + r.y;
+                xr[k] = gre * cs - gim * (-sn);
+                xi[k] = gre * (-sn) + gim * cs;
             } else if (MODE == 1) {
-                double2 r = load_res(a.res, PREC, idx);
-                xr = r.x;
-                xi = r.y;
+                const double2 r = load_res(a.res, PREC, idx);
+                xr[k] = r.x;
+                xi[k] = r.y;
             }
-            uint32_t h = (key * 2654435761u) >> (32 - HBITS);
-            while (true) {
-                uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
-                if (prev == kEmpty || prev == key) break;
-                h = (h + 1) & (HS - 1);
+        }
+        // 1. runs of equal cells within the thread
+        int last = -1;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!ok[k]) continue;
+            if (last >= 0 && key[k] == key[last]) {
+                xr[last] += xr[k];
+                xi[last] += xi[k];
+                xs[last] += xs[k];
+                cnt[last] += cnt[k];
+                ok[k] = false;
+            } else {
+                last = k;
             }
-            if (MODE != 2) {
-                atomicAdd(&hre[h], xr);
-                atomicAdd(&him[h], xi);
+        }
+        // 2. equal cells across the wave, one row slot at a time
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            bool pending = ok[k];
+            uint64_t todo = __ballot(pending);
+            while (todo) {
+                const int leader = __ffsll((unsigned long long)todo) - 1;
+                const uint32_t lk = (uint32_t)__shfl((int)key[k], leader);
+                const bool mine = pending && key[k] == lk;
+                double sr = mine ? xr[k] : 0., si = mine ? xi[k] : 0., ss = mine ? xs[k] : 0.;
+                uint32_t sc = mine ? cnt[k] : 0u;
+                for (int off = 32; off >= 1; off >>= 1) {
+                    if (MODE != 2) {
+                        sr += __shfl_xor(sr, off);
+                        si += __shfl_xor(si, off);
+                    }
+                    ss += __shfl_xor(ss, off);
+                    sc += (uint32_t)__shfl_xor((int)sc, off);
+                }
+                if (lane == leader) {
+                    uint32_t h = (lk * 2654435761u) >> (32 - HBITS);
+                    for (int probe = 0; probe < HS; probe++) {
+                        const uint32_t prev = atomicCAS(&hkey[h], kEmpty, lk);
+                        if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
+                        if (prev == kEmpty || prev == lk) break;
+                        h = (h + 1) & (HS - 1);
+                    }
+                    if (MODE != 2) {
+                        atomicAdd(&hre[h], sr);
+                        atomicAdd(&him[h], si);
+                    }
+                    atomicAdd(&hs2[h], ss);
+                    atomicAdd(&hcnt[h], sc);
+                }
+                pending = pending && !mine;
+                todo = __ballot(pending);
             }
-            atomicAdd(&hs2[h], (double)s2);
-            atomicAdd(&hcnt[h], 1u);
         }
         __syncthreads();
-        for (int s = tid; s < HS; s += BS) {
-            uint32_t key = hkey[s];
-            if (key != kEmpty) {
-                if (MODE != 2) {
-                    unsafeAtomicAdd(&acc_re[key], hre[s]);
-                    unsafeAtomicAdd(&acc_im[key], him[s]);
-                }
-                unsafeAtomicAdd(&acc_s2[key], hs2[s]);
-                unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
-                hkey[s] = kEmpty;
-                hre[s] = 0.;
-                him[s] = 0.;
-                hs2[s] = 0.;
-                hcnt[s] = 0;
+        // 3. flush the occupied slots
+        const uint32_t nocc = hn;
+        for (uint32_t j = tid; j < nocc; j += BS) {
+            const uint32_t s = hlist[j];
+            const uint32_t cell = hkey[s];
+            if (MODE != 2) {
+                unsafeAtomicAdd(&acc_re[cell], hre[s]);
+                unsafeAtomicAdd(&acc_im[cell], him[s]);
             }
+            unsafeAtomicAdd(&acc_s2[cell], hs2[s]);
+            unsafeAtomicAdd(&acc_w[cell], (double)hcnt[s]);
+            hkey[s] = kEmpty;
+            hre[s] = 0.;
+            him[s] = 0.;
+            hs2[s] = 0.;
+            hcnt[s] = 0;
         }
+        __syncthreads();
+        if (tid == 0) hn = 0;
         __syncthreads();
     }
 }
@@ -1185,14 +1246,15 @@ __device__ double marginalise_posterior_wave(int num, const double *x, double *y
     __syncthreads();
     if (xmax <= xmin) return NAN;
     __shared__ int st_init;
-    if (threadIdx.x == 0) st_init = spline_init(sp, x, y, num, 0);
+    __shared__ double spline_ws[7 * kMaxR];
+    if (threadIdx.x == 0) st_init = spline_init_ws(sp, x, y, num, 0, spline_ws);
     __syncthreads();
     if (st_init) return NAN;
     double sump = 0.;
     int st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
     if (st) {
         __syncthreads();
-        if (threadIdx.x == 0) spline_init(sp, x, y, num, 1);
+        if (threadIdx.x == 0) spline_init_ws(sp, x, y, num, 1, spline_ws);
         __syncthreads();
         st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
     }
@@ -1200,12 +1262,11 @@ __device__ double marginalise_posterior_wave(int num, const double *x, double *y
     return log(sump) + ymax;
 }
 
-__global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
+// likelihood.c:126-146 for one item: L_e from its L_r, by every thread of the work-group
+// (at least 64); sp, fvs, x, y in LDS.
+__device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs, double *x,
+                           double *y)
 {
-    __shared__ Spline sp;
-    __shared__ double fvs[64];
-    __shared__ double x[kMaxR], y[kMaxR];
-    const int item = blockIdx.x;
     const LikItem it = a.items[item];
     const FacetDev &F = a.facets[it.slot];
     const double *L_r = a.Lr + (size_t)item * kMaxR;
@@ -1248,6 +1309,277 @@ __global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
         a.Le[item] = L_e;
         a.err[item] = error;
     }
+}
+
+__global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
+{
+    __shared__ Spline sp;
+    __shared__ double fvs[64];
+    __shared__ double x[kMaxR], y[kMaxR];
+    rmarg_item(a, blockIdx.x, sp, fvs, x, y);
+}
+
+// ------------------------------------------------------------------ likelihood, tiled
+// Largest item whose first work-group is <= wg (wg_off is non-decreasing, every item has
+// at least one work-group).
+__device__ inline int tile_item(const unsigned *wg_off, int nitems, unsigned wg)
+{
+    int lo = 0, hi = nitems;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (wg_off[mid] <= wg)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Chunk sums in FP64.  PASS 0: {S2, value, dx, dy, dxx, dyy, dxy} of the unnormalised
+// model at zero offset (likelihood.c:214-236); PASS 1: {value, dx, dy, dxx, dyy, dxy} at
+// the offset (xo, yo) (likelihood.c:296-327).
+template <int PASS>
+__device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double ro, double xo,
+                                   double yo, unsigned beg, unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+#pragma unroll 2
+    for (unsigned i = beg + threadIdx.x; i < end; i += kTileBS) {
+        const double uu = F.uu[i], vv = F.vv[i];
+        const double k1 = a1 * uu + e2 * vv;
+        const double k2 = e2 * uu + a2 * vv;
+        const double rr = rsqrt(1. + sw * (k1 * k1 + k2 * k2));
+        const double M = rr * rr * rr;
+        const double hr = F.dre[i] * M;
+        const double hi = -F.dim[i] * M;
+        const double U = kTwoPi * uu, V = kTwoPi * vv;
+        if (PASS == 0) {
+            s[0] += M * M / F.s2[i];
+            s[1] += hr;
+            s[2] -= hi * U;
+            s[3] -= hi * V;
+            s[4] -= hr * U * U;
+            s[5] -= hr * V * V;
+            s[6] -= hr * U * V;
+        } else {
+            double sn, cs;
+            sincos(xo * U + yo * V, &sn, &cs);
+            const double re = hr * cs - hi * sn;
+            const double im = hr * sn + hi * cs;
+            s[0] += re;
+            s[1] -= im * U;
+            s[2] -= im * V;
+            s[3] -= re * U * U;
+            s[4] -= re * V * V;
+            s[5] -= re * U * V;
+        }
+    }
+}
+
+// The same sums from the FP32 facet copy: model and terms in FP32 (each thread adds at
+// most kTileCPT cells in FP32), the Newton phase formed and reduced in FP64 revolutions.
+template <int PASS>
+__device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double ro, double xo,
+                                   double yo, unsigned beg, unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const float sw = (float)(sf * (4 * kPi * kPi));
+    const float a1 = (float)(1. + e1), a2 = (float)(1. - e1), fe2 = (float)e2;
+    const float tp = 6.28318530717958648f;
+    float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+    for (unsigned i = beg + threadIdx.x; i < end; i += kTileBS) {
+        const float4 p = F.pk[i];
+        const float k1 = a1 * p.x + fe2 * p.y;
+        const float k2 = fe2 * p.x + a2 * p.y;
+        const float rr = rsqrtf(1.f + sw * (k1 * k1 + k2 * k2));
+        const float M = rr * rr * rr;
+        const float hr = p.z * M;
+        const float hi = -p.w * M;
+        const float U = tp * p.x, V = tp * p.y;
+        if (PASS == 0) {
+            f[0] += M * M * F.invs2[i];
+            f[1] += hr;
+            f[2] -= hi * U;
+            f[3] -= hi * V;
+            f[4] -= hr * U * U;
+            f[5] -= hr * V * V;
+            f[6] -= hr * U * V;
+        } else {
+            float sn, cs;
+            sincos_rev(xo * (double)p.x + yo * (double)p.y, sn, cs);
+            const float re = hr * cs - hi * sn;
+            const float im = hr * sn + hi * cs;
+            f[0] += re;
+            f[1] -= im * U;
+            f[2] -= im * V;
+            f[3] -= re * U * U;
+            f[4] -= re * V * V;
+            f[5] -= re * U * V;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = (double)f[j];
+}
+
+// The Newton loop of likelihood.c:284-350 one step at a time, by the last work-group of
+// a pass: returns 1 when the task has finished (L_r written).
+template <int PASS>
+__device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
+                                const double (&v)[7])
+{
+    double value, dx, dy, dxx, dyy, dxy;
+    if (PASS == 0) {
+        T->S2 = v[0];
+        value = v[1];
+        dx = v[2];
+        dy = v[3];
+        dxx = v[4];
+        dyy = v[5];
+        dxy = v[6];
+        T->xo = 0.;
+        T->yo = 0.;
+        T->norm = 1.;
+        T->iter = 0;
+    } else {
+        value = v[0];
+        dx = v[1];
+        dy = v[2];
+        dxx = v[3];
+        dyy = v[4];
+        dxy = v[5];
+    }
+    T->arrivals = 0;
+    const double det = dxx * dyy - dxy * dxy;
+    if (det > 0 && T->norm > 1.e-7 && T->iter < 50) {
+        const double incx = (dx * dyy - dy * dxy) / det;
+        const double incy = (dy * dxx - dx * dxy) / det;
+        T->xo -= incx;
+        T->yo -= incy;
+        T->iter += 1;
+        T->norm = sqrt(incx * incx + incy * incy);
+        T->active = 1;
+        return 0;
+    }
+    T->active = 0;
+    const double S2 = T->S2;
+    finish_lr(a, item, r, value / sqrt(S2), det / S2, T->iter);
+    return 1;
+}
+
+template <int PASS>
+__global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
+{
+    __shared__ double red[(kTileBS / 64) * 8];
+    __shared__ int s_last;
+    const LikLaunch &a = t.lk;
+    const unsigned wg = blockIdx.x;
+    const int item = tile_item(t.wg_off, a.nitems, wg);
+    const int nr = a.rg.numr - 1;
+    const unsigned first = t.wg_off[item];
+    const unsigned nch = (t.wg_off[item + 1] - first) / nr;
+    const unsigned local = wg - first;
+    const int r = (int)(local / nch) + 1;
+    const unsigned chunk = local - (unsigned)(r - 1) * nch;
+    TileTask *T = t.tasks + (size_t)item * kMaxR + r;
+    if (PASS == 1 && !T->active) return;  // finished task: the whole work-group leaves
+    const LikItem it = a.items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned beg = chunk * kTileCells;
+    const unsigned end = min(F.n, beg + kTileCells);
+    const double xo = PASS == 1 ? T->xo : 0., yo = PASS == 1 ? T->yo : 0.;
+    double s[7];
+    if (it.prec == 0)
+        tile_sums64<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
+    else
+        tile_sums32<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
+    block_allreduce<kTileBS, 7>(s, red);
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double *P = t.partials + (size_t)wg * 8;
+#pragma unroll
+        for (int j = 0; j < 7; j++) P[j] = s[j];
+        __threadfence();  // release the chunk sums before counting the arrival
+        const unsigned prev = atomicAdd(&T->arrivals, 1u);
+        s_last = (prev + 1 == nch);
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // acquire the other work-groups' chunk sums
+    const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1) * nch) * 8;
+    double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+    for (unsigned c = tid; c < nch; c += kTileBS)
+#pragma unroll
+        for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * 8 + j];
+    block_allreduce<kTileBS, 7>(v, red);
+    if (tid == 0) tile_step<PASS>(a, T, item, r, v);
+}
+
+__global__ void tile_init_kernel(TileLaunch t)
+{
+    const unsigned ntask = (unsigned)t.lk.nitems * kMaxR;
+    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask;
+         k += gridDim.x * blockDim.x) {
+        t.tasks[k].arrivals = 0;
+        t.tasks[k].active = 0;
+        t.tasks[k].iter = 0;
+    }
+}
+
+// status[0] = tasks still iterating, status[1] = largest Newton step count.
+__global__ __launch_bounds__(256) void tile_status_kernel(TileLaunch t)
+{
+    __shared__ unsigned cnt[4], mx[4];
+    const int nr = t.lk.rg.numr - 1;
+    const unsigned ntask = (unsigned)t.lk.nitems * nr;
+    unsigned c = 0, m = 0;
+    for (unsigned k = threadIdx.x; k < ntask; k += 256) {
+        const TileTask &T = t.tasks[(size_t)(k / nr) * kMaxR + k % nr + 1];
+        c += T.active ? 1u : 0u;
+        m = max(m, (unsigned)T.iter);
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        c += __shfl_xor(c, off);
+        m = max(m, (unsigned)__shfl_xor((int)m, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        cnt[threadIdx.x >> 6] = c;
+        mx[threadIdx.x >> 6] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        t.status[0] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+        t.status[1] = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
+    }
+}
+
+void launch_tile_init(hipStream_t s, const TileLaunch &t)
+{
+    const unsigned ntask = (unsigned)t.lk.nitems * kMaxR;
+    tile_init_kernel<<<(ntask + 255) / 256, 256, 0, s>>>(t);
+}
+
+void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
+{
+    if (pass == 0)
+        lik_tile_kernel<0><<<t.nwg, kTileBS, 0, s>>>(t);
+    else
+        lik_tile_kernel<1><<<t.nwg, kTileBS, 0, s>>>(t);
+}
+
+void launch_tile_status(hipStream_t s, const TileLaunch &t)
+{
+    tile_status_kernel<<<1, 256, 0, s>>>(t);
 }
 
 void launch_rmarg(hipStream_t s, const LikLaunch &a)

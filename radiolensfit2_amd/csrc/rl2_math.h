@@ -122,8 +122,10 @@ struct Spline {
     double x[kMaxR], y[kMaxR], c[kMaxR];
 };
 
-// natural cubic spline: tridiagonal system solved by LDL^T (GSL cspline_init semantics)
-RL2_HD int spline_init(Spline &s, const double *xa, const double *ya, int size, int linear)
+// natural cubic spline: tridiagonal system solved by LDL^T (GSL cspline_init semantics);
+// ws: 7 * kMaxR doubles of workspace (LDS on the device, so nothing spills to scratch)
+RL2_HD int spline_init_ws(Spline &s, const double *xa, const double *ya, int size, int linear,
+                          double *ws)
 {
     s.n = size;
     s.linear = linear;
@@ -135,7 +137,7 @@ RL2_HD int spline_init(Spline &s, const double *xa, const double *ya, int size, 
         if (!(xa[i - 1] < xa[i])) return 1;
     if (linear) return 0;
     const int max_index = size - 1, sys = max_index - 1;
-    double diag[kMaxR], off[kMaxR], g[kMaxR];
+    double *diag = ws, *off = ws + kMaxR, *g = ws + 2 * kMaxR;
     s.c[0] = 0.0;
     s.c[max_index] = 0.0;
     for (int i = 0; i < sys; i++) {
@@ -153,7 +155,7 @@ RL2_HD int spline_init(Spline &s, const double *xa, const double *ya, int size, 
         s.c[1] = g[0] / diag[0];
         return 0;
     }
-    double gam[kMaxR], alp[kMaxR], z[kMaxR];
+    double *gam = ws + 3 * kMaxR, *alp = ws + 4 * kMaxR, *z = ws + 5 * kMaxR;
     int status = 0;
     alp[0] = diag[0];
     gam[0] = off[0] / alp[0];
@@ -167,11 +169,17 @@ RL2_HD int spline_init(Spline &s, const double *xa, const double *ya, int size, 
     z[0] = g[0];
     for (int i = 1; i < sys; i++) z[i] = g[i] - gam[i - 1] * z[i - 1];
     double *x = s.c + 1;
-    double cc[kMaxR];
+    double *cc = ws + 6 * kMaxR;
     for (int i = 0; i < sys; i++) cc[i] = z[i] / alp[i];
     x[sys - 1] = cc[sys - 1];
     for (int i = sys - 2; i >= 0; i--) x[i] = cc[i] - gam[i] * x[i + 1];
     return status;
+}
+
+RL2_HD int spline_init(Spline &s, const double *xa, const double *ya, int size, int linear)
+{
+    double ws[7 * kMaxR];
+    return spline_init_ws(s, xa, ya, size, linear, ws);
 }
 
 // exp(spline(x)); 0 outside the knots (marf, marginalise_r.c:227-238)

@@ -36,7 +36,15 @@ struct LikBuffers {
     int *d_err = nullptr, *h_err = nullptr;
     double *d_ws = nullptr;
     int *d_iws = nullptr;
+    // tiled engine
+    unsigned *d_wg_off = nullptr, *h_wg_off = nullptr;  // cap + 1
+    TileTask *d_tasks = nullptr;                         // cap * kMaxR
+    unsigned *d_status = nullptr, *h_status = nullptr;   // 2
+    double *d_partials = nullptr;
+    size_t partials_cap = 0;                             // work-groups
+    int tile_passes = 6;     // Newton passes queued before the first status check
     bool ensure(int n);
+    bool ensure_partials(size_t nwg);
     void release();
 };
 
@@ -58,7 +66,7 @@ struct Obs {
     std::vector<FacetDev> h_facets;
     LikBuffers lik;
     RGrid rg;
-    int lik_block = 256;     // threads per likelihood work-group
+    int lik_block = 0;       // 0: tiled engine; 64..1024: one work-group per (item, r)
     int loglik_prec = 0;     // precision of rl2_obs_loglik requests (0 FP64, 1 mixed)
     ~Obs();
     bool ensure_slots(int n);

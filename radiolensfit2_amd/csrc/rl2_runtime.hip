@@ -2,6 +2,7 @@
 // driver and the rl2_* C-ABI (include/rl2.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <math.h>
 #include <mutex>
@@ -129,7 +130,14 @@ bool LikBuffers::ensure(int n)
     while (c < n) c *= 2;
     bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
               dmalloc(&d_nit, (size_t)c * kMaxR) && dmalloc(&d_Le, c) && dmalloc(&d_err, c) &&
-              dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts);
+              dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts) &&
+              dmalloc(&d_wg_off, c + 1) && dmalloc(&d_tasks, (size_t)c * kMaxR) &&
+              dmalloc(&d_status, 2);
+    ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * sizeof(TileTask), g_stream),
+                      "memset");
+    ok = ok && hip_ok(hipHostMalloc((void **)&h_wg_off, (c + 1) * sizeof(unsigned)),
+                      "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_status, 2 * sizeof(unsigned)), "hipHostMalloc");
     ok = ok && hip_ok(hipHostMalloc((void **)&h_items, c * sizeof(LikItem)), "hipHostMalloc") &&
          hip_ok(hipHostMalloc((void **)&h_Lr, (size_t)c * kMaxR * sizeof(double)), "hipHostMalloc") &&
          hip_ok(hipHostMalloc((void **)&h_nit, (size_t)c * kMaxR * sizeof(int)), "hipHostMalloc") &&
@@ -147,6 +155,13 @@ void LikBuffers::release()
     dfree(d_err);
     dfree(d_ws);
     dfree(d_iws);
+    dfree(d_wg_off);
+    dfree(d_tasks);
+    dfree(d_status);
+    if (h_wg_off) hipHostFree(h_wg_off);
+    if (h_status) hipHostFree(h_status);
+    h_wg_off = nullptr;
+    h_status = nullptr;
     if (h_items) hipHostFree(h_items);
     if (h_Lr) hipHostFree(h_Lr);
     if (h_nit) hipHostFree(h_nit);
@@ -158,6 +173,19 @@ void LikBuffers::release()
     h_Le = nullptr;
     h_err = nullptr;
     cap = 0;
+}
+bool LikBuffers::ensure_partials(size_t nwg)
+{
+    if (nwg <= partials_cap) return true;
+    dfree(d_partials);
+    size_t c = 4096;
+    while (c < nwg) c *= 2;
+    if (!dmalloc(&d_partials, c * 8)) {
+        partials_cap = 0;
+        return false;
+    }
+    partials_cap = c;
+    return true;
 }
 
 // ------------------------------------------------------------------ observation
@@ -188,6 +216,8 @@ Obs::~Obs()
     }
     dfree(d_facets);
     lik.release();
+    dfree(lik.d_partials);
+    lik.partials_cap = 0;
 }
 
 bool Obs::ensure_slots(int n)
@@ -354,25 +384,87 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     std::vector<int> hp(n);
     for (int i = 0; i < n; i++) hp[i] = items[i].prec;
     a.host_prec = hp.data();
-    {
-        KTimer t(RL2_K_LIKELIHOOD);
-        launch_likelihood(g_stream, a);
-    }
-    {
-        KTimer t(RL2_K_RMARG);
-        launch_rmarg(g_stream, a);
-    }
-    if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
-    hipMemcpyAsync(b.h_Le, b.d_Le, n * sizeof(double), hipMemcpyDeviceToHost, g_stream);
-    hipMemcpyAsync(b.h_err, b.d_err, n * sizeof(int), hipMemcpyDeviceToHost, g_stream);
+    bool tiled = o.lik_block == 0;
+    for (int i = 0; i < n && tiled; i++)
+        if (o.h_facets[items[i].slot].mod) tiled = false;  // given model: one group per (e, r)
     const bool want_detail = Lr || nit || g_stats_on;
-    if (want_detail) {
-        hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double), hipMemcpyDeviceToHost,
-                       g_stream);
-        hipMemcpyAsync(b.h_nit, b.d_nit, (size_t)n * kMaxR * sizeof(int), hipMemcpyDeviceToHost,
-                       g_stream);
+    auto copy_out = [&]() {
+        hipMemcpyAsync(b.h_Le, b.d_Le, n * sizeof(double), hipMemcpyDeviceToHost, g_stream);
+        hipMemcpyAsync(b.h_err, b.d_err, n * sizeof(int), hipMemcpyDeviceToHost, g_stream);
+        if (want_detail) {
+            hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double),
+                           hipMemcpyDeviceToHost, g_stream);
+            hipMemcpyAsync(b.h_nit, b.d_nit, (size_t)n * kMaxR * sizeof(int),
+                           hipMemcpyDeviceToHost, g_stream);
+        }
+    };
+    if (tiled) {
+        // chunks per (item, r) from the host copy of each slot's cell count
+        const int nr = o.rg.numr - 1;
+        unsigned total = 0;
+        b.h_wg_off[0] = 0;
+        for (int i = 0; i < n; i++) {
+            const unsigned cells = o.slots[items[i].slot].n;
+            const unsigned nchunk = std::max(1u, (cells + kTileCells - 1) / kTileCells);
+            total += nchunk * (unsigned)nr;
+            b.h_wg_off[i + 1] = total;
+        }
+        if (!b.ensure_partials(total)) return false;
+        hipMemcpyAsync(b.d_wg_off, b.h_wg_off, (n + 1) * sizeof(unsigned),
+                       hipMemcpyHostToDevice, g_stream);
+        TileLaunch t;
+        t.lk = a;
+        t.wg_off = b.d_wg_off;
+        t.tasks = b.d_tasks;
+        t.partials = b.d_partials;
+        t.status = b.d_status;
+        t.nwg = total;
+        launch_tile_init(g_stream, t);
+        {
+            KTimer k(RL2_K_LIKELIHOOD);
+            launch_tile_pass(g_stream, t, 0);
+        }
+        // Newton passes are queued ahead of the status check (the previous evaluation's
+        // largest step count + 1); a pass leaves finished tasks at once.  The
+        // r-marginalisation is queued with them and repeated if tasks were still iterating.
+        int queued = b.tile_passes, passes = 0;
+        while (true) {
+            for (int p = 0; p < queued; p++) {
+                KTimer k(RL2_K_LIKELIHOOD);
+                launch_tile_pass(g_stream, t, 1);
+            }
+            passes += queued;
+            {
+                KTimer k(RL2_K_RMARG);
+                launch_rmarg(g_stream, a);
+            }
+            launch_tile_status(g_stream, t);
+            hipMemcpyAsync(b.h_status, b.d_status, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                           g_stream);
+            copy_out();
+            if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+            if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+            if (b.h_status[0] == 0) break;
+            if (passes > 50) {  // the step count is capped at 50 (likelihood.c:284)
+                set_error("likelihood: Newton passes did not terminate");
+                return false;
+            }
+            queued = 2;
+        }
+        b.tile_passes = std::min(12, std::max(2, (int)b.h_status[1] + 1));
+    } else {
+        {
+            KTimer t(RL2_K_LIKELIHOOD);
+            launch_likelihood(g_stream, a);
+        }
+        {
+            KTimer t(RL2_K_RMARG);
+            launch_rmarg(g_stream, a);
+        }
+        if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+        copy_out();
+        if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
     }
-    if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
     memcpy(Le, b.h_Le, n * sizeof(double));
     memcpy(err, b.h_err, n * sizeof(int));
     if (Lr) memcpy(Lr, b.h_Lr, (size_t)n * kMaxR * sizeof(double));
@@ -415,7 +507,7 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
 {
     const int B = opt.batch > 0 ? opt.batch : 1;
     if (!o.ensure_slots(B)) return false;
-    o.lik_block = opt.lik_block > 0 ? opt.lik_block : 256;
+    o.lik_block = opt.lik_block > 0 ? opt.lik_block : 0;
     BatchEval eval = [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                          std::vector<int> &err) {
         return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);
@@ -853,7 +945,7 @@ int rl2_obs_fit(rl2_obs *obs, int slot, const rl2_fit_options *opt, rl2_gal_resu
         op.line_chunk = 4;
     }
     std::vector<FitRequest> reqs{{slot, res}};
-    o.lik_block = op.lik_block > 0 ? op.lik_block : 256;
+    o.lik_block = op.lik_block > 0 ? op.lik_block : 0;
     bool ok = fit_batch(reqs, op, [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                                       std::vector<int> &err) {
         return lik_eval(o, items, Le.data(), err.data(), nullptr, nullptr);

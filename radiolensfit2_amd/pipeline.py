@@ -30,10 +30,10 @@ def rprior_for(flux, ro):
     return rp
 
 
-def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=512,
+def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0,
                 lik_precision=0):
-    """rl2_fit_options; lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex +
-    FP64 sampling."""
+    """rl2_fit_options; lik_block 0: tiled likelihood, else threads per (e, r) work-group;
+    lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex + FP64 sampling."""
     return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0, lik_precision)
 
 

@@ -84,8 +84,9 @@ def test_fit_matches_oracle(dev, case, g):
     ro = P.r_grid()
     dev.load_facet(fv, fs, uu, vv, slot=1)
     dev.set_prior(P.rprior_for(cat.flux[g], ro), slot=1)
-    for spec in (0, 1):
-        r = dev.fit(slot=1, nm_speculate=spec, line_chunk=1 + 3 * spec)
+    # tiled likelihood without / with speculation, then one work-group per (e, r) task
+    for spec, blk in ((0, 0), (1, 0), (1, 256)):
+        r = dev.fit(slot=1, nm_speculate=spec, line_chunk=1 + 3 * spec, lik_block=blk)
         assert r.error == oerr
         assert abs(r.e1 - e1) <= 1e-8 and abs(r.e2 - e2) <= 1e-8, (r.e1, e1, r.e2, e2)
         assert abs(r.maxL - maxL) <= 1e-8
