@@ -157,6 +157,33 @@ int rl2_stats_get(int kernel, rl2_kernel_stat *out);
 /* Block until all work queued by the library has finished. */
 int rl2_synchronize(void);
 
+/* ---- learned simplex proposal ----
+ * No counterpart in the reference: a 770-256-256-256-9 MLP (GELU, BF16 MFMA products with
+ * FP32 accumulation, FP32 master weights, Adam) that reads a galaxy's binned facet and
+ * predicts the likelihood peak (e1, e2, log r) and a 3x3 Cholesky factor of its precision.
+ * It may only replace the simplex start point and first step (galaxy_fitting.c:59-67);
+ * rl2_fit_options.proposal enables it in rl2_process / rl2_obs_fit. */
+typedef struct rl2_proposal rl2_proposal;
+int rl2_proposal_num_params(void);
+int rl2_proposal_num_inputs(void);   /* 770: 16x16 (re, im, weight) bins, log flux, log R */
+int rl2_proposal_num_outputs(void);  /* 9: mu(3), log diag(3), off-diagonal (3) */
+int rl2_proposal_create(rl2_proposal **out, uint64_t seed);
+int rl2_proposal_destroy(rl2_proposal *p);
+int rl2_proposal_get_params(rl2_proposal *p, float *host);
+int rl2_proposal_set_params(rl2_proposal *p, const float *host);
+int rl2_proposal_get_grads(rl2_proposal *p, float *host);
+/* outputs[batch][9] for inputs[batch][770] */
+int rl2_proposal_forward(rl2_proposal *p, int batch, const float *inputs, float *outputs);
+/* mean Gaussian negative log-likelihood of targets[batch][3] = (e1, e2, log r) and its
+ * gradient (left in the network, rl2_proposal_get_grads) */
+int rl2_proposal_gradients(rl2_proposal *p, int batch, const float *inputs,
+                           const float *targets, float *mean_loss);
+/* one Adam step (lr) on the gradient of that loss */
+int rl2_proposal_train_step(rl2_proposal *p, int batch, const float *inputs,
+                            const float *targets, float lr, float *mean_loss);
+/* network inputs of the facet extracted into `slot` (catalog scalelength R_mu, flux) */
+int rl2_obs_features(rl2_obs *obs, int slot, double R_mu, double flux, float *out /* 770 */);
+
 #ifdef __cplusplus
 }
 #endif

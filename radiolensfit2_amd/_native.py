@@ -115,6 +115,20 @@ _HIP_SIGS = {
     "rl2_synchronize": (C.c_int, []),
     "rl2_fit_with_callback": (C.c_int, [LOGLIK_CB, C.c_void_p, C.c_int, C.c_int,
                                         C.POINTER(GalResult)]),
+    # learned proposal
+    "rl2_proposal_num_params": (C.c_int, []),
+    "rl2_proposal_num_inputs": (C.c_int, []),
+    "rl2_proposal_num_outputs": (C.c_int, []),
+    "rl2_proposal_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
+    "rl2_proposal_destroy": (C.c_int, [C.c_void_p]),
+    "rl2_proposal_get_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "rl2_proposal_set_params": (C.c_int, [C.c_void_p, c_float_p]),
+    "rl2_proposal_get_grads": (C.c_int, [C.c_void_p, c_float_p]),
+    "rl2_proposal_forward": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p]),
+    "rl2_proposal_gradients": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, c_float_p]),
+    "rl2_proposal_train_step": (C.c_int, [C.c_void_p, C.c_int, c_float_p, c_float_p, C.c_float,
+                                          c_float_p]),
+    "rl2_obs_features": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_double, c_float_p]),
     # reference-signature drop-ins (include/rl2_reference.h)
     "sersic_shape": (C.c_double, [C.c_double] * 4),
     "gaussian_shape": (C.c_double, [C.c_double] * 4),

@@ -31,8 +31,8 @@ COMMON = ["-O3", "-std=c++20", "-fPIC", "-ffp-contract=off", f"-I{INC}", f"-I{CS
 DEVICE = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-HIP_SOURCES = ["rl2_kernels.hip", "rl2_runtime.hip"]
-HOST_SOURCES = ["rl2_fit.cpp", "rl2_dropin.cpp"]
+HIP_SOURCES = ["rl2_kernels.hip", "rl2_runtime.hip", "rl2_mlp.hip"]
+HOST_SOURCES = ["rl2_fit.cpp", "rl2_dropin.cpp", "rl2_proposal.cpp"]
 
 
 def _run(cmd):

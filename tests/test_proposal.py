@@ -1,0 +1,141 @@
+"""Learned proposal network: the CPU oracle's backward pass against finite differences, and
+the device forward / backward kernels against the oracle.
+
+CPU: the oracle (oracle/mlp_oracle.py, float64, no bf16 rounding) is differentiated
+numerically at random parameter coordinates; the analytic gradient must agree to 1e-6
+relative.  GPU: the device network (BF16 MFMA operands, FP32 accumulation) loaded with the
+same parameters gives the oracle's outputs and gradients with bf16-rounded operands to a
+relative Frobenius error below 1e-2 (bf16 has an 8-bit mantissa), and the MFMA product
+alone matches a float64 product of the rounded operands to 1e-5.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+
+
+def _batch(B, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(B, M.N_IN)).astype(np.float32)
+    y = np.column_stack([rng.uniform(-0.5, 0.5, B), rng.uniform(-0.5, 0.5, B),
+                         rng.uniform(-1, 1, B)]).astype(np.float32)
+    return x, y
+
+
+def test_oracle_layout():
+    assert M.n_params() == 256 * 770 + 256 + 2 * (256 * 256 + 256) + 9 * 256 + 9
+
+
+def test_oracle_gradient_matches_finite_differences():
+    p = M.init_params(3)
+    # larger output layer so every term of the loss is exercised
+    W4, b4 = M.unpack(p)[3]
+    W4 *= 10
+    b4[:] = np.random.default_rng(1).normal(scale=0.3, size=b4.shape)
+    x, y = _batch(6, 4)
+    x, y = x.astype(np.float64), y.astype(np.float64)
+    loss, g = M.gradients(p, x, y, bf16_ops=False)
+    rng = np.random.default_rng(5)
+    # coordinates from every layer's weights and biases
+    idx = []
+    k = 0
+    for i, o in M.DIMS:
+        idx += list(k + rng.integers(0, o * i, 6))
+        idx += list(k + o * i + rng.integers(0, o, 2))
+        k += o * i + o
+    for j in idx:
+        h = 1e-6 * max(1.0, abs(p[j]))
+        pp, pm = p.copy(), p.copy()
+        pp[j] += h
+        pm[j] -= h
+        lp = M.nll(M.forward(pp, x, False)[0], y)[0].mean()
+        lm = M.nll(M.forward(pm, x, False)[0], y)[0].mean()
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[j]) <= 1e-6 * max(1e-3, abs(fd)) + 1e-9, (j, fd, g[j])
+
+
+def test_bf16_rounding():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 3.0e-3], np.float32)
+    r = M.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even
+    assert r[2] == 1.0 + 2 ** -7         # above the tie rounds up
+    assert r[3] == -2.5
+    assert abs(r[4] - 3.0e-3) < 3.0e-3 * 2 ** -8
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.fixture(scope="module")
+def net(gpu):
+    from radiolensfit2_amd import _native as N
+    H = N.hip()
+    h = C.c_void_p()
+    N.check(H.rl2_proposal_create(C.byref(h), 7), "rl2_proposal_create")
+    yield H, h
+    H.rl2_proposal_destroy(h)
+
+
+@pytest.mark.gpu
+def test_gpu_forward_matches_oracle(net):
+    H, h = net
+    p = M.init_params(11).astype(np.float32)
+    W4, b4 = M.unpack(p)[3]
+    W4 *= 10
+    H.rl2_proposal_set_params(h, p.ctypes.data_as(C.POINTER(C.c_float)))
+    x, _ = _batch(40, 12)
+    out = np.zeros((40, M.N_OUT), np.float32)
+    print("proposal forward, batch 40", flush=True)
+    assert H.rl2_proposal_forward(h, 40, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                  out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+    ref, _ = M.forward(p.astype(np.float64), x.astype(np.float64), bf16_ops=True)
+    assert _rel(out, ref) < 1e-2
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_oracle(net):
+    H, h = net
+    p = M.init_params(13).astype(np.float32)
+    W4, b4 = M.unpack(p)[3]
+    W4 *= 10
+    H.rl2_proposal_set_params(h, p.ctypes.data_as(C.POINTER(C.c_float)))
+    x, y = _batch(64, 14)
+    loss = C.c_float()
+    print("proposal gradients, batch 64", flush=True)
+    assert H.rl2_proposal_gradients(h, 64, x.ctypes.data_as(C.POINTER(C.c_float)),
+                                    y.ctypes.data_as(C.POINTER(C.c_float)), C.byref(loss)) == 0
+    g = np.zeros(M.n_params(), np.float32)
+    H.rl2_proposal_get_grads(h, g.ctypes.data_as(C.POINTER(C.c_float)))
+    ref_loss, ref_g = M.gradients(p.astype(np.float64), x.astype(np.float64),
+                                  y.astype(np.float64), bf16_ops=True)
+    assert abs(loss.value - ref_loss) <= 1e-2 * max(1.0, abs(ref_loss))
+    k = 0
+    for l, (i, o) in enumerate(M.DIMS):  # per layer: weights, then biases
+        assert _rel(g[k:k + o * i], ref_g[k:k + o * i]) < 2e-2, ("W", l)
+        k += o * i
+        assert _rel(g[k:k + o], ref_g[k:k + o]) < 2e-2, ("b", l)
+        k += o
+
+
+@pytest.mark.gpu
+def test_gpu_training_reduces_loss(net):
+    """A linear-in-features target is learnable: Adam steps lower the loss."""
+    H, h = net
+    p = M.init_params(17).astype(np.float32)
+    H.rl2_proposal_set_params(h, p.ctypes.data_as(C.POINTER(C.c_float)))
+    rng = np.random.default_rng(18)
+    x = rng.normal(size=(256, M.N_IN)).astype(np.float32)
+    A = rng.normal(scale=0.05, size=(M.N_IN, 3))
+    y = np.tanh(x @ A).astype(np.float32)
+    fp = C.POINTER(C.c_float)
+    losses = []
+    for it in range(60):
+        loss = C.c_float()
+        assert H.rl2_proposal_train_step(h, 256, x.ctypes.data_as(fp), y.ctypes.data_as(fp),
+                                         C.c_float(2e-3), C.byref(loss)) == 0
+        losses.append(loss.value)
+    print("loss", losses[0], "->", losses[-1], flush=True)
+    assert losses[-1] < losses[0] - 1.0
