@@ -152,6 +152,10 @@ def main():
     ap.add_argument("--line-chunk", type=int, default=4)
     ap.add_argument("--lik-block", type=int, default=0,
                     help="0: tiled likelihood; 64..1024: one work-group per (e, r) task")
+    ap.add_argument("--tile-cells", type=int, default=0,
+                    help="tiled likelihood cells per work-group (0: chosen per launch)")
+    ap.add_argument("--grid-variant", type=int, default=0, choices=[0, 1],
+                    help="extraction gridding: 0 per-visibility LDS hash, 1 wave-aggregated")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -180,6 +184,8 @@ def main():
         sys.exit(f"steps*batch = {n_gal} galaxies, catalog has {args.ngal}")
     obs, cat = make_geometry_and_catalog(cfg, args.ngal)
     dev = make_device_observation(obs, cat, res_prec)
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_CELLS, args.tile_cells), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, args.grid_variant), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)

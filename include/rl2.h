@@ -157,6 +157,15 @@ int rl2_stats_get(int kernel, rl2_kernel_stat *out);
 /* Block until all work queued by the library has finished. */
 int rl2_synchronize(void);
 
+/* ---- launch-shape knobs (set by an autotuner; process-wide) ----
+ * RL2_TUNE_GRID_VARIANT: extraction gridding, 0 per-visibility LDS hash (default),
+ *   1 wave-aggregated cells.
+ * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
+ *   in [2048, 65536], or 0 (default) to choose per launch. */
+enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1 };
+int rl2_tune_set(int knob, int value);
+int rl2_tune_get(int knob, int *value);
+
 /* ---- learned simplex proposal ----
  * No counterpart in the reference: a 770-256-256-256-9 MLP (GELU, BF16 MFMA products with
  * FP32 accumulation, FP32 master weights, Adam) that reads a galaxy's binned facet and

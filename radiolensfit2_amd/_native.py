@@ -25,6 +25,8 @@ c_bool_p = C.POINTER(C.c_bool)
 RL2_OK = 0
 RL2_PREC_FP64 = 0
 RL2_PREC_MIXED = 1
+RL2_TUNE_GRID_VARIANT = 0
+RL2_TUNE_TILE_CELLS = 1
 (RL2_K_EXTRACT, RL2_K_COMPACT, RL2_K_LIKELIHOOD, RL2_K_RMARG, RL2_K_SUBTRACT,
  RL2_K_SOURCES) = range(6)
 KERNEL_NAMES = ["extract", "compact", "likelihood", "rmarg", "subtract", "sources"]
@@ -113,6 +115,8 @@ _HIP_SIGS = {
     "rl2_stats_reset": (C.c_int, []),
     "rl2_stats_get": (C.c_int, [C.c_int, C.POINTER(KernelStat)]),
     "rl2_synchronize": (C.c_int, []),
+    "rl2_tune_set": (C.c_int, [C.c_int, C.c_int]),
+    "rl2_tune_get": (C.c_int, [C.c_int, c_int_p]),
     "rl2_fit_with_callback": (C.c_int, [LOGLIK_CB, C.c_void_p, C.c_int, C.c_int,
                                         C.POINTER(GalResult)]),
     # learned proposal

@@ -79,6 +79,14 @@ struct GridLaunch {
 };
 void launch_grid(hipStream_t s, const GridLaunch &a);
 
+// Launch-shape knobs (rl2_tune_set): read by the launchers on the host.
+struct Tuning {
+    int grid_variant = 0;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
+                               // 1 wave-aggregated cells
+    unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
+};
+Tuning &tuning();
+
 // Compaction of dense accumulators into the facet arrays; clears the accumulators.
 // n_out (device) receives the number of non-empty cells. sum_w_dense (optional, device)
 // receives the dense weights before clearing.
@@ -115,28 +123,29 @@ constexpr int kQagInts = 2 * kQagLimit;
 void launch_likelihood(hipStream_t s, const LikLaunch &a);
 void launch_rmarg(hipStream_t s, const LikLaunch &a);
 
-// Tiled likelihood.  Every (item, scalelength) task is split into chunks of kTileCells
+// Tiled likelihood.  Every (item, scalelength) task is split into chunks of chunk_cells
 // facet cells with one work-group per chunk.  A pass (the zero-offset sums, or one Newton
-// iteration) is one launch: each work-group writes its chunk's sums, and the last
-// work-group of a task to arrive adds the chunk sums in chunk order and takes the Newton
-// step or finishes L_r.  The r-marginalisation follows as its own launch.
-constexpr int kTileBS = 256, kTileCPT = 8, kTileCells = kTileBS * kTileCPT;
+// iteration) is two launches: every work-group writes its chunk's sums, then one wave per
+// task adds them in chunk order and takes the Newton step or finishes L_r.  The
+// r-marginalisation follows as its own launch.
+constexpr int kTileBS = 256;
+constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
+constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
 struct TileTask {
     double xo, yo;             // position offset of the current Newton iterate
     double norm;               // |increment| of the last step (1 before the first)
     double S2;                 // sum M^2 / sigma^2 of the unnormalised model
     int iter;                  // Newton steps taken
     int active;                // 1 while another Newton pass is due
-    unsigned arrivals;         // work-groups of the current pass that have finished
-    unsigned pad;
 };
 struct TileLaunch {
     LikLaunch lk;              // items, facets, r grid, outputs and QAG workspace
     const unsigned *wg_off;    // nitems + 1: first work-group of each item (nr * chunks)
     TileTask *tasks;           // nitems * kMaxR
     double *partials;          // 8 doubles per work-group
-    unsigned *status;         // [0] tasks still active, [1] largest Newton step count
+    unsigned *status;          // [0] tasks still active, [1] largest Newton step count
     unsigned nwg;
+    unsigned chunk_cells;      // facet cells per work-group
 };
 void launch_tile_init(hipStream_t s, const TileLaunch &t);
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);

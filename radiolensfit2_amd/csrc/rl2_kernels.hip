@@ -162,8 +162,113 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 }
 
 // ------------------------------------------------------------------ extraction + gridding
-// One work-group owns RPT*256 consecutive (sorted) rows, each thread RPT consecutive rows.
-// Sorted rows are neighbours in uv, so most visibilities of a wave fall in a few cells.
+// Variant 0.  One work-group owns RPT*256 consecutive (sorted) rows, thread t rows
+// t, t+256, ...  Per channel each thread adds its RPT visibilities into an LDS
+// open-addressing table keyed by cell index (at most RPT*256 distinct keys, table size
+// 2*RPT*256, so probing always terminates), then the table is flushed to the global
+// accumulators with one atomic per distinct cell.  Sorted rows make neighbouring threads
+// hit the same cells, so hot cells near the uv origin receive one global atomic per
+// work-group instead of one per visibility.
+template <int MODE, int PREC, int RPT>
+__global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
+{
+    constexpr int BS = 256;
+    constexpr int HS = 2 * RPT * BS;
+    constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
+    static_assert((1 << HBITS) == HS, "hash size must be a power of two");
+    __shared__ uint32_t hkey[HS];
+    __shared__ double hre[HS], him[HS], hs2[HS];
+    __shared__ uint32_t hcnt[HS];
+    const int tid = threadIdx.x;
+    for (int s = tid; s < HS; s += BS) {
+        hkey[s] = kEmpty;
+        hre[s] = 0.;
+        him[s] = 0.;
+        hs2[s] = 0.;
+        hcnt[s] = 0;
+    }
+    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS);
+    double ru[RPT], rv[RPT], rp[RPT];
+    bool rok[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        unsigned long i = row0 + tid + (unsigned long)k * BS;
+        rok[k] = i < a.nrow;
+        ru[k] = rok[k] ? a.u[i] : 0.;
+        rv[k] = rok[k] ? a.v[i] : 0.;
+        double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
+        rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
+    }
+    const unsigned long ncell = (unsigned long)a.facet * a.facet;
+    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
+           *acc_w = a.acc + 3 * ncell;
+    __syncthreads();
+    for (unsigned c = 0; c < a.nch; c++) {
+        const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
+        const double wn2 = wn * wn, spectra = a.spec[c];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            if (!rok[k]) continue;
+            const size_t idx = (size_t)c * a.nrow + row0 + tid + (size_t)k * BS;
+            const float s2 = a.s2f[idx];
+            if (signbit(s2)) continue;  // flagged
+            uint32_t key;
+            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key)) continue;
+            double xr = 0., xi = 0.;
+            if (MODE == 0) {
+                // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
+                // (source_extraction.c:51-65).
+                double phase = wn * rp[k];
+                double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
+                double sn, cs;
+                sincos(phase, &sn, &cs);
+                double2 r = load_res(a.res, PREC, idx);
+                double gre = shape * cs + r.x;
+                double gim = shape * sn + r.y;
+                xr = gre * cs - gim * (-sn);
+                xi = gre * (-sn) + gim * cs;
+            } else if (MODE == 1) {
+                double2 r = load_res(a.res, PREC, idx);
+                xr = r.x;
+                xi = r.y;
+            }
+            uint32_t h = (key * 2654435761u) >> (32 - HBITS);
+            while (true) {
+                uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                if (prev == kEmpty || prev == key) break;
+                h = (h + 1) & (HS - 1);
+            }
+            if (MODE != 2) {
+                atomicAdd(&hre[h], xr);
+                atomicAdd(&him[h], xi);
+            }
+            atomicAdd(&hs2[h], (double)s2);
+            atomicAdd(&hcnt[h], 1u);
+        }
+        __syncthreads();
+        for (int s = tid; s < HS; s += BS) {
+            uint32_t key = hkey[s];
+            if (key != kEmpty) {
+                if (MODE != 2) {
+                    unsafeAtomicAdd(&acc_re[key], hre[s]);
+                    unsafeAtomicAdd(&acc_im[key], him[s]);
+                }
+                unsafeAtomicAdd(&acc_s2[key], hs2[s]);
+                unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+                hkey[s] = kEmpty;
+                hre[s] = 0.;
+                him[s] = 0.;
+                hs2[s] = 0.;
+                hcnt[s] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Variant 1.  One work-group owns RPT*256 consecutive (sorted) rows, each thread RPT
+// consecutive rows.  Sorted rows are neighbours in uv, so most visibilities of a wave fall
+// in a few cells.
 // Per channel:
 //  1. each thread merges runs of equal cells among its own RPT visibilities;
 //  2. per row slot, the wave sums the values of lanes sharing a cell (one pass per
@@ -175,7 +280,7 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 // LDS atomics are therefore issued once per distinct cell per wave instead of once per
 // visibility (same-address LDS atomics serialise across the 64 lanes).
 template <int MODE, int PREC, int RPT>
-__global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
+__global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
 {
     constexpr int BS = 256;
     constexpr int HS = RPT * BS;
@@ -325,19 +430,28 @@ __global__ __launch_bounds__(256) void grid_kernel(GridLaunch a)
     }
 }
 
-void launch_grid(hipStream_t s, const GridLaunch &a)
+template <int MODE, int PREC>
+static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
 {
     constexpr int RPT = 4;
     dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+    if (tuning().grid_variant == 1)
+        grid_wave_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
+    else
+        grid_direct_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
+}
+
+void launch_grid(hipStream_t s, const GridLaunch &a)
+{
     if (a.mode == 0) {
         if (a.prec == 0)
-            grid_kernel<0, 0, RPT><<<grid, 256, 0, s>>>(a);
+            launch_grid_variant<0, 0>(s, a);
         else
-            grid_kernel<0, 1, RPT><<<grid, 256, 0, s>>>(a);
+            launch_grid_variant<0, 1>(s, a);
     } else if (a.mode == 1) {
-        grid_kernel<1, 0, RPT><<<grid, 256, 0, s>>>(a);
+        launch_grid_variant<1, 0>(s, a);
     } else {
-        grid_kernel<2, 0, RPT><<<grid, 256, 0, s>>>(a);
+        launch_grid_variant<2, 0>(s, a);
     }
 }
 
@@ -1459,7 +1573,6 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
         dyy = v[4];
         dxy = v[5];
     }
-    T->arrivals = 0;
     const double det = dxx * dyy - dxy * dxy;
     if (det > 0 && T->norm > 1.e-7 && T->iter < 50) {
         const double incx = (dx * dyy - dy * dxy) / det;
@@ -1481,7 +1594,6 @@ template <int PASS>
 __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
-    __shared__ int s_last;
     const LikLaunch &a = t.lk;
     const unsigned wg = blockIdx.x;
     const int item = tile_item(t.wg_off, a.nitems, wg);
@@ -1491,12 +1603,12 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
     const unsigned local = wg - first;
     const int r = (int)(local / nch) + 1;
     const unsigned chunk = local - (unsigned)(r - 1) * nch;
-    TileTask *T = t.tasks + (size_t)item * kMaxR + r;
+    const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;  // finished task: the whole work-group leaves
     const LikItem it = a.items[item];
     const FacetDev &F = a.facets[it.slot];
-    const unsigned beg = chunk * kTileCells;
-    const unsigned end = min(F.n, beg + kTileCells);
+    const unsigned beg = chunk * t.chunk_cells;
+    const unsigned end = min(F.n, beg + t.chunk_cells);
     const double xo = PASS == 1 ? T->xo : 0., yo = PASS == 1 ? T->yo : 0.;
     double s[7];
     if (it.prec == 0)
@@ -1504,25 +1616,37 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
     else
         tile_sums32<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
     block_allreduce<kTileBS, 7>(s, red);
-    const int tid = threadIdx.x;
-    if (tid == 0) {
+    if (threadIdx.x == 0) {
         double *P = t.partials + (size_t)wg * 8;
 #pragma unroll
         for (int j = 0; j < 7; j++) P[j] = s[j];
-        __threadfence();  // release the chunk sums before counting the arrival
-        const unsigned prev = atomicAdd(&T->arrivals, 1u);
-        s_last = (prev + 1 == nch);
     }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();  // acquire the other work-groups' chunk sums
+}
+
+// One wave per (item, r) task: the task's chunk sums added in a fixed order (lanes over
+// chunks, then a butterfly), then the Newton step or the end of the task.
+template <int PASS>
+__global__ __launch_bounds__(256) void lik_step_kernel(TileLaunch t)
+{
+    const LikLaunch &a = t.lk;
+    const int nr = a.rg.numr - 1;
+    const unsigned k = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= (unsigned)a.nitems * nr) return;
+    const int item = (int)(k / nr), r = (int)(k % nr) + 1;
+    TileTask *T = t.tasks + (size_t)item * kMaxR + r;
+    if (PASS == 1 && !T->active) return;
+    const unsigned first = t.wg_off[item];
+    const unsigned nch = (t.wg_off[item + 1] - first) / nr;
     const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1) * nch) * 8;
     double v[7] = {0., 0., 0., 0., 0., 0., 0.};
-    for (unsigned c = tid; c < nch; c += kTileBS)
+    for (unsigned c = lane; c < nch; c += 64)
 #pragma unroll
         for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * 8 + j];
-    block_allreduce<kTileBS, 7>(v, red);
-    if (tid == 0) tile_step<PASS>(a, T, item, r, v);
+#pragma unroll
+    for (int j = 0; j < 7; j++)
+        for (int off = 32; off >= 1; off >>= 1) v[j] += __shfl_xor(v[j], off);
+    if (lane == 0) tile_step<PASS>(a, T, item, r, v);
 }
 
 __global__ void tile_init_kernel(TileLaunch t)
@@ -1530,7 +1654,6 @@ __global__ void tile_init_kernel(TileLaunch t)
     const unsigned ntask = (unsigned)t.lk.nitems * kMaxR;
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask;
          k += gridDim.x * blockDim.x) {
-        t.tasks[k].arrivals = 0;
         t.tasks[k].active = 0;
         t.tasks[k].iter = 0;
     }
@@ -1571,10 +1694,14 @@ void launch_tile_init(hipStream_t s, const TileLaunch &t)
 
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
-    if (pass == 0)
+    const unsigned ntask = (unsigned)t.lk.nitems * (t.lk.rg.numr - 1);
+    if (pass == 0) {
         lik_tile_kernel<0><<<t.nwg, kTileBS, 0, s>>>(t);
-    else
+        lik_step_kernel<0><<<(ntask + 3) / 4, 256, 0, s>>>(t);
+    } else {
         lik_tile_kernel<1><<<t.nwg, kTileBS, 0, s>>>(t);
+        lik_step_kernel<1><<<(ntask + 3) / 4, 256, 0, s>>>(t);
+    }
 }
 
 void launch_tile_status(hipStream_t s, const TileLaunch &t)

@@ -70,6 +70,11 @@ bool hip_ok(hipError_t e, const char *what)
     return false;
 }
 hipStream_t stream() { return g_stream; }
+Tuning &tuning()
+{
+    static Tuning t;
+    return t;
+}
 
 KTimer::KTimer(int k) : kernel(k)
 {
@@ -399,13 +404,25 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         }
     };
     if (tiled) {
-        // chunks per (item, r) from the host copy of each slot's cell count
+        // chunks per (item, r) from the host copy of each slot's cell count; the chunk
+        // doubles from kTileMinCells while the launch keeps >= kTileTargetGroups groups
         const int nr = o.rg.numr - 1;
+        auto groups = [&](unsigned chunk) {
+            unsigned long long g = 0;
+            for (int i = 0; i < n; i++)
+                g += std::max(1u, (o.slots[items[i].slot].n + chunk - 1) / chunk);
+            return g * (unsigned)nr;
+        };
+        unsigned chunk = tuning().tile_cells;
+        if (chunk == 0) {
+            chunk = kTileMinCells;
+            while (chunk < kTileMaxCells && groups(2 * chunk) >= kTileTargetGroups) chunk *= 2;
+        }
         unsigned total = 0;
         b.h_wg_off[0] = 0;
         for (int i = 0; i < n; i++) {
             const unsigned cells = o.slots[items[i].slot].n;
-            const unsigned nchunk = std::max(1u, (cells + kTileCells - 1) / kTileCells);
+            const unsigned nchunk = std::max(1u, (cells + chunk - 1) / chunk);
             total += nchunk * (unsigned)nr;
             b.h_wg_off[i + 1] = total;
         }
@@ -419,6 +436,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         t.partials = b.d_partials;
         t.status = b.d_status;
         t.nwg = total;
+        t.chunk_cells = chunk;
         launch_tile_init(g_stream, t);
         {
             KTimer k(RL2_K_LIKELIHOOD);
@@ -1020,6 +1038,37 @@ int rl2_synchronize(void)
 {
     if (!g_stream) return RL2_OK;
     return hip_ok(hipStreamSynchronize(g_stream), "synchronize") ? RL2_OK : RL2_ERR_HIP;
+}
+
+int rl2_tune_set(int knob, int value)
+{
+    switch (knob) {
+    case RL2_TUNE_GRID_VARIANT:
+        if (value != 0 && value != 1) break;
+        tuning().grid_variant = value;
+        return RL2_OK;
+    case RL2_TUNE_TILE_CELLS:
+        if (value != 0 && (value < (int)kTileMinCells || value > (int)kTileMaxCells ||
+                           (value & (value - 1))))
+            break;
+        tuning().tile_cells = (unsigned)value;
+        return RL2_OK;
+    default:
+        break;
+    }
+    set_error("rl2_tune_set: knob " + std::to_string(knob) + " does not take " +
+              std::to_string(value));
+    return RL2_ERR_ARG;
+}
+
+int rl2_tune_get(int knob, int *value)
+{
+    if (!value) return RL2_ERR_ARG;
+    switch (knob) {
+    case RL2_TUNE_GRID_VARIANT: *value = tuning().grid_variant; return RL2_OK;
+    case RL2_TUNE_TILE_CELLS: *value = (int)tuning().tile_cells; return RL2_OK;
+    default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
+    }
 }
 
 }  // extern "C"

@@ -25,14 +25,16 @@ def _case(c):
 
 
 def _grid(lib, g, fn="gridding_visibilities"):
+    """Grid a fixture case with the oracle (fn orl2_gridding) or librl2hip's drop-in."""
     facet = int(g["facet"])
     ncell = facet * facet
     nv, ns, sw = np.zeros(ncell, np.complex128), np.zeros(ncell), np.zeros(ncell)
     vis = np.ascontiguousarray(g["vis"])
     s2 = np.ascontiguousarray(g["s2"])
     flag = np.ascontiguousarray(g["flag"])
+    cp = O.cptr if fn.startswith("orl2") else N.cplx_s
     getattr(lib, fn)(O.ptr(g["wn"]), len(g["wn"]), len(g["u"]), O.ptr(g["u"]), O.ptr(g["v"]),
-                     O.cptr(vis), O.ptr(s2, O.c_float_p), float(g["len"]), facet, O.cptr(nv),
+                     cp(vis), O.ptr(s2, O.c_float_p), float(g["len"]), facet, cp(nv),
                      O.ptr(ns), O.ptr(flag, O.c_bool_p), O.ptr(sw))
     return nv, ns, sw
 
@@ -63,27 +65,10 @@ def test_oracle_reproduces_reference_average_facets():
     assert np.array_equal(v, GOLD["avg_vis"]) and np.array_equal(s, GOLD["avg_s2"])
 
 
-def _hip_sigs(H):
-    dp, fp, bp = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_bool)
-    H.gridding_visibilities.argtypes = [dp, C.c_uint, C.c_uint, dp, dp, dp, fp, C.c_double,
-                                        C.c_uint, dp, dp, bp, dp]
-    H.gridding_visibilities.restype = None
-    H.evaluate_facet_coords.argtypes = [dp, dp, C.c_double, C.c_uint, dp]
-    H.evaluate_facet_coords.restype = C.c_uint
-    H.evaluate_uv_grid_size.argtypes = [C.c_int, C.c_int, C.c_double, dp, C.c_uint, C.c_uint,
-                                        dp, dp, C.c_uint, bp]
-    H.evaluate_uv_grid_size.restype = C.c_uint
-    H.average_facets.argtypes = [C.c_ulong, dp, dp, dp]
-    H.average_facets.restype = None
-    H.facet_size.argtypes = [C.c_double, C.c_double]
-    H.facet_size.restype = C.c_uint
-
-
 @pytest.mark.gpu
 @pytest.mark.parametrize("c", CASES)
 def test_gpu_gridding_matches_reference_fixtures(gpu, c):
     H = N.hip()
-    _hip_sigs(H)
     g = _case(c)
     print(f"golden case {c}: facet {int(g['facet'])}, rows {len(g['u'])}", flush=True)
     nv, ns, sw = _grid(H, g)
@@ -107,8 +92,7 @@ def test_gpu_gridding_matches_reference_fixtures(gpu, c):
 @pytest.mark.gpu
 def test_gpu_average_facets_matches_reference_fixture(gpu):
     H = N.hip()
-    _hip_sigs(H)
     v, s = GOLD["avg_vis_in"].copy(), GOLD["avg_s2_in"].copy()
-    H.average_facets(len(v), O.cptr(v), O.ptr(s), O.ptr(GOLD["avg_w"]))
+    H.average_facets(len(v), N.cplx_s(v), O.ptr(s), O.ptr(GOLD["avg_w"]))
     np.testing.assert_array_equal(v, GOLD["avg_vis"])
     np.testing.assert_array_equal(s, GOLD["avg_s2"])
