@@ -8,10 +8,14 @@ Workload (default): 197 dishes (dense 1 km core, log-radial to 150 km), 19,306 b
 path is RadioLensfit2's: extract -> fit (simplex + sampling) -> subtract, galaxies in
 decreasing flux order, failed fits re-fitted after the others.
 
-A step is one batch of --batch galaxies through that path.  The residual is checkpointed
-after set-up; --warmup steps run and the checkpoint is restored, so every configuration
-times exactly --steps steps over the same first steps*batch galaxies (their bad-list
-re-fit included), inputs resident in HBM.  One JSON line is printed.
+A step is one batch of --batch galaxies through that path.  The timed galaxies are every
+--stride-th galaxy of the flux-ordered catalog (default every 10th: 100 galaxies spanning
+the whole flux and size range), processed in flux order against the residual of the full
+catalog.  The residual is checkpointed after set-up; --warmup steps run and the checkpoint
+is restored, so every configuration times exactly --steps steps over the same galaxies
+(their bad-list re-fit included), inputs resident in HBM.  The same galaxies are then run
+again with per-launch HIP events on the library stream for the kernel rooflines.  One JSON
+line is printed.
 
 Correctness figures: --cpu-sample galaxies re-run on the FP64 CPU oracle from the same
 residual (CPU baseline + parity); --compare-fp64 re-runs the timed galaxies with the FP64
@@ -45,8 +49,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def subset(cat, lo, hi):
-    return synth.Catalog(*(np.ascontiguousarray(getattr(cat, f)[lo:hi]) for f in
+def subset(cat, lo, hi, stride=1):
+    """Galaxies lo, lo + stride, ... below hi of a flux-ordered catalog (still flux-ordered)."""
+    return synth.Catalog(*(np.ascontiguousarray(getattr(cat, f)[lo:hi:stride]) for f in
                            ("flux", "scale", "e1", "e2", "l", "m", "snr")))
 
 
@@ -135,7 +140,7 @@ def roofline(kstats, lik_peak):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=1)
     ap.add_argument("--residual", choices=["c128", "c64"], default="c128",
@@ -144,6 +149,9 @@ def main():
                     help="0 FP64; 1 mixed throughout; 2 mixed simplex + FP64 sampling")
     ap.add_argument("--config", choices=["197dish", "64dish"], default="197dish")
     ap.add_argument("--ngal", type=int, default=1000)
+    ap.add_argument("--stride", type=int, default=10,
+                    help="timed galaxies are every stride-th of the flux-ordered catalog, so "
+                         "the sample spans the whole flux and size range")
     ap.add_argument("--cpu-sample", type=int, default=1,
                     help="galaxies re-run on the FP64 CPU oracle (0: skip)")
     ap.add_argument("--compare-fp64", action="store_true",
@@ -180,20 +188,29 @@ def main():
     res_prec = N.RL2_PREC_FP64 if args.residual == "c128" else N.RL2_PREC_MIXED
     B = args.batch
     n_gal = args.steps * B
-    if max(n_gal, args.warmup * B) > args.ngal:
-        sys.exit(f"steps*batch = {n_gal} galaxies, catalog has {args.ngal}")
-    obs, cat = make_geometry_and_catalog(cfg, args.ngal)
-    dev = make_device_observation(obs, cat, res_prec)
+    if max(n_gal, args.warmup * B) * args.stride > args.ngal:
+        sys.exit(f"steps*batch*stride = {n_gal * args.stride} galaxies, catalog has {args.ngal}")
+    obs, full_cat = make_geometry_and_catalog(cfg, args.ngal)
+    cat = subset(full_cat, 0, args.ngal, args.stride)  # the timed sub-catalog
+    dev = make_device_observation(obs, full_cat, res_prec)
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_CELLS, args.tile_cells), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, args.grid_variant), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)
     resid0 = dev.residual() if args.cpu_sample > 0 else None
-    res, st, elapsed, kstats = run_path(dev, cat, n_gal, args.warmup * B, opts)
+    res, st, elapsed, _ = run_path(dev, cat, n_gal, args.warmup * B, opts, timed=False)
     value = n_gal / elapsed
     log(f"[bench] timed {args.steps} steps ({n_gal} galaxies): {elapsed:.3f}s -> "
         f"{value:.2f} gal/s; bad first/final {st.n_bad_first}/{st.n_bad_final}")
+    # the same galaxies again from the same residual with per-launch HIP events on the
+    # library stream (kept out of the timed run: event records add host work per launch)
+    dev.restore()
+    res_i, _, elapsed_i, kstats = run_path(dev, cat, n_gal, 0, opts, timed=True)
+    rerun_de = np.array([max(abs(res_i[k].e1 - res[k].e1), abs(res_i[k].e2 - res[k].e2))
+                         for k in range(n_gal)])
+    log(f"[bench] instrumented re-run: {elapsed_i:.3f}s; |de| between the two runs: max "
+        f"{rerun_de.max():.3e}, galaxies above 1e-10: {int((rerun_de > 1e-10).sum())}")
     for k, s in kstats.items():
         log(f"[bench]   {k:10s} {s['ms']:10.2f} ms  {s['launches']:7d} launches")
     mixed = args.lik_precision != 0 or res_prec != N.RL2_PREC_FP64
@@ -205,7 +222,7 @@ def main():
         if mixed:
             if res_prec != N.RL2_PREC_FP64:
                 dev.close()
-                dev64 = make_device_observation(obs, cat, N.RL2_PREC_FP64)
+                dev64 = make_device_observation(obs, full_cat, N.RL2_PREC_FP64)
             else:
                 dev64 = dev
                 dev64.restore()
@@ -254,7 +271,8 @@ def main():
         "config": {
             "workload": (f"{cfg.n_dish} dishes, {obs.nrow // cfg.n_times} baselines x "
                          f"{cfg.n_times} times x {cfg.n_chan} ch ({obs.nvis / 1e6:.1f}M vis), "
-                         f"{args.ngal}-galaxy catalog; timed galaxies 0..{n_gal - 1}"),
+                         f"{args.ngal}-galaxy catalog; timed galaxies: every {args.stride}th "
+                         f"of the flux-ordered catalog, {n_gal} galaxies"),
             "batch": B,
             "path": ("plain" if B == 1 else "batched") + ("-mixed" if mixed else "-fp64"),
             "residual": args.residual,
@@ -268,6 +286,8 @@ def main():
         "parity": parity,
         "acceptance_vs_fp64": accept,
         "bad_fits": {"first_pass": st.n_bad_first, "after_refit": st.n_bad_final},
+        "rerun_reproducibility": {"max_abs_de": float(rerun_de.max()),
+                                  "galaxies_above_1e-10": int((rerun_de > 1e-10).sum())},
     }
     print(json.dumps(out), flush=True)
 

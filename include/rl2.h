@@ -166,6 +166,18 @@ enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 
+/* ---- test hooks (host-side; no GPU needed) ----
+ * The fit control flow driven by a caller-supplied log-likelihood; returns the number of
+ * batched evaluation rounds. */
+int rl2_fit_with_callback(double (*loglik)(double e1, double e2, int *error, void *ctx),
+                          void *ctx, int nm_speculate, int line_chunk, rl2_gal_result *res);
+/* sin(2 pi t), cos(2 pi t) as the FP64 kernels evaluate them. */
+void rl2_test_sincos_rev64(int n, const double *t, double *s, double *c);
+/* Cell indices by the kernels' reciprocal-product form and by the reference's division;
+ * returns the number of disagreements (0xffffffff marks a cell outside the facet). */
+int rl2_test_cell_index(int n, const double *u, const double *v, double inv_lambda, double len,
+                        double inc, unsigned int facet, unsigned int *fast, unsigned int *exact);
+
 /* ---- learned simplex proposal ----
  * No counterpart in the reference: a 770-256-256-256-9 MLP (GELU, BF16 MFMA products with
  * FP32 accumulation, FP32 master weights, Adam) that reads a galaxy's binned facet and

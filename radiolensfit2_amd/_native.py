@@ -116,6 +116,9 @@ _HIP_SIGS = {
     "rl2_stats_get": (C.c_int, [C.c_int, C.POINTER(KernelStat)]),
     "rl2_synchronize": (C.c_int, []),
     "rl2_tune_set": (C.c_int, [C.c_int, C.c_int]),
+    "rl2_test_sincos_rev64": (None, [C.c_int, c_double_p, c_double_p, c_double_p]),
+    "rl2_test_cell_index": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double, C.c_double,
+                                      C.c_double, C.c_uint, c_uint_p, c_uint_p]),
     "rl2_tune_get": (C.c_int, [C.c_int, c_int_p]),
     "rl2_fit_with_callback": (C.c_int, [LOGLIK_CB, C.c_void_p, C.c_int, C.c_int,
                                         C.POINTER(GalResult)]),

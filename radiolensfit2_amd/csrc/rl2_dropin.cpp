@@ -715,4 +715,28 @@ void data_processing(bool re_fitting, unsigned int *bad_list, int nsources, doub
         report("data_processing");
 }
 
+// ---- test hooks: the shared scalar helpers the kernels use, evaluated on the host ----
+void rl2_test_sincos_rev64(int n, const double *t, double *s, double *c)
+{
+    for (int i = 0; i < n; i++) rl2::sincos_rev64(t[i], s[i], c[i]);
+}
+
+// Cell of each (u, v) by the kernels' reciprocal-product form and by the reference's
+// divisions; returns the number of disagreements.
+int rl2_test_cell_index(int n, const double *u, const double *v, double inv_lambda, double len,
+                        double inc, unsigned int facet, unsigned int *fast, unsigned int *exact)
+{
+    int bad = 0;
+    const double inv_inc = 1.0 / inc;
+    for (int i = 0; i < n; i++) {
+        uint32_t a = 0, b = 0;
+        const bool oka = rl2::cell_index_fast(u[i], v[i], inv_lambda, len, inc, inv_inc, facet, &a);
+        const bool okb = rl2::cell_index(u[i], v[i], inv_lambda, len, inc, facet, &b);
+        fast[i] = oka ? a : 0xffffffffu;
+        exact[i] = okb ? b : 0xffffffffu;
+        bad += fast[i] != exact[i];
+    }
+    return bad;
+}
+
 }  // extern "C"

@@ -32,6 +32,8 @@ struct LikItem {
     int slot;
     int prec;                  // 0: FP64 kernel, 1: mixed-precision kernel
     double e1, e2;
+    unsigned wg_first;         // tiled engine: first work-group of this item
+    unsigned pad;
 };
 
 struct RGrid {
@@ -139,12 +141,13 @@ struct TileTask {
     int active;                // 1 while another Newton pass is due
 };
 struct TileLaunch {
-    LikLaunch lk;              // items, facets, r grid, outputs and QAG workspace
-    const unsigned *wg_off;    // nitems + 1: first work-group of each item (nr * chunks)
+    LikLaunch lk;              // items (with their first work-group), facets, r grid,
+                               // outputs and QAG workspace
     TileTask *tasks;           // nitems * kMaxR
     double *partials;          // 8 doubles per work-group
     unsigned *status;          // [0] tasks still active, [1] largest Newton step count
-    unsigned nwg;
+    double *flops;             // FLOP-equivalents of the evaluation (sum over tasks)
+    unsigned nwg;              // work-groups of a pass (nr * chunks summed over items)
     unsigned chunk_cells;      // facet cells per work-group
 };
 void launch_tile_init(hipStream_t s, const TileLaunch &t);
@@ -153,7 +156,8 @@ void launch_tile_status(hipStream_t s, const TileLaunch &t);
 
 // Residual update for one fitted galaxy: R -= model(e) - model(0).
 void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,
-                     const double *v, const double *w, const double *wn, const double *spec,
+                     const double *v, const double *w, const double *wn,
+                     const double *inv_lambda, const double *spec,
                      const GalaxyGeom &fit, const GalaxyGeom &round, void *res);
 
 // Drop-in helpers on plain device arrays.

@@ -40,6 +40,17 @@ __device__ inline double model_shape(const GalaxyGeom &g, double u, double v, do
     return shape * (spectra * g.flux);
 }
 
+// The same model with (1 + s w |k|^2)^(-3/2) as rsqrt(den)^3 (one reciprocal square
+// root instead of a square root and a division; a few ulp from the reference's form).
+__device__ inline double model_shape_rsq(const GalaxyGeom &g, double u, double v, double wn2,
+                                         double spectra)
+{
+    const double k1 = fma(g.e2, v, (1. + g.e1) * u);
+    const double k2 = fma(1. - g.e1, v, g.e2 * u);
+    const double rr = rsqrt(fma(g.sf * wn2, fma(k1, k1, k2 * k2), 1.));
+    return rr * rr * rr * (spectra * g.flux);
+}
+
 // ------------------------------------------------------------------ sources / noise
 // R += sign * sum_g flux_g spec_c shape_g exp(i k_c (u l + v m + w n)).
 // One thread per row; galaxies staged through LDS in tiles of 128.
@@ -200,6 +211,7 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
         rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
     }
     const unsigned long ncell = (unsigned long)a.facet * a.facet;
+    const double inv_inc = 1.0 / a.inc;
     double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
            *acc_w = a.acc + 3 * ncell;
     __syncthreads();
@@ -213,15 +225,15 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
             const float s2 = a.s2f[idx];
             if (signbit(s2)) continue;  // flagged
             uint32_t key;
-            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key)) continue;
+            if (!cell_index_fast(ru[k], rv[k], inv_lambda, a.len, a.inc, inv_inc, a.facet, &key))
+                continue;
             double xr = 0., xi = 0.;
             if (MODE == 0) {
                 // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
-                // (source_extraction.c:51-65).
-                double phase = wn * rp[k];
-                double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
+                // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
+                double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
                 double sn, cs;
-                sincos(phase, &sn, &cs);
+                sincos_rev64(rp[k] * inv_lambda, sn, cs);
                 double2 r = load_res(a.res, PREC, idx);
                 double gre = shape * cs + r.x;
                 double gim = shape * sn + r.y;
@@ -313,6 +325,7 @@ __global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
         rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
     }
     const unsigned long ncell = (unsigned long)a.facet * a.facet;
+    const double inv_inc = 1.0 / a.inc;
     double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
            *acc_w = a.acc + 3 * ncell;
     __syncthreads();
@@ -332,16 +345,17 @@ __global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
             const size_t idx = (size_t)c * a.nrow + row0 + k;
             const float s2 = a.s2f[idx];
             if (signbit(s2)) continue;  // flagged
-            if (!cell_index(ru[k], rv[k], inv_lambda, a.len, a.inc, a.facet, &key[k])) continue;
+            if (!cell_index_fast(ru[k], rv[k], inv_lambda, a.len, a.inc, inv_inc, a.facet,
+                                 &key[k]))
+                continue;
             ok[k] = true;
             xs[k] = (double)s2;
             if (MODE == 0) {
                 // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
-                // (source_extraction.c:51-65).
-                const double phase = wn * rp[k];
-                const double shape = model_shape(a.g, ru[k], rv[k], wn2, spectra);
+                // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
+                const double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
                 double sn, cs;
-                sincos(phase, &sn, &cs);
+                sincos_rev64(rp[k] * inv_lambda, sn, cs);
                 const double2 r = load_res(a.res, PREC, idx);
                 const double gre = shape * cs + r.x;
                 const double gim = shape * sn + r.y;
@@ -754,10 +768,11 @@ __device__ inline void sincos_rev(double t, float &sn, float &cs)
     const float y = x - 0.25f * q;
     const float th = 6.28318530717958648f * y;
     const float z = th * th;
-    const float s = th * (1.f + z * (-1.f / 6 + z * (1.f / 120 + z * (-1.f / 5040 +
-                                                                  z * (1.f / 362880)))));
-    const float c = 1.f + z * (-0.5f + z * (1.f / 24 + z * (-1.f / 720 + z * (1.f / 40320 +
-                                                                          z * (-1.f / 3628800)))));
+    const float ps = fmaf(fmaf(fmaf(1.f / 362880, z, -1.f / 5040), z, 1.f / 120), z, -1.f / 6);
+    const float s = fmaf(th * z, ps, th);
+    const float pc = fmaf(fmaf(fmaf(fmaf(-1.f / 3628800, z, 1.f / 40320), z, -1.f / 720), z,
+                               1.f / 24), z, -0.5f);
+    const float c = fmaf(pc, z, 1.f);
     const int k = ((int)q) & 3;
     sn = (k == 0) ? s : (k == 1) ? c : (k == 2) ? -s : -c;
     cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
@@ -1434,19 +1449,28 @@ __global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
 }
 
 // ------------------------------------------------------------------ likelihood, tiled
-// Largest item whose first work-group is <= wg (wg_off is non-decreasing, every item has
-// at least one work-group).
-__device__ inline int tile_item(const unsigned *wg_off, int nitems, unsigned wg)
+// Largest item whose first work-group is <= wg (first work-groups are non-decreasing and
+// every item has at least one work-group).
+__device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg)
 {
     int lo = 0, hi = nitems;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (wg_off[mid] <= wg)
+        if (items[mid].wg_first <= wg)
             lo = mid;
         else
             hi = mid;
     }
     return lo;
+}
+
+// Work-groups [first, first + nr * chunks) of `item`.
+__device__ inline void tile_range(const TileLaunch &t, int item, unsigned *first, unsigned *nch)
+{
+    const int nr = t.lk.rg.numr - 1;
+    *first = t.lk.items[item].wg_first;
+    const unsigned end = (item + 1 < t.lk.nitems) ? t.lk.items[item + 1].wg_first : t.nwg;
+    *nch = (end - *first) / nr;
 }
 
 // Chunk sums in FP64.  PASS 0: {S2, value, dx, dy, dxx, dyy, dxy} of the unnormalised
@@ -1467,9 +1491,9 @@ __device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double 
 #pragma unroll 2
     for (unsigned i = beg + threadIdx.x; i < end; i += kTileBS) {
         const double uu = F.uu[i], vv = F.vv[i];
-        const double k1 = a1 * uu + e2 * vv;
-        const double k2 = e2 * uu + a2 * vv;
-        const double rr = rsqrt(1. + sw * (k1 * k1 + k2 * k2));
+        const double k1 = fma(e2, vv, a1 * uu);
+        const double k2 = fma(a2, vv, e2 * uu);
+        const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
         const double M = rr * rr * rr;
         const double hr = F.dre[i] * M;
         const double hi = -F.dim[i] * M;
@@ -1477,28 +1501,32 @@ __device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double 
         if (PASS == 0) {
             s[0] += M * M / F.s2[i];
             s[1] += hr;
-            s[2] -= hi * U;
-            s[3] -= hi * V;
-            s[4] -= hr * U * U;
-            s[5] -= hr * V * V;
-            s[6] -= hr * U * V;
+            s[2] = fma(-hi, U, s[2]);
+            s[3] = fma(-hi, V, s[3]);
+            const double hU = hr * U;
+            s[4] = fma(-hU, U, s[4]);
+            s[5] = fma(-hr * V, V, s[5]);
+            s[6] = fma(-hU, V, s[6]);
         } else {
+            // phase xo U + yo V = 2 pi (xo u + yo v): evaluated in revolutions
             double sn, cs;
-            sincos(xo * U + yo * V, &sn, &cs);
-            const double re = hr * cs - hi * sn;
-            const double im = hr * sn + hi * cs;
+            sincos_rev64(fma(xo, uu, yo * vv), sn, cs);
+            const double re = fma(hr, cs, -hi * sn);
+            const double im = fma(hr, sn, hi * cs);
             s[0] += re;
-            s[1] -= im * U;
-            s[2] -= im * V;
-            s[3] -= re * U * U;
-            s[4] -= re * V * V;
-            s[5] -= re * U * V;
+            s[1] = fma(-im, U, s[1]);
+            s[2] = fma(-im, V, s[2]);
+            const double rU = re * U;
+            s[3] = fma(-rU, U, s[3]);
+            s[4] = fma(-re * V, V, s[4]);
+            s[5] = fma(-rU, V, s[5]);
         }
     }
 }
 
-// The same sums from the FP32 facet copy: model and terms in FP32 (each thread adds at
-// most kTileCPT cells in FP32), the Newton phase formed and reduced in FP64 revolutions.
+// The same sums from the FP32 facet copy: model and terms in FP32, the per-thread FP32
+// partials added into FP64 every kFlush cells; the Newton phase formed and reduced in
+// FP64 revolutions.
 template <int PASS>
 __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double ro, double xo,
                                    double yo, unsigned beg, unsigned end, double (&s)[7])
@@ -1510,40 +1538,47 @@ __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double 
     const float sw = (float)(sf * (4 * kPi * kPi));
     const float a1 = (float)(1. + e1), a2 = (float)(1. - e1), fe2 = (float)e2;
     const float tp = 6.28318530717958648f;
-    float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-    for (unsigned i = beg + threadIdx.x; i < end; i += kTileBS) {
-        const float4 p = F.pk[i];
-        const float k1 = a1 * p.x + fe2 * p.y;
-        const float k2 = fe2 * p.x + a2 * p.y;
-        const float rr = rsqrtf(1.f + sw * (k1 * k1 + k2 * k2));
-        const float M = rr * rr * rr;
-        const float hr = p.z * M;
-        const float hi = -p.w * M;
-        const float U = tp * p.x, V = tp * p.y;
-        if (PASS == 0) {
-            f[0] += M * M * F.invs2[i];
-            f[1] += hr;
-            f[2] -= hi * U;
-            f[3] -= hi * V;
-            f[4] -= hr * U * U;
-            f[5] -= hr * V * V;
-            f[6] -= hr * U * V;
-        } else {
-            float sn, cs;
-            sincos_rev(xo * (double)p.x + yo * (double)p.y, sn, cs);
-            const float re = hr * cs - hi * sn;
-            const float im = hr * sn + hi * cs;
-            f[0] += re;
-            f[1] -= im * U;
-            f[2] -= im * V;
-            f[3] -= re * U * U;
-            f[4] -= re * V * V;
-            f[5] -= re * U * V;
-        }
-    }
 #pragma unroll
-    for (int j = 0; j < 7; j++) s[j] = (double)f[j];
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    for (unsigned base = beg + threadIdx.x; base < end; base += kFlush * kTileBS) {
+        float f[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const unsigned stop = min(end, base + kFlush * kTileBS);
+#pragma unroll 2
+        for (unsigned i = base; i < stop; i += kTileBS) {
+            const float4 p = F.pk[i];
+            const float k1 = fmaf(fe2, p.y, a1 * p.x);
+            const float k2 = fmaf(a2, p.y, fe2 * p.x);
+            const float rr = rsqrtf(fmaf(sw, fmaf(k1, k1, k2 * k2), 1.f));
+            const float M = rr * rr * rr;
+            const float hr = p.z * M;
+            const float hi = -p.w * M;
+            const float U = tp * p.x, V = tp * p.y;
+            if (PASS == 0) {
+                f[0] = fmaf(M * M, F.invs2[i], f[0]);
+                f[1] += hr;
+                f[2] = fmaf(-hi, U, f[2]);
+                f[3] = fmaf(-hi, V, f[3]);
+                const float hU = hr * U;
+                f[4] = fmaf(-hU, U, f[4]);
+                f[5] = fmaf(-hr * V, V, f[5]);
+                f[6] = fmaf(-hU, V, f[6]);
+            } else {
+                float sn, cs;
+                sincos_rev(fma(xo, (double)p.x, yo * (double)p.y), sn, cs);
+                const float re = fmaf(hr, cs, -hi * sn);
+                const float im = fmaf(hr, sn, hi * cs);
+                f[0] += re;
+                f[1] = fmaf(-im, U, f[1]);
+                f[2] = fmaf(-im, V, f[2]);
+                const float rU = re * U;
+                f[3] = fmaf(-rU, U, f[3]);
+                f[4] = fmaf(-re * V, V, f[4]);
+                f[5] = fmaf(-rU, V, f[5]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 7; j++) s[j] += (double)f[j];
+    }
 }
 
 // The Newton loop of likelihood.c:284-350 one step at a time, by the last work-group of
@@ -1596,10 +1631,9 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
     __shared__ double red[(kTileBS / 64) * 8];
     const LikLaunch &a = t.lk;
     const unsigned wg = blockIdx.x;
-    const int item = tile_item(t.wg_off, a.nitems, wg);
-    const int nr = a.rg.numr - 1;
-    const unsigned first = t.wg_off[item];
-    const unsigned nch = (t.wg_off[item + 1] - first) / nr;
+    const int item = tile_item(a.items, a.nitems, wg);
+    unsigned first, nch;
+    tile_range(t, item, &first, &nch);
     const unsigned local = wg - first;
     const int r = (int)(local / nch) + 1;
     const unsigned chunk = local - (unsigned)(r - 1) * nch;
@@ -1636,8 +1670,8 @@ __global__ __launch_bounds__(256) void lik_step_kernel(TileLaunch t)
     const int item = (int)(k / nr), r = (int)(k % nr) + 1;
     TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;
-    const unsigned first = t.wg_off[item];
-    const unsigned nch = (t.wg_off[item + 1] - first) / nr;
+    unsigned first, nch;
+    tile_range(t, item, &first, &nch);
     const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1) * nch) * 8;
     double v[7] = {0., 0., 0., 0., 0., 0., 0.};
     for (unsigned c = lane; c < nch; c += 64)
@@ -1659,30 +1693,39 @@ __global__ void tile_init_kernel(TileLaunch t)
     }
 }
 
-// status[0] = tasks still iterating, status[1] = largest Newton step count.
+// status[0] = tasks still iterating, status[1] = largest Newton step count,
+// *flops = sum over tasks of n (37 + 25 N_it) (model + normalisation 19, zero-offset sums
+// 18, 25 per Newton pass; transcendentals, sqrt and division counted as one).
 __global__ __launch_bounds__(256) void tile_status_kernel(TileLaunch t)
 {
     __shared__ unsigned cnt[4], mx[4];
+    __shared__ double fl[4];
     const int nr = t.lk.rg.numr - 1;
     const unsigned ntask = (unsigned)t.lk.nitems * nr;
     unsigned c = 0, m = 0;
+    double f = 0.;
     for (unsigned k = threadIdx.x; k < ntask; k += 256) {
-        const TileTask &T = t.tasks[(size_t)(k / nr) * kMaxR + k % nr + 1];
+        const int item = (int)(k / nr);
+        const TileTask &T = t.tasks[(size_t)item * kMaxR + k % nr + 1];
         c += T.active ? 1u : 0u;
         m = max(m, (unsigned)T.iter);
+        f += (double)t.lk.facets[t.lk.items[item].slot].n * (37. + 25. * T.iter);
     }
     for (int off = 32; off >= 1; off >>= 1) {
         c += __shfl_xor(c, off);
         m = max(m, (unsigned)__shfl_xor((int)m, off));
+        f += __shfl_xor(f, off);
     }
     if ((threadIdx.x & 63) == 0) {
         cnt[threadIdx.x >> 6] = c;
         mx[threadIdx.x >> 6] = m;
+        fl[threadIdx.x >> 6] = f;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         t.status[0] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
         t.status[1] = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
+        *t.flops = fl[0] + fl[1] + fl[2] + fl[3];
     }
 }
 
@@ -1743,6 +1786,7 @@ __global__ __launch_bounds__(256) void subtract_kernel(unsigned nch, unsigned lo
                                                        const double *__restrict__ v,
                                                        const double *__restrict__ w,
                                                        const double *__restrict__ wn,
+                                                       const double *__restrict__ inv_lambda,
                                                        const double *__restrict__ spec,
                                                        GalaxyGeom fit, GalaxyGeom rnd, void *res)
 {
@@ -1752,11 +1796,10 @@ __global__ __launch_bounds__(256) void subtract_kernel(unsigned nch, unsigned lo
     const double p = ui * fit.l0 + vi * fit.m0 + wi * fit.n0;
     for (unsigned c = 0; c < nch; c++) {
         const double k = wn[c], k2 = k * k, spectra = spec[c];
-        const double phase = k * p;
         double sn, cs;
-        sincos(phase, &sn, &cs);
-        const double sr = model_shape(rnd, ui, vi, k2, spectra);
-        const double sfit = model_shape(fit, ui, vi, k2, spectra);
+        sincos_rev64(p * inv_lambda[c], sn, cs);  // phase k p in revolutions
+        const double sr = model_shape_rsq(rnd, ui, vi, k2, spectra);
+        const double sfit = model_shape_rsq(fit, ui, vi, k2, spectra);
         const size_t idx = (size_t)c * nrow + i;
         double2 r = load_res(res, PREC, idx);
         r.x = (r.x - sfit * cs) + sr * cs;
@@ -1766,14 +1809,17 @@ __global__ __launch_bounds__(256) void subtract_kernel(unsigned nch, unsigned lo
 }
 
 void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,
-                     const double *v, const double *w, const double *wn, const double *spec,
+                     const double *v, const double *w, const double *wn,
+                     const double *inv_lambda, const double *spec,
                      const GalaxyGeom &fit, const GalaxyGeom &round, void *res)
 {
     dim3 grid((unsigned)((nrow + 255) / 256));
     if (prec == 0)
-        subtract_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, fit, round, res);
+        subtract_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, inv_lambda, spec, fit,
+                                                 round, res);
     else
-        subtract_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, spec, fit, round, res);
+        subtract_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, inv_lambda, spec, fit,
+                                                 round, res);
 }
 
 // ------------------------------------------------------------------ drop-in helpers

@@ -74,6 +74,62 @@ RL2_HD bool cell_index(double u, double v, double inv_lambda, double len, double
     return pc < (unsigned long long)facet * facet;
 }
 
+// sin(2 pi t), cos(2 pi t) in FP64.  t - rint(t) and the octant split y = r - q/4
+// (q = rint(4 r)) are exact, |y| <= 1/8; theta = 2 pi y; Taylor series of sin and cos to
+// degree 15 / 18 in Horner form with fused multiply-adds (truncation below 5e-17 and
+// 3e-18 on |theta| <= pi/4).  The phase arrives in revolutions, so no multiple of 2 pi is
+// ever subtracted.
+RL2_HD void sincos_rev64(double t, double &sn, double &cs)
+{
+    const double r = t - rint(t);
+    const double q = rint(4. * r);
+    const double y = r - 0.25 * q;
+    const double th = 6.283185307179586477 * y;
+    const double z = th * th;
+    double ps = -1. / 1307674368000.;
+    ps = fma(ps, z, 1. / 6227020800.);
+    ps = fma(ps, z, -1. / 39916800.);
+    ps = fma(ps, z, 1. / 362880.);
+    ps = fma(ps, z, -1. / 5040.);
+    ps = fma(ps, z, 1. / 120.);
+    ps = fma(ps, z, -1. / 6.);
+    const double s = fma(th * z, ps, th);
+    double pc = -1. / 6402373705728000.;
+    pc = fma(pc, z, 1. / 20922789888000.);
+    pc = fma(pc, z, -1. / 87178291200.);
+    pc = fma(pc, z, 1. / 479001600.);
+    pc = fma(pc, z, -1. / 3628800.);
+    pc = fma(pc, z, 1. / 40320.);
+    pc = fma(pc, z, -1. / 720.);
+    pc = fma(pc, z, 1. / 24.);
+    pc = fma(pc, z, -0.5);
+    const double c = fma(pc, z, 1.);
+    const int k = ((int)q) & 3;
+    sn = (k == 0) ? s : (k == 1) ? c : (k == 2) ? -s : -c;
+    cs = (k == 0) ? c : (k == 1) ? -s : (k == 2) ? -c : s;
+}
+
+// cell_index (evaluate_uv_grid.c:166-178) with the two divisions by the cell size replaced
+// by products with its reciprocal.  The product is within a few ulp of the correctly
+// rounded quotient, so their integer parts can only differ when the quotient lies within
+// that distance of an integer; there (|q - rint(q)| < 1e-9, far above the few-ulp error
+// for quotients below 2^21) the exact division is taken, so the cell is always the
+// reference's.
+RL2_HD bool cell_index_fast(double u, double v, double inv_lambda, double len, double inc,
+                            double inv_inc, unsigned int facet, uint32_t *pc_out)
+{
+    const double a = u * inv_lambda + len;
+    const double b = v * inv_lambda + len;
+    double qa = a * inv_inc, qb = b * inv_inc;
+    if (fabs(qa - rint(qa)) < 1e-9) qa = a / inc;
+    if (fabs(qb - rint(qb)) < 1e-9) qb = b / inc;
+    const unsigned int pu = (unsigned int)qa;
+    const unsigned int pv = (unsigned int)qb;
+    const unsigned long long pc = (unsigned long long)pv * facet + pu;
+    *pc_out = (uint32_t)pc;
+    return pc < (unsigned long long)facet * facet;
+}
+
 // Exponential integral Ei(x), x > 0: series gamma + ln x + sum x^k/(k k!) for x <= 40,
 // asymptotic e^x/x sum k!/x^k above (replaces gsl_sf_expint_Ei_e, likelihood.c:409).
 RL2_HD double expint_ei(double x)

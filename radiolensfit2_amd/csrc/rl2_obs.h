@@ -32,19 +32,22 @@ struct LikBuffers {
     LikItem *d_items = nullptr, *h_items = nullptr;
     double *d_Lr = nullptr, *h_Lr = nullptr;
     int *d_nit = nullptr, *h_nit = nullptr;
+    // results block, copied back in one piece: status[2] u32 | FLOPs f64 | L_e[cap] | err[cap]
+    char *d_out = nullptr, *h_out = nullptr;
+    unsigned *d_status = nullptr, *h_status = nullptr;   // tasks still active, max steps
+    double *d_flops = nullptr, *h_flops = nullptr;
     double *d_Le = nullptr, *h_Le = nullptr;
     int *d_err = nullptr, *h_err = nullptr;
     double *d_ws = nullptr;
     int *d_iws = nullptr;
     // tiled engine
-    unsigned *d_wg_off = nullptr, *h_wg_off = nullptr;  // cap + 1
     TileTask *d_tasks = nullptr;                         // cap * kMaxR
-    unsigned *d_status = nullptr, *h_status = nullptr;   // 2
     double *d_partials = nullptr;
     size_t partials_cap = 0;                             // work-groups
     int tile_passes = 6;     // Newton passes queued before the first status check
     bool ensure(int n);
     bool ensure_partials(size_t nwg);
+    size_t out_bytes(int n) const;
     void release();
 };
 

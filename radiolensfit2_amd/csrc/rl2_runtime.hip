@@ -133,50 +133,51 @@ bool LikBuffers::ensure(int n)
     release();
     int c = 256;
     while (c < n) c *= 2;
+    const size_t ob = 16 + 12 * (size_t)c;  // status[2] u32, FLOPs f64, L_e[c] f64, err[c] i32
     bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
-              dmalloc(&d_nit, (size_t)c * kMaxR) && dmalloc(&d_Le, c) && dmalloc(&d_err, c) &&
+              dmalloc(&d_nit, (size_t)c * kMaxR) && dmalloc(&d_out, ob) &&
               dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts) &&
-              dmalloc(&d_wg_off, c + 1) && dmalloc(&d_tasks, (size_t)c * kMaxR) &&
-              dmalloc(&d_status, 2);
+              dmalloc(&d_tasks, (size_t)c * kMaxR);
     ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * sizeof(TileTask), g_stream),
                       "memset");
-    ok = ok && hip_ok(hipHostMalloc((void **)&h_wg_off, (c + 1) * sizeof(unsigned)),
-                      "hipHostMalloc") &&
-         hip_ok(hipHostMalloc((void **)&h_status, 2 * sizeof(unsigned)), "hipHostMalloc");
-    ok = ok && hip_ok(hipHostMalloc((void **)&h_items, c * sizeof(LikItem)), "hipHostMalloc") &&
+    ok = ok && hip_ok(hipHostMalloc((void **)&h_out, ob), "hipHostMalloc") &&
+         hip_ok(hipHostMalloc((void **)&h_items, c * sizeof(LikItem)), "hipHostMalloc") &&
          hip_ok(hipHostMalloc((void **)&h_Lr, (size_t)c * kMaxR * sizeof(double)), "hipHostMalloc") &&
-         hip_ok(hipHostMalloc((void **)&h_nit, (size_t)c * kMaxR * sizeof(int)), "hipHostMalloc") &&
-         hip_ok(hipHostMalloc((void **)&h_Le, c * sizeof(double)), "hipHostMalloc") &&
-         hip_ok(hipHostMalloc((void **)&h_err, c * sizeof(int)), "hipHostMalloc");
-    if (ok) cap = c;
-    return ok;
+         hip_ok(hipHostMalloc((void **)&h_nit, (size_t)c * kMaxR * sizeof(int)), "hipHostMalloc");
+    if (!ok) return false;
+    cap = c;
+    d_status = (unsigned *)d_out;
+    d_flops = (double *)(d_out + 8);
+    d_Le = (double *)(d_out + 16);
+    d_err = (int *)(d_out + 16 + 8 * (size_t)c);
+    h_status = (unsigned *)h_out;
+    h_flops = (double *)(h_out + 8);
+    h_Le = (double *)(h_out + 16);
+    h_err = (int *)(h_out + 16 + 8 * (size_t)c);
+    return true;
 }
+size_t LikBuffers::out_bytes(int n) const { return 16 + 8 * (size_t)cap + 4 * (size_t)n; }
 void LikBuffers::release()
 {
     dfree(d_items);
     dfree(d_Lr);
     dfree(d_nit);
-    dfree(d_Le);
-    dfree(d_err);
+    dfree(d_out);
     dfree(d_ws);
     dfree(d_iws);
-    dfree(d_wg_off);
     dfree(d_tasks);
-    dfree(d_status);
-    if (h_wg_off) hipHostFree(h_wg_off);
-    if (h_status) hipHostFree(h_status);
-    h_wg_off = nullptr;
-    h_status = nullptr;
+    if (h_out) hipHostFree(h_out);
     if (h_items) hipHostFree(h_items);
     if (h_Lr) hipHostFree(h_Lr);
     if (h_nit) hipHostFree(h_nit);
-    if (h_Le) hipHostFree(h_Le);
-    if (h_err) hipHostFree(h_err);
+    h_out = nullptr;
     h_items = nullptr;
     h_Lr = nullptr;
     h_nit = nullptr;
-    h_Le = nullptr;
-    h_err = nullptr;
+    d_status = h_status = nullptr;
+    d_flops = h_flops = nullptr;
+    d_Le = h_Le = nullptr;
+    d_err = h_err = nullptr;
     cap = 0;
 }
 bool LikBuffers::ensure_partials(size_t nwg)
@@ -357,7 +358,8 @@ bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double
     GalaxyGeom rnd = make_geom(l0, m0, flux, R_mu, 0., 0.);
     {
         KTimer t(RL2_K_SUBTRACT);
-        launch_subtract(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn, o.d_spec, fit,
+        launch_subtract(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn, o.d_invlam,
+                        o.d_spec, fit,
                         rnd, o.d_res);
     }
     stats_add_work(RL2_K_SUBTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_SUBTRACT) + o.nrow * 24.);
@@ -372,7 +374,6 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     if (!o.lik.ensure(n)) return false;
     LikBuffers &b = o.lik;
     memcpy(b.h_items, items.data(), n * sizeof(LikItem));
-    hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice, g_stream);
     LikLaunch a;
     a.nitems = n;
     a.items = b.d_items;
@@ -392,10 +393,10 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     bool tiled = o.lik_block == 0;
     for (int i = 0; i < n && tiled; i++)
         if (o.h_facets[items[i].slot].mod) tiled = false;  // given model: one group per (e, r)
-    const bool want_detail = Lr || nit || g_stats_on;
+    const bool want_detail = Lr || nit || (g_stats_on && !tiled);
+    // one copy back per evaluation: status, FLOP count, L_e and the error flags
     auto copy_out = [&]() {
-        hipMemcpyAsync(b.h_Le, b.d_Le, n * sizeof(double), hipMemcpyDeviceToHost, g_stream);
-        hipMemcpyAsync(b.h_err, b.d_err, n * sizeof(int), hipMemcpyDeviceToHost, g_stream);
+        hipMemcpyAsync(b.h_out, b.d_out, b.out_bytes(n), hipMemcpyDeviceToHost, g_stream);
         if (want_detail) {
             hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double),
                            hipMemcpyDeviceToHost, g_stream);
@@ -419,22 +420,21 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             while (chunk < kTileMaxCells && groups(2 * chunk) >= kTileTargetGroups) chunk *= 2;
         }
         unsigned total = 0;
-        b.h_wg_off[0] = 0;
         for (int i = 0; i < n; i++) {
             const unsigned cells = o.slots[items[i].slot].n;
             const unsigned nchunk = std::max(1u, (cells + chunk - 1) / chunk);
+            b.h_items[i].wg_first = total;
             total += nchunk * (unsigned)nr;
-            b.h_wg_off[i + 1] = total;
         }
         if (!b.ensure_partials(total)) return false;
-        hipMemcpyAsync(b.d_wg_off, b.h_wg_off, (n + 1) * sizeof(unsigned),
-                       hipMemcpyHostToDevice, g_stream);
+        hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
+                       g_stream);
         TileLaunch t;
         t.lk = a;
-        t.wg_off = b.d_wg_off;
         t.tasks = b.d_tasks;
         t.partials = b.d_partials;
         t.status = b.d_status;
+        t.flops = b.d_flops;
         t.nwg = total;
         t.chunk_cells = chunk;
         launch_tile_init(g_stream, t);
@@ -457,8 +457,6 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 launch_rmarg(g_stream, a);
             }
             launch_tile_status(g_stream, t);
-            hipMemcpyAsync(b.h_status, b.d_status, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
-                           g_stream);
             copy_out();
             if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
             if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
@@ -470,7 +468,11 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             queued = 2;
         }
         b.tile_passes = std::min(12, std::max(2, (int)b.h_status[1] + 1));
+        // FLOP-equivalents summed on the device: n * (19 + 18 + 25 * N_it) per (item, r)
+        if (g_stats_on) stats_add_work(RL2_K_LIKELIHOOD, *b.h_flops, 0.);
     } else {
+        hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
+                       g_stream);
         {
             KTimer t(RL2_K_LIKELIHOOD);
             launch_likelihood(g_stream, a);
@@ -482,20 +484,20 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
         copy_out();
         if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+        if (g_stats_on) {
+            double fl = 0.;
+            for (int i = 0; i < n; i++) {
+                const double nn = o.slots[items[i].slot].n;
+                for (int r = 1; r < o.rg.numr; r++)
+                    fl += nn * (37. + 25. * b.h_nit[(size_t)i * kMaxR + r]);
+            }
+            stats_add_work(RL2_K_LIKELIHOOD, fl, 0.);
+        }
     }
     memcpy(Le, b.h_Le, n * sizeof(double));
     memcpy(err, b.h_err, n * sizeof(int));
     if (Lr) memcpy(Lr, b.h_Lr, (size_t)n * kMaxR * sizeof(double));
     if (nit) memcpy(nit, b.h_nit, (size_t)n * kMaxR * sizeof(int));
-    if (g_stats_on) {
-        // FLOP-equivalents per (item, r): n * (19 + 18 + 25 * N_it)
-        double fl = 0.;
-        for (int i = 0; i < n; i++) {
-            const double nn = o.slots[items[i].slot].n;
-            for (int r = 1; r < o.rg.numr; r++) fl += nn * (37. + 25. * b.h_nit[(size_t)i * kMaxR + r]);
-        }
-        stats_add_work(RL2_K_LIKELIHOOD, fl, 0.);
-    }
     return true;
 }
 

@@ -171,3 +171,42 @@ def test_rlms_roundtrip(tmp_path):
     want = (data[..., 0].astype(np.float64) * 1e6 + 1j * data[..., 1].astype(np.float64) * 1e6)
     np.testing.assert_array_equal(vis.reshape(nch, nrow).T, want)
     assert (s2[:nrow] == np.float32(6.25)).all() and (s2[nrow:] == 0).all()
+
+
+def test_kernel_sincos_in_revolutions_matches_mpmath():
+    """sin/cos(2 pi t) as the FP64 kernels evaluate them, against 50-digit mpmath, for
+    phases up to 1e4 revolutions; the error budget is a few ulp of 1."""
+    import mpmath
+    H = N.hip()
+    rng = np.random.default_rng(21)
+    t = np.concatenate([rng.uniform(-1e4, 1e4, 300), rng.uniform(-1, 1, 300),
+                        np.array([0.0, 0.125, 0.25, 0.375, 0.5, -0.125, 1e-9, 123456.875])])
+    s, c = np.zeros_like(t), np.zeros_like(t)
+    H.rl2_test_sincos_rev64(len(t), N.dptr(t), N.dptr(s), N.dptr(c))
+    mpmath.mp.dps = 50
+    es = max(abs(float(mpmath.sin(2 * mpmath.pi * mpmath.mpf(float(x)))) - y) for x, y in zip(t, s))
+    ec = max(abs(float(mpmath.cos(2 * mpmath.pi * mpmath.mpf(float(x)))) - y) for x, y in zip(t, c))
+    assert es < 1e-15 and ec < 1e-15, (es, ec)
+
+
+def test_kernel_cell_index_equals_reference_division():
+    """The reciprocal-product cell index equals the reference's division everywhere,
+    including points exactly on cell edges and one ulp either side of them."""
+    H = N.hip()
+    facet, len_l, inv_lambda = 874, 9.7e5, 3.33
+    inc = 2 * len_l / facet
+    rng = np.random.default_rng(22)
+    u = rng.uniform(-len_l, len_l, 20000) / inv_lambda
+    v = rng.uniform(-len_l, len_l, 20000) / inv_lambda
+    # edges: (u inv_lambda + len) = k inc, and its neighbours
+    k = rng.integers(0, facet + 1, 3000)
+    edge = (k * inc - len_l) / inv_lambda
+    ue = np.concatenate([edge, np.nextafter(edge, np.inf), np.nextafter(edge, -np.inf)])
+    u = np.concatenate([u, ue])
+    v = np.concatenate([v, ue[::-1]])
+    fast = np.zeros(len(u), np.uint32)
+    exact = np.zeros(len(u), np.uint32)
+    bad = H.rl2_test_cell_index(len(u), N.dptr(u), N.dptr(v), inv_lambda, len_l, inc, facet,
+                                fast.ctypes.data_as(N.c_uint_p), exact.ctypes.data_as(N.c_uint_p))
+    assert bad == 0
+    assert (fast == exact).all()
