@@ -72,7 +72,10 @@ typedef struct {
                               of that many threads per (e, r) task */
     int verbose;
     int lik_precision;     /* 0: FP64 likelihood throughout; 1: mixed-precision likelihood
-                              throughout; 2: mixed for the simplex, FP64 for the sampling */
+                              throughout; 2: mixed for the simplex, FP64 for the sampling;
+                              3: FP64 simplex, mixed sampling; 4: as 3, with samples whose
+                              threshold test lies within the FP32 error bound re-evaluated
+                              in FP64 */
 } rl2_fit_options;
 
 const char *rl2_last_error(void);
@@ -158,8 +161,9 @@ int rl2_stats_get(int kernel, rl2_kernel_stat *out);
 int rl2_synchronize(void);
 
 /* ---- launch-shape knobs (set by an autotuner; process-wide) ----
- * RL2_TUNE_GRID_VARIANT: extraction gridding, 0 per-visibility LDS hash (default),
- *   1 wave-aggregated cells.
+ * RL2_TUNE_GRID_VARIANT: extraction gridding, 0 per-visibility LDS hash,
+ *   1 wave-aggregated cells, 2 / 3 / 4 runs of 8 / 4 / 4 consecutive rows merged per
+ *   thread (LDS tables of 2048 / 1024 / 2048 slots); default 3.
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
  *   in [2048, 65536], or 0 (default) to choose per launch. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1 };

@@ -83,8 +83,8 @@ void launch_grid(hipStream_t s, const GridLaunch &a);
 
 // Launch-shape knobs (rl2_tune_set): read by the launchers on the host.
 struct Tuning {
-    int grid_variant = 0;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
-                               // 1 wave-aggregated cells
+    int grid_variant = 3;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
+                               // 1 wave-aggregated cells, 2 / 3 per-thread runs of 8 / 4
     unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
 };
 Tuning &tuning();
@@ -100,6 +100,16 @@ size_t compact_scratch_words(unsigned facet);
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
                        const double *vv, const double *dre, const double *dim, const double *s2,
                        float4 *pk, float *invs2);
+
+struct TileTask {
+    double xo, yo;             // position offset of the current Newton iterate
+    double norm;               // |increment| of the last step (1 before the first)
+    double S2;                 // sum M^2 / sigma^2 of the unnormalised model
+    int iter;                  // Newton steps taken
+    int active;                // 1 while another Newton pass is due
+    int fragile;               // the finished L_r sits at a branch that FP32 sums can flip
+    int pad;
+};
 
 // Likelihood: per (item, r) work-group L_r and Newton iterations; then r-marginalisation.
 struct LikLaunch {
@@ -118,7 +128,10 @@ struct LikLaunch {
     double *ho = nullptr;      // optional nitems * kMaxR: cross-correlation peak value
     double *det = nullptr;     // optional nitems * kMaxR: Hessian determinant at the peak
     const int *host_prec = nullptr;  // host copy of items[].prec (selects the kernels)
+    const TileTask *tasks = nullptr; // tiled engine: per-(item, r) state (fragile flags)
 };
+// err bit set for a mixed-precision item one of whose L_r is fragile
+constexpr int kErrFragile = 0x100;
 constexpr int kQagLimit = 1000;
 constexpr int kQagDoubles = 4 * kQagLimit;
 constexpr int kQagInts = 2 * kQagLimit;
@@ -133,13 +146,7 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
-struct TileTask {
-    double xo, yo;             // position offset of the current Newton iterate
-    double norm;               // |increment| of the last step (1 before the first)
-    double S2;                 // sum M^2 / sigma^2 of the unnormalised model
-    int iter;                  // Newton steps taken
-    int active;                // 1 while another Newton pass is due
-};
+
 struct TileLaunch {
     LikLaunch lk;              // items (with their first work-group), facets, r grid,
                                // outputs and QAG workspace

@@ -176,10 +176,12 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 // Variant 0.  One work-group owns RPT*256 consecutive (sorted) rows, thread t rows
 // t, t+256, ...  Per channel each thread adds its RPT visibilities into an LDS
 // open-addressing table keyed by cell index (at most RPT*256 distinct keys, table size
-// 2*RPT*256, so probing always terminates), then the table is flushed to the global
-// accumulators with one atomic per distinct cell.  Sorted rows make neighbouring threads
-// hit the same cells, so hot cells near the uv origin receive one global atomic per
-// work-group instead of one per visibility.
+// 2*RPT*256, so probing always terminates); the occupied slots (listed as they are
+// claimed) are then flushed to the global accumulators with one atomic per cell and
+// component.  Sorted rows make neighbouring threads hit the same cells, so hot cells near
+// the uv origin receive one global atomic per work-group instead of one per visibility.
+// The next channel's sigma^2 and residual are loaded before the current channel is
+// gridded, so their latency overlaps the arithmetic and the LDS traffic.
 template <int MODE, int PREC, int RPT>
 __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
 {
@@ -190,6 +192,8 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
     __shared__ uint32_t hkey[HS];
     __shared__ double hre[HS], him[HS], hs2[HS];
     __shared__ uint32_t hcnt[HS];
+    __shared__ uint32_t hlist[HS];
+    __shared__ uint32_t hn;
     const int tid = threadIdx.x;
     for (int s = tid; s < HS; s += BS) {
         hkey[s] = kEmpty;
@@ -198,6 +202,7 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
         hs2[s] = 0.;
         hcnt[s] = 0;
     }
+    if (tid == 0) hn = 0;
     const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS);
     double ru[RPT], rv[RPT], rp[RPT];
     bool rok[RPT];
@@ -214,16 +219,39 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
     const double inv_inc = 1.0 / a.inc;
     double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
            *acc_w = a.acc + 3 * ncell;
+    // channel-0 inputs
+    float ns2[RPT];
+    double2 nres[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const size_t idx = row0 + tid + (size_t)k * BS;
+        ns2[k] = rok[k] ? a.s2f[idx] : -1.f;
+        nres[k] = (rok[k] && MODE != 2) ? load_res(a.res, PREC, idx) : make_double2(0., 0.);
+    }
     __syncthreads();
     for (unsigned c = 0; c < a.nch; c++) {
         const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
         const double wn2 = wn * wn, spectra = a.spec[c];
+        float cs2[RPT];
+        double2 cres[RPT];
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
-            if (!rok[k]) continue;
-            const size_t idx = (size_t)c * a.nrow + row0 + tid + (size_t)k * BS;
-            const float s2 = a.s2f[idx];
-            if (signbit(s2)) continue;  // flagged
+            cs2[k] = ns2[k];
+            cres[k] = nres[k];
+        }
+        if (c + 1 < a.nch) {
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                const size_t idx = (size_t)(c + 1) * a.nrow + row0 + tid + (size_t)k * BS;
+                ns2[k] = rok[k] ? a.s2f[idx] : -1.f;
+                nres[k] = (rok[k] && MODE != 2) ? load_res(a.res, PREC, idx)
+                                                : make_double2(0., 0.);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const float s2 = cs2[k];
+            if (!rok[k] || signbit(s2)) continue;  // outside the observation, or flagged
             uint32_t key;
             if (!cell_index_fast(ru[k], rv[k], inv_lambda, a.len, a.inc, inv_inc, a.facet, &key))
                 continue;
@@ -231,22 +259,22 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
             if (MODE == 0) {
                 // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
                 // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
-                double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
+                const double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
                 double sn, cs;
                 sincos_rev64(rp[k] * inv_lambda, sn, cs);
-                double2 r = load_res(a.res, PREC, idx);
-                double gre = shape * cs + r.x;
-                double gim = shape * sn + r.y;
+                const double2 r = cres[k];
+                const double gre = shape * cs + r.x;
+                const double gim = shape * sn + r.y;
                 xr = gre * cs - gim * (-sn);
                 xi = gre * (-sn) + gim * cs;
             } else if (MODE == 1) {
-                double2 r = load_res(a.res, PREC, idx);
-                xr = r.x;
-                xi = r.y;
+                xr = cres[k].x;
+                xi = cres[k].y;
             }
             uint32_t h = (key * 2654435761u) >> (32 - HBITS);
             while (true) {
-                uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
                 if (prev == kEmpty || prev == key) break;
                 h = (h + 1) & (HS - 1);
             }
@@ -258,22 +286,143 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
             atomicAdd(&hcnt[h], 1u);
         }
         __syncthreads();
-        for (int s = tid; s < HS; s += BS) {
-            uint32_t key = hkey[s];
-            if (key != kEmpty) {
-                if (MODE != 2) {
-                    unsafeAtomicAdd(&acc_re[key], hre[s]);
-                    unsafeAtomicAdd(&acc_im[key], him[s]);
-                }
-                unsafeAtomicAdd(&acc_s2[key], hs2[s]);
-                unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
-                hkey[s] = kEmpty;
-                hre[s] = 0.;
-                him[s] = 0.;
-                hs2[s] = 0.;
-                hcnt[s] = 0;
+        const uint32_t nocc = hn;
+        for (uint32_t j = tid; j < nocc; j += BS) {
+            const uint32_t s = hlist[j];
+            const uint32_t key = hkey[s];
+            if (MODE != 2) {
+                unsafeAtomicAdd(&acc_re[key], hre[s]);
+                unsafeAtomicAdd(&acc_im[key], him[s]);
+            }
+            unsafeAtomicAdd(&acc_s2[key], hs2[s]);
+            unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+            hkey[s] = kEmpty;
+            hre[s] = 0.;
+            him[s] = 0.;
+            hs2[s] = 0.;
+            hcnt[s] = 0;
+        }
+        __syncthreads();
+        if (tid == 0) hn = 0;
+        __syncthreads();
+    }
+}
+
+// Variants 2-4.  Each thread owns RPT consecutive sorted rows and, per channel, adds runs
+// of consecutive visibilities that fall in the same cell in registers before one LDS
+// update per run: in the dense uv core, where many rows share a cell, this removes most of
+// the same-address LDS atomics that serialise across a wave.  A channel inserts at most
+// RPT*256 runs into the HS-slot table (HS >= RPT*256, so probing always finds a slot);
+// flush as variant 0.
+template <int MODE, int PREC, int RPT, int HS>
+__global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
+{
+    constexpr int BS = 256;
+    static_assert(HS >= RPT * BS, "a free slot must exist for every run of a channel");
+    constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
+    static_assert((1 << HBITS) == HS, "hash size must be a power of two");
+    __shared__ uint32_t hkey[HS];
+    __shared__ double hre[HS], him[HS], hs2[HS];
+    __shared__ uint32_t hcnt[HS];
+    __shared__ uint32_t hlist[HS];
+    __shared__ uint32_t hn;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < HS; s += BS) {
+        hkey[s] = kEmpty;
+        hre[s] = 0.;
+        him[s] = 0.;
+        hs2[s] = 0.;
+        hcnt[s] = 0;
+    }
+    if (tid == 0) hn = 0;
+    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS) + (unsigned long)tid * RPT;
+    const unsigned long ncell = (unsigned long)a.facet * a.facet;
+    const double inv_inc = 1.0 / a.inc;
+    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
+           *acc_w = a.acc + 3 * ncell;
+    auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {
+        uint32_t h = (key * 2654435761u) >> (32 - HBITS);
+        while (true) {
+            const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+            if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
+            if (prev == kEmpty || prev == key) break;
+            h = (h + 1) & (HS - 1);
+        }
+        if (MODE != 2) {
+            atomicAdd(&hre[h], xr);
+            atomicAdd(&him[h], xi);
+        }
+        atomicAdd(&hs2[h], xs);
+        atomicAdd(&hcnt[h], n);
+    };
+    __syncthreads();
+    for (unsigned c = 0; c < a.nch; c++) {
+        const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
+        const double wn2 = wn * wn, spectra = a.spec[c];
+        uint32_t rkey = kEmpty, rn = 0;
+        double rre = 0., rim = 0., rs2 = 0.;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const unsigned long i = row0 + k;
+            if (i >= a.nrow) continue;
+            const size_t idx = (size_t)c * a.nrow + i;
+            const float s2 = a.s2f[idx];
+            if (signbit(s2)) continue;  // flagged
+            const double u = a.u[i], v = a.v[i];
+            uint32_t key;
+            if (!cell_index_fast(u, v, inv_lambda, a.len, a.inc, inv_inc, a.facet, &key)) continue;
+            double xr = 0., xi = 0.;
+            if (MODE == 0) {
+                // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
+                // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
+                const double p = u * a.g.l0 + v * a.g.m0 + a.w[i] * a.g.n0;
+                const double shape = model_shape_rsq(a.g, u, v, wn2, spectra);
+                double sn, cs;
+                sincos_rev64(p * inv_lambda, sn, cs);
+                const double2 r = load_res(a.res, PREC, idx);
+                const double gre = shape * cs + r.x;
+                const double gim = shape * sn + r.y;
+                xr = gre * cs - gim * (-sn);
+                xi = gre * (-sn) + gim * cs;
+            } else if (MODE == 1) {
+                const double2 r = load_res(a.res, PREC, idx);
+                xr = r.x;
+                xi = r.y;
+            }
+            if (key == rkey) {
+                rre += xr;
+                rim += xi;
+                rs2 += (double)s2;
+                rn++;
+            } else {
+                if (rn) emit(rkey, rre, rim, rs2, rn);
+                rkey = key;
+                rre = xr;
+                rim = xi;
+                rs2 = (double)s2;
+                rn = 1;
             }
         }
+        if (rn) emit(rkey, rre, rim, rs2, rn);
+        __syncthreads();
+        const uint32_t nocc = hn;
+        for (uint32_t j = tid; j < nocc; j += BS) {
+            const uint32_t s = hlist[j];
+            const uint32_t key = hkey[s];
+            if (MODE != 2) {
+                unsafeAtomicAdd(&acc_re[key], hre[s]);
+                unsafeAtomicAdd(&acc_im[key], him[s]);
+            }
+            unsafeAtomicAdd(&acc_s2[key], hs2[s]);
+            unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+            hkey[s] = kEmpty;
+            hre[s] = 0.;
+            him[s] = 0.;
+            hs2[s] = 0.;
+            hcnt[s] = 0;
+        }
+        __syncthreads();
+        if (tid == 0) hn = 0;
         __syncthreads();
     }
 }
@@ -447,12 +596,29 @@ __global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
 template <int MODE, int PREC>
 static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
 {
-    constexpr int RPT = 4;
-    dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
-    if (tuning().grid_variant == 1)
+    if (tuning().grid_variant == 1) {
+        constexpr int RPT = 4;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
         grid_wave_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
-    else
+    } else if (tuning().grid_variant == 2) {
+        constexpr int RPT = 8;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        grid_run_kernel<MODE, PREC, RPT, 2048><<<grid, 256, 0, s>>>(a);
+    } else if (tuning().grid_variant == 3) {
+        constexpr int RPT = 4;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        grid_run_kernel<MODE, PREC, RPT, 1024><<<grid, 256, 0, s>>>(a);
+    } else if (tuning().grid_variant == 4) {
+        constexpr int RPT = 4;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        grid_run_kernel<MODE, PREC, RPT, 2048><<<grid, 256, 0, s>>>(a);
+    } else {
+        // 512 rows per work-group: a 1024-slot table (36 KB of LDS) leaves room for four
+        // work-groups per CU
+        constexpr int RPT = 2;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
         grid_direct_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
+    }
 }
 
 void launch_grid(hipStream_t s, const GridLaunch &a)
@@ -1434,6 +1600,9 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs
         L_e = kBad;
     }
     if (L_e == kBad) error = 1;
+    if (a.tasks && it.prec == 1)
+        for (int r = 1; r < numR; r++)
+            if (a.tasks[(size_t)item * kMaxR + r].fragile) error |= kErrFragile;
     if (threadIdx.x == 0) {
         a.Le[item] = L_e;
         a.err[item] = error;
@@ -1621,7 +1790,14 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
     }
     T->active = 0;
     const double S2 = T->S2;
-    finish_lr(a, item, r, value / sqrt(S2), det / S2, T->iter);
+    // branches an FP32 sum could decide differently: the determinant sign, the Newton
+    // step cap, the position-marginalisation cut x <= Delta (likelihood.c:395)
+    const double ho = value / sqrt(S2);
+    const double scale = fabs(dxx * dyy) + dxy * dxy;
+    const double B = ho * ho * 0.5, Delta = 0.5 * 5.991;
+    T->fragile = (fabs(det) <= 1e-5 * scale) || T->iter >= 50 ||
+                 fabs(B - Delta) <= 1e-4 * fabs(B) || !isfinite(det);
+    finish_lr(a, item, r, ho, det / S2, T->iter);
     return 1;
 }
 
@@ -1689,6 +1865,7 @@ __global__ void tile_init_kernel(TileLaunch t)
     for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask;
          k += gridDim.x * blockDim.x) {
         t.tasks[k].active = 0;
+        t.tasks[k].fragile = 0;
         t.tasks[k].iter = 0;
     }
 }

@@ -429,6 +429,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         if (!b.ensure_partials(total)) return false;
         hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
                        g_stream);
+        a.tasks = b.d_tasks;
         TileLaunch t;
         t.lk = a;
         t.tasks = b.d_tasks;
@@ -937,6 +938,7 @@ int rl2_obs_loglik(rl2_obs *obs, int slot, int npts, const double *e1, const dou
     if (n_iter) nit.resize((size_t)npts * kMaxR);
     if (!lik_eval(o, items, L_e, error, L_r ? Lr.data() : nullptr, n_iter ? nit.data() : nullptr))
         return RL2_ERR_HIP;
+    for (int i = 0; i < npts; i++) error[i] &= ~kErrFragile;
     const int nr = o.rg.numr;
     for (int i = 0; i < npts; i++)
         for (int r = 0; r < nr; r++) {
@@ -1046,7 +1048,7 @@ int rl2_tune_set(int knob, int value)
 {
     switch (knob) {
     case RL2_TUNE_GRID_VARIANT:
-        if (value != 0 && value != 1) break;
+        if (value < 0 || value > 4) break;
         tuning().grid_variant = value;
         return RL2_OK;
     case RL2_TUNE_TILE_CELLS:

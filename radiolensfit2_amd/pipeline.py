@@ -110,6 +110,13 @@ class DeviceObservation:
         n = n.value
         return fv[:n].copy(), fs[:n].copy(), uu[:n].copy(), vv[:n].copy(), sw
 
+    def extract_only(self, l0, m0, flux, R_mu, facet, slot=0):
+        """Extraction into a device facet slot without copying the facet back; returns n."""
+        n = C.c_uint(0)
+        N.check(N.hip().rl2_obs_extract(self.h, slot, l0, m0, flux, R_mu, facet, None, None,
+                                        None, None, None, C.byref(n)), "rl2_obs_extract")
+        return n.value
+
     def set_prior(self, rprior, slot=0):
         rp = np.ascontiguousarray(rprior, dtype=np.float64)
         N.check(N.hip().rl2_obs_set_prior(self.h, slot, N.dptr(rp)), "rl2_obs_set_prior")
