@@ -53,12 +53,16 @@ typedef struct {
     unsigned int facet, ncoords;
     int n_evals;        /* loglikelihood evaluations requested by the fit */
     int subtracted;     /* 1 when the fitted model was removed from the residual */
+    int start;          /* simplex start: 0 the reference's (0, 0), 1 the learned proposal */
+    int rounds;         /* batched likelihood launches the fit waited for */
 } rl2_gal_result;
 
 typedef struct {
     double extraction_s, fitting_s, subtraction_s;
     int n_bad_first, n_bad_final;
 } rl2_run_stats;
+
+typedef struct rl2_proposal rl2_proposal;
 
 /* Fit-driver options (0 = default for every field). */
 typedef struct {
@@ -76,6 +80,17 @@ typedef struct {
                               3: FP64 simplex, mixed sampling; 4: as 3, with samples whose
                               threshold test lies within the FP32 error bound re-evaluated
                               in FP64 */
+    rl2_proposal *proposal; /* learned proposal for the simplex start and first step, with
+                              the reference's start (0, 0), step 0.1 as the fallback when the
+                              proposal's likelihood is lower; NULL: the reference's start */
+    float *features_out;   /* optional, rl2_process: ngal * 770 network inputs of each
+                              galaxy's first-pass facet (training data) */
+    double *labels_out;    /* optional, rl2_process: ngal * 3 = (e1, e2, log r) of each
+                              first-pass fit, r maximising L_r + log prior at that e;
+                              NaN for failed fits (training labels) */
+    int proposal_mode;     /* 0: the proposal sets the simplex start, first step and the
+                              sampling line-search depth; 1: only the line-search depth
+                              (speculation: results identical to the reference start) */
 } rl2_fit_options;
 
 const char *rl2_last_error(void);
@@ -188,7 +203,6 @@ int rl2_test_cell_index(int n, const double *u, const double *v, double inv_lamb
  * predicts the likelihood peak (e1, e2, log r) and a 3x3 Cholesky factor of its precision.
  * It may only replace the simplex start point and first step (galaxy_fitting.c:59-67);
  * rl2_fit_options.proposal enables it in rl2_process / rl2_obs_fit. */
-typedef struct rl2_proposal rl2_proposal;
 int rl2_proposal_num_params(void);
 int rl2_proposal_num_inputs(void);   /* 770: 16x16 (re, im, weight) bins, log flux, log R */
 int rl2_proposal_num_outputs(void);  /* 9: mu(3), log diag(3), off-diagonal (3) */

@@ -36,7 +36,8 @@ class GalResult(C.Structure):
     _fields_ = [("e1", C.c_double), ("e2", C.c_double), ("var_e1", C.c_double),
                 ("var_e2", C.c_double), ("oneDimvar", C.c_double), ("maxL", C.c_double),
                 ("error", C.c_int), ("pass_", C.c_int), ("facet", C.c_uint),
-                ("ncoords", C.c_uint), ("n_evals", C.c_int), ("subtracted", C.c_int)]
+                ("ncoords", C.c_uint), ("n_evals", C.c_int), ("subtracted", C.c_int),
+                ("start", C.c_int), ("rounds", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -48,7 +49,9 @@ class RunStats(C.Structure):
 class FitOptions(C.Structure):
     _fields_ = [("batch", C.c_int), ("do_refit", C.c_int), ("nm_speculate", C.c_int),
                 ("line_chunk", C.c_int), ("lik_block", C.c_int), ("verbose", C.c_int),
-                ("lik_precision", C.c_int)]
+                ("lik_precision", C.c_int), ("proposal", C.c_void_p),
+                ("features_out", C.POINTER(C.c_float)), ("labels_out", C.POINTER(C.c_double)),
+                ("proposal_mode", C.c_int)]
 
 
 class KernelStat(C.Structure):

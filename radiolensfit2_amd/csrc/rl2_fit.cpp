@@ -61,6 +61,10 @@ struct Ctx {
     int n_verified = 0;   // FP64 re-evaluations requested by the verification
     int prec_nm = 0, prec_samp = 0;  // likelihood precision of the simplex / sampling phase
     bool verify = false;  // re-evaluate in FP64 the mixed values near a sampling threshold
+    bool has_prop = false;  // learned proposal for the simplex start
+    double prop_e1 = 0., prop_e2 = 0., prop_step = 0.1;
+    int start = 0;          // 1 when the simplex started from the proposal
+    int rounds = 0;         // batched launches this fit waited for
     int cur_prec = 0;                // precision of the pending requests
 };
 
@@ -173,12 +177,34 @@ Task fit_task(Ctx *c)
     {
     // ---- gsl_multimin_fminimizer_set (nmsimplex_set) ----
     Simplex s;
-    const double x0[2] = {c->start_e1, c->start_e2};
-    double xs[2][2] = {{x0[0] + c->step, x0[1]}, {x0[0], x0[1] + c->step}};
+    double x0[2] = {c->start_e1, c->start_e2};
+    double step = c->step;
+    double xs[2][2] = {{x0[0] + step, x0[1]}, {x0[0], x0[1] + step}};
     req.clear();
     int k0 = add_f(x0), ka = add_f(xs[0]), kb = add_f(xs[1]);
+    int kp0 = -1, kpa = -1, kpb = -1;
+    double xp[2] = {c->prop_e1, c->prop_e2};
+    double xps[2][2] = {{xp[0] + c->prop_step, xp[1]}, {xp[0], xp[1] + c->prop_step}};
+    if (c->has_prop) {
+        // the proposal's start triangle in the same launch; it replaces the reference's
+        // start (galaxy_fitting.c:59-67) only when its likelihood is higher
+        kp0 = add_f(xp);
+        kpa = add_f(xps[0]);
+        kpb = add_f(xps[1]);
+    }
     co_await Await{c, &req};
-    nref += (k0 >= 0) + (ka >= 0) + (kb >= 0);
+    nref += (k0 >= 0) + (ka >= 0) + (kb >= 0) + (kp0 >= 0) + (kpa >= 0) + (kpb >= 0);
+    if (c->has_prop && kp0 >= 0 && isfinite(fv(kp0)) && fv(kp0) < fv(k0)) {
+        c->start = 1;
+        x0[0] = xp[0];
+        x0[1] = xp[1];
+        step = c->prop_step;
+        for (int i = 0; i < 2; i++)
+            for (int j = 0; j < 2; j++) xs[i][j] = xps[i][j];
+        k0 = kp0;
+        ka = kpa;
+        kb = kpb;
+    }
     for (int j = 0; j < 2; j++) {
         s.x1[0][j] = x0[j];
         s.x1[1][j] = xs[0][j];
@@ -539,6 +565,8 @@ sampling:
     R.maxL = maxL;
     R.error = error;
     R.n_evals = nref;
+    R.start = c->start;
+    R.rounds = c->rounds;
     co_return;
 }
 
@@ -564,6 +592,11 @@ bool fit_batch(const std::vector<FitRequest> &reqs, const rl2_fit_options &opt,
         ctx[i].prec_nm = (lp == 1 || lp == 2) ? 1 : 0;
         ctx[i].prec_samp = (lp == 1 || lp == 3 || lp == 4) ? 1 : 0;
         ctx[i].verify = lp == 4;
+        ctx[i].has_prop = reqs[i].has_prop;
+        ctx[i].prop_e1 = reqs[i].prop_e1;
+        ctx[i].prop_e2 = reqs[i].prop_e2;
+        ctx[i].prop_step = reqs[i].prop_step;
+        if (reqs[i].prop_chunk > 0) ctx[i].line_chunk = reqs[i].prop_chunk;
         tasks.push_back(fit_task(&ctx[i]));
     }
     for (auto &t : tasks) t.h.resume();
@@ -590,6 +623,7 @@ bool fit_batch(const std::vector<FitRequest> &reqs, const rl2_fit_options &opt,
                 k++;
             }
             ctx[i].pending = nullptr;
+            ctx[i].rounds++;
             tasks[i].h.resume();
         }
     }

@@ -160,7 +160,7 @@ void launch_mlp_adam(hipStream_t s, long n, float *p, const float *g, float *m, 
 // Facet features: cells binned by q = 2 pi |(u, v)| R (R the catalog scalelength in
 // radians; 16 linear bins over q in [0, 4), the last open) and by position angle folded to
 // [0, pi) (the conjugate point carries the same information, Im changes sign); per bin the
-// mean of D sigma^2 / flux (re, im) and the count relative to the mean count; then
+// asinh of the mean of D sigma^2 / flux (re, im) and log(1 + count / mean count); then
 // log flux and log R in arcsec.  One work-group per facet slot.
 __global__ __launch_bounds__(256) void mlp_features_kernel(const FacetDev *facets, int slot,
                                                           double q_scale, double flux,
@@ -189,12 +189,13 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(const FacetDev *facet
         atomicAdd(&sw[b], 1.f);
     }
     __syncthreads();
+    // bin means are noise-dominated where few cells fall: asinh keeps them O(1..10)
     const float mean_count = n > 0 ? (float)n / kMlpBins : 1.f;
     for (int b = threadIdx.x; b < kMlpBins; b += blockDim.x) {
         const float w = sw[b];
-        out[3 * b + 0] = w > 0 ? sre[b] / w : 0.f;
-        out[3 * b + 1] = w > 0 ? sim[b] / w : 0.f;
-        out[3 * b + 2] = w / mean_count;
+        out[3 * b + 0] = w > 0 ? asinhf(sre[b] / w) : 0.f;
+        out[3 * b + 1] = w > 0 ? asinhf(sim[b] / w) : 0.f;
+        out[3 * b + 2] = log1pf(w / mean_count);
     }
     if (threadIdx.x == 0) {
         out[3 * kMlpBins] = (float)log(flux);

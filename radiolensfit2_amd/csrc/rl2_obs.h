@@ -71,6 +71,8 @@ struct Obs {
     RGrid rg;
     int lik_block = 0;       // 0: tiled engine; 64..1024: one work-group per (item, r)
     int loglik_prec = 0;     // precision of rl2_obs_loglik requests (0 FP64, 1 mixed)
+    float *d_feat = nullptr; // learned-proposal inputs of a batch (training data)
+    size_t feat_cap = 0;
     ~Obs();
     bool ensure_slots(int n);
     void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device
@@ -109,6 +111,11 @@ struct FitRequest {
     // likelihood_sampling only (likelihood.c:428-554): start from (e1, e2, maxL) in *res
     bool sampling_only = false;
     int np_max = kNpMax;
+    // learned proposal: simplex start point and first step; the reference's start is
+    // evaluated in the same launch and kept when its likelihood is higher
+    bool has_prop = false;
+    double prop_e1 = 0., prop_e2 = 0., prop_step = 0.1;
+    int prop_chunk = 0;   // line-search steps per launch from the predicted width (0: default)
 };
 using BatchEval = std::function<bool(const std::vector<LikItem> &, std::vector<double> &,
                                      std::vector<int> &)>;

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "rl2_mlp.h"
 #include "rl2_obs.h"
 
 namespace rl2 {
@@ -221,6 +222,7 @@ Obs::~Obs()
         dfree(s.scan);
     }
     dfree(d_facets);
+    dfree(d_feat);
     lik.release();
     dfree(lik.d_partials);
     lik.partials_cap = 0;
@@ -520,6 +522,107 @@ static void prior_for(Obs &o, int slot, double flux)
 
 // One pass of data_processing.c:49-142 over `todo` (batch > 1: data_processing-mpi.c:65-216).
 // pass 0 collects failed fits in `new_bad`; pass 1 (re-fit) reports them with e = 0.
+// Network inputs of the nb facets just extracted (slots 0..nb-1): copied out as training
+// data and/or run through the learned proposal, whose peak and width become each fit's
+// simplex start and first step.
+static bool proposal_inputs(Obs &o, const std::vector<unsigned> &todo, size_t b0, size_t nb,
+                            int pass, const double *flux, const double *scale,
+                            const rl2_fit_options &opt, std::vector<FitRequest> &reqs)
+{
+    float *x = nullptr;
+    if (opt.proposal) {
+        if (!opt.proposal->net.ensure((int)nb)) return false;
+        x = opt.proposal->net.x;
+    } else {
+        if (o.feat_cap < nb) {
+            dfree(o.d_feat);
+            if (!dmalloc(&o.d_feat, nb * kMlpIn)) return false;
+            o.feat_cap = nb;
+        }
+        x = o.d_feat;
+    }
+    for (size_t k = 0; k < nb; k++) {
+        const unsigned gal = todo[b0 + k];
+        launch_mlp_features(g_stream, o.d_facets, (int)k, scale[gal], flux[gal], x + k * kMlpIn);
+    }
+    if (opt.features_out && pass == 0)
+        for (size_t k = 0; k < nb; k++)
+            hipMemcpyAsync(opt.features_out + (size_t)todo[b0 + k] * kMlpIn, x + k * kMlpIn,
+                           kMlpIn * sizeof(float), hipMemcpyDeviceToHost, g_stream);
+    if (opt.proposal) {
+        Mlp &net = opt.proposal->net;
+        if (!net.forward(g_stream, (int)nb)) return false;
+        std::vector<float> out(nb * kMlpOut);
+        hipMemcpyAsync(out.data(), net.act[kMlpLayers - 1], out.size() * sizeof(float),
+                       hipMemcpyDeviceToHost, g_stream);
+        if (!hip_ok(hipStreamSynchronize(g_stream), "proposal")) return false;
+        for (size_t k = 0; k < nb; k++) {
+            const float *q = out.data() + k * kMlpOut;
+            // covariance of (e1, e2, log r) = (L L^T)^-1 for the precision factor L
+            const double l00 = exp((double)q[3]), l11 = exp((double)q[4]), l22 = exp((double)q[5]);
+            const double l10 = q[6], l20 = q[7], l21 = q[8];
+            // inverse of the lower-triangular L
+            const double i00 = 1. / l00, i11 = 1. / l11, i22 = 1. / l22;
+            const double i10 = -l10 * i00 * i11;
+            const double i21 = -l21 * i11 * i22;
+            const double i20 = -(l20 * i00 + l21 * i10) * i22;
+            // Sigma = L^-T L^-1: the e1, e2 variances
+            const double v1 = i00 * i00 + i10 * i10 + i20 * i20;
+            const double v2 = i11 * i11 + i21 * i21;
+            const double sd = sqrt(fmax(v1, v2));
+            if (!(isfinite(q[0]) && isfinite(q[1]) && isfinite(sd))) continue;
+            reqs[k].has_prop = opt.proposal_mode == 0;
+            reqs[k].prop_e1 = q[0];
+            reqs[k].prop_e2 = q[1];
+            reqs[k].prop_step = std::min(0.1, std::max(0.01, 2. * sd));
+            // the sampling line searches stop near 2.45 sigma (L / Lmax = 0.05 for a
+            // Gaussian): request that many 0.05 steps per launch (speculation only)
+            reqs[k].prop_chunk = std::min(8, std::max(2, (int)ceil(2.45 * sd / 0.05) + 1));
+        }
+    }
+    return hip_ok(hipGetLastError(), "proposal inputs") &&
+           hip_ok(hipStreamSynchronize(g_stream), "proposal inputs");
+}
+
+// Training labels of nb fitted galaxies: (e1, e2, log r*) with r* maximising
+// L_r + log prior at the fitted ellipticity; NaN for failed fits.
+static bool fit_labels(Obs &o, const std::vector<unsigned> &todo, size_t b0, size_t nb,
+                       const rl2_gal_result *results, double *labels)
+{
+    std::vector<LikItem> items(nb);
+    for (size_t k = 0; k < nb; k++) {
+        const rl2_gal_result &r = results[todo[b0 + k]];
+        items[k] = {(int)k, 0, r.e1, r.e2};
+    }
+    std::vector<double> Le(nb), Lr(nb * kMaxR);
+    std::vector<int> err(nb);
+    if (!lik_eval(o, items, Le.data(), err.data(), Lr.data(), nullptr)) return false;
+    for (size_t k = 0; k < nb; k++) {
+        const unsigned gal = todo[b0 + k];
+        const rl2_gal_result &r = results[gal];
+        double *lab = labels + 3 * (size_t)gal;
+        int best = -1;
+        double bv = -1e300;
+        for (int i = 1; i < o.rg.numr; i++) {
+            const double lr = Lr[k * kMaxR + i];
+            if (lr <= kBad) continue;
+            const double val = lr + o.slots[k].rprior[i];
+            if (val > bv) {
+                bv = val;
+                best = i;
+            }
+        }
+        if (r.error || best < 0) {
+            lab[0] = lab[1] = lab[2] = NAN;
+        } else {
+            lab[0] = r.e1;
+            lab[1] = r.e2;
+            lab[2] = log(o.rg.ro[best]);
+        }
+    }
+    return true;
+}
+
 bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const double *l,
                   const double *m, const double *flux, const double *scale,
                   const rl2_fit_options &opt, rl2_gal_result *results,
@@ -553,11 +656,16 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
             r.pass = pass;
             r.subtracted = 0;
         }
-        double t1 = now_s();
-        *t_ext += t1 - t0;
         std::vector<FitRequest> reqs;
         for (size_t k = 0; k < nb; k++) reqs.push_back({(int)k, &results[todo[b0 + k]]});
+        if (opt.proposal || (opt.features_out && pass == 0)) {
+            if (!proposal_inputs(o, todo, b0, nb, pass, flux, scale, opt, reqs)) return false;
+        }
+        double t1 = now_s();
+        *t_ext += t1 - t0;
         if (!fit_batch(reqs, opt, eval)) return false;
+        if (opt.labels_out && pass == 0 && !fit_labels(o, todo, b0, nb, results, opt.labels_out))
+            return false;
         double t2 = now_s();
         *t_fit += t2 - t1;
         for (size_t k = 0; k < nb; k++) {

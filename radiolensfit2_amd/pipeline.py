@@ -31,10 +31,17 @@ def rprior_for(flux, ro):
 
 
 def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0,
-                lik_precision=0):
+                lik_precision=0, proposal=None, features_out=None, labels_out=None,
+                proposal_mode=0):
     """rl2_fit_options; lik_block 0: tiled likelihood, else threads per (e, r) work-group;
-    lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex + FP64 sampling."""
-    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0, lik_precision)
+    lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex + FP64 sampling, 3: FP64
+    simplex + mixed sampling, 4: as 3 with FP64 verification; proposal: an rl2_proposal
+    handle (proposal_mode 0: simplex start and speculation depth, 1: speculation depth
+    only, results identical to the reference start); features_out / labels_out: float32
+    (ngal, 770) / float64 (ngal, 3) arrays."""
+    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0, lik_precision,
+                        proposal, None if features_out is None else N.fptr(features_out),
+                        None if labels_out is None else N.dptr(labels_out), proposal_mode)
 
 
 class DeviceObservation:
