@@ -146,17 +146,75 @@ def same_facet_acceptance(dev64, cat, n, opts):
             "error_flags_equal": bool(same.all())}
 
 
+def run_variants(dev, obs, cat, n_gal, opts, res_plain, plain_value):
+    """The same galaxies through the batched and mixed-precision paths (the same observation
+    and residual), and the learned proposal on fresh 64-dish galaxies (the array it was
+    trained on).  Untimed by the driver's clock: each variant reports its own timed region."""
+    out = {}
+
+    def de_vs(res):
+        de = np.array([max(abs(res[k].e1 - res_plain[k].e1), abs(res[k].e2 - res_plain[k].e2))
+                       for k in range(n_gal)])
+        same = np.array([res[k].error == res_plain[k].error for k in range(n_gal)])
+        ok = (de <= 1e-4) & same
+        return {"fraction": round(float(ok.mean()), 4), "median_abs_de": float(np.median(de)),
+                "max_abs_de": float(de.max())}
+
+    for batch in (16, 64):
+        dev.restore()
+        res, st, dt, _ = run_path(dev, cat, n_gal, 0, dict(opts, batch=batch), timed=False)
+        out[f"batched{batch}_fp64"] = {
+            "galaxies_per_s": round(n_gal / dt, 3), "speedup_vs_plain": round(n_gal / dt / plain_value, 3),
+            "parity": "FP64, the batch semantics of the oracle (test_process_matches_oracle, batch 4)",
+            "agreement_with_serial_loop": de_vs(res),
+            "note": "batch semantics of data_processing-mpi.c: a batch is extracted against one "
+                    "residual, so fits of overlapping galaxies differ from the serial loop"}
+        log(f"[bench] variant batch {batch}: {n_gal / dt:.2f} gal/s")
+    dev.restore()
+    mopts = dict(opts, lik_precision=4)
+    res, st, dt, _ = run_path(dev, cat, n_gal, 0, mopts, timed=False)
+    out["mixed_sampling_fp64_simplex"] = {
+        "galaxies_per_s": round(n_gal / dt, 3), "speedup_vs_plain": round(n_gal / dt / plain_value, 3),
+        "lik_precision": 4, "within_1e-4_of_plain_loop": de_vs(res),
+        "per_galaxy_same_facet": same_facet_acceptance(dev, cat, n_gal, mopts)}
+    log(f"[bench] variant mixed sampling: {n_gal / dt:.2f} gal/s")
+    try:
+        from radiolensfit2_amd import proposal as PR
+        net = PR.Proposal(0)
+        net.load(os.path.join(ROOT, "radiolensfit2_amd", "proposal_c1.npy"))
+        ev = PR.evaluate(synth.ARRAY_64, net, 1000, seed=9000, batch=64, log=log)
+        net.close()
+        out["learned_proposal_64dish"] = {
+            "workload": "1000 fresh galaxies, 64-dish array (the training array), batch 64",
+            **ev}
+    except OSError as e:  # weights file absent
+        out["learned_proposal_64dish"] = {"skipped": str(e)}
+    dev.restore()
+    return out
+
+
+def measured_traffic():
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes over
+    this workload (each counter scaled by what it reads for a copy of known size)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
 def roofline(kstats, lik_peak):
     lik, ext = kstats["likelihood"], kstats["extract"]
+    tr = measured_traffic()
     lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
     ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
     r_lik = {"kernel": "likelihood_kernel", "bound": "valu", "achieved": round(lik_tf, 4),
              "peak": lik_peak, "unit": "TFLOP/s", "frac": round(lik_tf / lik_peak, 5),
-             "traffic": None, "launches": lik["launches"],
+             "traffic": tr.get("likelihood_launch_bytes"), "launches": lik["launches"],
              "avg_launch_us": round(1e3 * lik["ms"] / max(1, lik["launches"]), 3)}
     r_ext = {"kernel": "grid_kernel (extraction)", "bound": "hbm", "achieved": round(ext_gbs, 2),
              "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ext_gbs / PEAK_HBM_GBS, 5),
-             "traffic": None, "launches": ext["launches"],
+             "traffic": tr.get("extraction_launch_bytes"), "launches": ext["launches"],
              "avg_launch_us": round(1e3 * ext["ms"] / max(1, ext["launches"]), 3)}
     return (r_lik if lik["ms"] >= ext["ms"] else r_ext), r_lik, r_ext
 
@@ -192,6 +250,8 @@ def main():
                     help="extraction gridding: 0 per-visibility LDS hash, 1 wave-aggregated, "
                          "2 / 3 / 4 per-thread runs of 8 / 4 / 4 rows, tables of 2048 / "
                          "1024 / 2048 slots")
+    ap.add_argument("--no-variants", dest="variants", action="store_false",
+                    help="skip the batched, mixed-precision and learned-proposal variants")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -278,6 +338,10 @@ def main():
         else:
             accept = {"reference": "this run is the FP64 path", "fraction": 1.0}
 
+    variants = None
+    if args.variants and not mixed and B == 1:
+        variants = run_variants(dev, obs, cat, n_gal, opts, res, value)
+
     cpu = parity = None
     if args.cpu_sample > 0:
         ncpu = min(args.cpu_sample, n_gal)
@@ -325,6 +389,7 @@ def main():
         "bad_fits": {"first_pass": st.n_bad_first, "after_refit": st.n_bad_final},
         "rerun_reproducibility": {"max_abs_de": float(rerun_de.max()),
                                   "galaxies_above_1e-10": int((rerun_de > 1e-10).sum())},
+        "variants": variants,
     }
     print(json.dumps(out), flush=True)
 

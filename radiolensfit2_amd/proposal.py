@@ -195,7 +195,9 @@ def evaluate(cfg, net, ngal, seed=9000, batch=64, log=print):
         ok = (de <= 1e-4) & (a["error"] == b["error"])
         summary[k]["within_1e-4_of_plain"] = float(ok.mean())
         summary[k]["median_abs_de_vs_plain"] = float(np.median(de))
-        summary[k]["fallback_fraction"] = 1.0 - summary[k]["proposal_start_fraction"]
+    # galaxies whose fit kept the reference's start (mode 1 never uses the proposal start)
+    summary["learned_start"]["fallback_fraction"] = \
+        1.0 - summary["learned_start"]["proposal_start_fraction"]
     return summary
 
 
