@@ -250,6 +250,12 @@ def main():
                     help="extraction gridding: 0 per-visibility LDS hash, 1 wave-aggregated, "
                          "2 / 3 / 4 per-thread runs of 8 / 4 / 4 rows, tables of 2048 / "
                          "1024 / 2048 slots")
+    ap.add_argument("--phase-tables", type=int, default=1, choices=[0, 1],
+                    help="FP64 Newton phases from per-pass column x row tables (1) or a "
+                         "sin/cos per cell (0)")
+    ap.add_argument("--fuse-subtract", type=int, default=1, choices=[0, 1],
+                    help="make each subtraction in the same residual pass as the next "
+                         "extraction (1) or in a pass of its own (0)")
     ap.add_argument("--no-variants", dest="variants", action="store_false",
                     help="skip the batched, mixed-precision and learned-proposal variants")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -283,6 +289,8 @@ def main():
     dev = make_device_observation(obs, full_cat, res_prec)
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_CELLS, args.tile_cells), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, args.grid_variant), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, args.phase_tables), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, args.fuse_subtract), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)

@@ -180,8 +180,13 @@ int rl2_synchronize(void);
  *   1 wave-aggregated cells, 2 / 3 / 4 runs of 8 / 4 / 4 consecutive rows merged per
  *   thread (LDS tables of 2048 / 1024 / 2048 slots); default 3.
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
- *   in [2048, 65536], or 0 (default) to choose per launch. */
-enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1 };
+ *   in [2048, 65536], or 0 (default) to choose per launch.
+ * RL2_TUNE_PHASE_TABLES: 1 (default) evaluates the FP64 Newton phases on extracted facets
+ *   as products of per-pass column and row tables; 0 evaluates sin/cos per cell.
+ * RL2_TUNE_FUSE_SUBTRACT: 1 (default) makes the last subtraction of a batch in the same pass
+ *   over the residual as the next extraction (gridding variant 3); 0 keeps them apart. */
+enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
+       RL2_TUNE_FUSE_SUBTRACT = 3 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

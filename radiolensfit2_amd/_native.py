@@ -23,10 +23,13 @@ c_ubyte_p = C.POINTER(C.c_ubyte)
 c_bool_p = C.POINTER(C.c_bool)
 
 RL2_OK = 0
+(RL2_ERR_NO_DEVICE, RL2_ERR_ARG, RL2_ERR_HIP, RL2_ERR_RANGE, RL2_ERR_STATE) = (-1, -2, -3, -4, -5)
 RL2_PREC_FP64 = 0
 RL2_PREC_MIXED = 1
 RL2_TUNE_GRID_VARIANT = 0
 RL2_TUNE_TILE_CELLS = 1
+RL2_TUNE_PHASE_TABLES = 2
+RL2_TUNE_FUSE_SUBTRACT = 3
 (RL2_K_EXTRACT, RL2_K_COMPACT, RL2_K_LIKELIHOOD, RL2_K_RMARG, RL2_K_SUBTRACT,
  RL2_K_SOURCES) = range(6)
 KERNEL_NAMES = ["extract", "compact", "likelihood", "rmarg", "subtract", "sources"]

@@ -305,7 +305,8 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     a.facet = sizeg;
     a.acc = acc.p;
     launch_grid(stream(), a);
-    launch_compact(stream(), acc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, dn.p, csw.p,
+    launch_compact(stream(), acc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
+                   dn.p, csw.p,
                    scan.p);
     unsigned n = 0;
     dn.down(&n, 1);
@@ -365,7 +366,8 @@ static unsigned compact_host(unsigned sizeg, double len, const double *re, const
     DBuf<double> dacc(4 * ncell), cu(ncell), cv(ncell), cre(ncell), cim(ncell), cs2(ncell);
     DBuf<unsigned> scan(compact_scratch_words(sizeg) + 1), dn(1);
     dacc.up(acc.data(), 4 * ncell);
-    launch_compact(stream(), dacc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, dn.p, nullptr,
+    launch_compact(stream(), dacc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
+                   dn.p, nullptr,
                    scan.p);
     unsigned n = 0;
     dn.down(&n, 1);

@@ -22,6 +22,13 @@ struct FacetDev {
     const double *mod;         // optional given model (drop-in cross_correlation); else null
     const float4 *pk;          // FP32 copy {u, v, D_re, D_im} for the mixed-precision kernel
     const float *invs2;        // FP32 1 / sigma^2
+    // Grid structure of a facet written by extraction (null / 0 for a caller-given facet):
+    // cell k sits at row cid[k] >> 16, column cid[k] & 0xffff, and its centre is
+    // (ctr[column], ctr[row]), bit-identical to uu[k], vv[k].
+    const uint32_t *cid;
+    const double *ctr;
+    unsigned int grid_facet;
+    unsigned int pad0;
     unsigned int n;
     unsigned int pad;
     double rprior[kMaxR];      // log prior per scalelength index
@@ -33,7 +40,7 @@ struct LikItem {
     int prec;                  // 0: FP64 kernel, 1: mixed-precision kernel
     double e1, e2;
     unsigned wg_first;         // tiled engine: first work-group of this item
-    unsigned pad;
+    unsigned tab_first;        // tiled engine: first phase-table entry of this item
 };
 
 struct RGrid {
@@ -78,6 +85,11 @@ struct GridLaunch {
     double len, inc;
     unsigned facet;
     double *acc;               // 4 * facet^2
+    // mode 0, gridding variant 3 only: apply this post-fit update to the residual first
+    // (written back through res_w), then grid the updated visibilities
+    int sub = 0;
+    GalaxyGeom sfit, srnd;
+    void *res_w = nullptr;
 };
 void launch_grid(hipStream_t s, const GridLaunch &a);
 
@@ -86,14 +98,19 @@ struct Tuning {
     int grid_variant = 3;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
                                // 1 wave-aggregated cells, 2 / 3 per-thread runs of 8 / 4
     unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
+    int phase_tables = 1;      // RL2_TUNE_PHASE_TABLES: separable Newton phases on grid facets
+    int fuse_subtract = 1;     // RL2_TUNE_FUSE_SUBTRACT: a subtraction rides on the next extraction
 };
 Tuning &tuning();
 
 // Compaction of dense accumulators into the facet arrays; clears the accumulators.
 // n_out (device) receives the number of non-empty cells. sum_w_dense (optional, device)
 // receives the dense weights before clearing.
+// cid / ctr (optional): packed row and column of each compacted cell, and the centre
+// coordinate of every grid index (facet entries).
 void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
-                    double *vv, double *dre, double *dim, double *s2, unsigned *n_out,
+                    double *vv, double *dre, double *dim, double *s2, uint32_t *cid, double *ctr,
+                    unsigned *n_out,
                     double *sum_w_dense, unsigned *scratch_blocks);
 size_t compact_scratch_words(unsigned facet);
 // FP32 packing of a compacted facet (count read from the device descriptor).
@@ -129,6 +146,10 @@ struct LikLaunch {
     double *det = nullptr;     // optional nitems * kMaxR: Hessian determinant at the peak
     const int *host_prec = nullptr;  // host copy of items[].prec (selects the kernels)
     const TileTask *tasks = nullptr; // tiled engine: per-(item, r) state (fragile flags)
+    // tiled engine: the r-marginalisation launch also writes the evaluation's status
+    // (tasks still iterating, largest step count) and FLOP-equivalents when set
+    unsigned *status = nullptr;
+    double *flops = nullptr;
 };
 // err bit set for a mixed-precision item one of whose L_r is fragile
 constexpr int kErrFragile = 0x100;
@@ -146,20 +167,23 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
+constexpr unsigned kTileLdsBytes = 40960;  // column-table staging per work-group (<= 1706 columns)
 
 struct TileLaunch {
     LikLaunch lk;              // items (with their first work-group), facets, r grid,
                                // outputs and QAG workspace
     TileTask *tasks;           // nitems * kMaxR
     double *partials;          // 8 doubles per work-group
-    unsigned *status;          // [0] tasks still active, [1] largest Newton step count
-    double *flops;             // FLOP-equivalents of the evaluation (sum over tasks)
     unsigned nwg;              // work-groups of a pass (nr * chunks summed over items)
     unsigned chunk_cells;      // facet cells per work-group
+    // Phase tables of the FP64 Newton passes on grid facets: per (item, r) task,
+    // e^{2 pi i xo ctr[k]} for the columns then e^{2 pi i yo ctr[k]} for the rows
+    // (2 * grid_facet entries from items[i].tab_first + (r - 1) * 2 * grid_facet); null: the
+    // passes evaluate sin/cos per cell.
+    double2 *tab;
+    unsigned tab_lds_facet;    // widest grid_facet of the launch's table items (LDS staging)
 };
-void launch_tile_init(hipStream_t s, const TileLaunch &t);
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
-void launch_tile_status(hipStream_t s, const TileLaunch &t);
 
 // Residual update for one fitted galaxy: R -= model(e) - model(0).
 void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,

@@ -51,6 +51,22 @@ __device__ inline double model_shape_rsq(const GalaxyGeom &g, double u, double v
     return rr * rr * rr * (spectra * g.flux);
 }
 
+// One visibility of the post-fit update (data_processing.c:109-135): R = (R - fit) + round,
+// both models with the phase rotor of the fitted galaxy's position (p = u l + v m + w n,
+// the phase k p in revolutions is p / lambda).
+__device__ inline double2 subtract_vis(const GalaxyGeom &fit, const GalaxyGeom &rnd, double p,
+                                       double u, double v, double k2, double spectra,
+                                       double inv_lambda, double2 r)
+{
+    double sn, cs;
+    sincos_rev64(p * inv_lambda, sn, cs);
+    const double sr = model_shape_rsq(rnd, u, v, k2, spectra);
+    const double sfit = model_shape_rsq(fit, u, v, k2, spectra);
+    r.x = (r.x - sfit * cs) + sr * cs;
+    r.y = (r.y - sfit * sn) + sr * sn;
+    return r;
+}
+
 // ------------------------------------------------------------------ sources / noise
 // R += sign * sum_g flux_g spec_c shape_g exp(i k_c (u l + v m + w n)).
 // One thread per row; galaxies staged through LDS in tiles of 128.
@@ -314,7 +330,11 @@ __global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
 // the same-address LDS atomics that serialise across a wave.  A channel inserts at most
 // RPT*256 runs into the HS-slot table (HS >= RPT*256, so probing always finds a slot);
 // flush as variant 0.
-template <int MODE, int PREC, int RPT, int HS>
+// SUB: first apply the previous galaxy's post-fit update (a.sfit, a.srnd) to each of the
+// thread's visibilities and write it back, then grid the updated value: the subtraction and
+// the next extraction share one pass over the residual, and every visibility gets the same
+// arithmetic, in the same order, as subtract_kernel followed by this kernel.
+template <int MODE, int PREC, int RPT, int HS, bool SUB = false>
 __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
 {
     constexpr int BS = 256;
@@ -355,37 +375,74 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
         atomicAdd(&hs2[h], xs);
         atomicAdd(&hcnt[h], n);
     };
+    // the rows' u, v and phase numerator p = ul + vm + wn stay in registers for all
+    // channels (the channel only scales them)
+    double ru[RPT], rv[RPT], rp[RPT], rps[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const unsigned long i = row0 + k;
+        const bool in = i < a.nrow;
+        ru[k] = in ? a.u[i] : 0.;
+        rv[k] = in ? a.v[i] : 0.;
+        const double wi = (in && MODE == 0) ? a.w[i] : 0.;
+        rp[k] = (in && MODE == 0) ? ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0 : 0.;
+        rps[k] = (in && SUB) ? ru[k] * a.sfit.l0 + rv[k] * a.sfit.m0 + wi * a.sfit.n0 : 0.;
+    }
     __syncthreads();
     for (unsigned c = 0; c < a.nch; c++) {
         const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
         const double wn2 = wn * wn, spectra = a.spec[c];
         uint32_t rkey = kEmpty, rn = 0;
         double rre = 0., rim = 0., rs2 = 0.;
+        // the thread's RPT consecutive visibilities are loaded together, so each load
+        // instruction's cache lines are still resident for the next one (issued one row
+        // at a time between the gridding work, they were fetched from HBM ~3 times over)
+        float vs2[RPT];
+        double2 vres[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const unsigned long i = row0 + k;
+            const size_t idx = (size_t)c * a.nrow + i;
+            vs2[k] = i < a.nrow ? a.s2f[idx] : -1.f;
+            vres[k] = (MODE != 2 && i < a.nrow) ? load_res(a.res, PREC, idx) : make_double2(0., 0.);
+        }
+        if (SUB) {
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                const unsigned long i = row0 + k;
+                if (i >= a.nrow) continue;
+                const size_t idx = (size_t)c * a.nrow + i;
+                const double2 r = subtract_vis(a.sfit, a.srnd, rps[k], ru[k], rv[k], wn2,
+                                               spectra, inv_lambda, vres[k]);
+                store_res(a.res_w, PREC, idx, r);
+                // the value a separate pass would read back
+                vres[k] = PREC == 0 ? r : make_double2((double)(float)r.x, (double)(float)r.y);
+            }
+        }
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
             const unsigned long i = row0 + k;
             if (i >= a.nrow) continue;
-            const size_t idx = (size_t)c * a.nrow + i;
-            const float s2 = a.s2f[idx];
+            const float s2 = vs2[k];
             if (signbit(s2)) continue;  // flagged
-            const double u = a.u[i], v = a.v[i];
+            const double u = ru[k], v = rv[k];
             uint32_t key;
             if (!cell_index_fast(u, v, inv_lambda, a.len, a.inc, inv_inc, a.facet, &key)) continue;
             double xr = 0., xi = 0.;
             if (MODE == 0) {
                 // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
                 // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
-                const double p = u * a.g.l0 + v * a.g.m0 + a.w[i] * a.g.n0;
+                const double p = rp[k];
                 const double shape = model_shape_rsq(a.g, u, v, wn2, spectra);
                 double sn, cs;
                 sincos_rev64(p * inv_lambda, sn, cs);
-                const double2 r = load_res(a.res, PREC, idx);
+                const double2 r = vres[k];
                 const double gre = shape * cs + r.x;
                 const double gim = shape * sn + r.y;
                 xr = gre * cs - gim * (-sn);
                 xi = gre * (-sn) + gim * cs;
             } else if (MODE == 1) {
-                const double2 r = load_res(a.res, PREC, idx);
+                const double2 r = vres[k];
                 xr = r.x;
                 xi = r.y;
             }
@@ -607,7 +664,10 @@ static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
     } else if (tuning().grid_variant == 3) {
         constexpr int RPT = 4;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
-        grid_run_kernel<MODE, PREC, RPT, 1024><<<grid, 256, 0, s>>>(a);
+        if (MODE == 0 && a.sub)
+            grid_run_kernel<MODE, PREC, RPT, 1024, true><<<grid, 256, 0, s>>>(a);
+        else
+            grid_run_kernel<MODE, PREC, RPT, 1024><<<grid, 256, 0, s>>>(a);
     } else if (tuning().grid_variant == 4) {
         constexpr int RPT = 4;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
@@ -699,7 +759,8 @@ __global__ __launch_bounds__(1024) void compact_scan_kernel(unsigned *bcount, un
 
 __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
     double *acc, unsigned facet, unsigned long ncell, double len, const unsigned *boff,
-    double *uu, double *vv, double *dre, double *dim, double *s2o, double *sum_w_dense)
+    double *uu, double *vv, double *dre, double *dim, double *s2o, uint32_t *cid, double *ctr,
+    double *sum_w_dense)
 {
     __shared__ unsigned tscan[kCompactBS];
     double *acc_re = acc, *acc_im = acc + ncell, *acc_s2 = acc + 2 * ncell, *acc_w = acc + 3 * ncell;
@@ -719,6 +780,10 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
     }
     unsigned n = boff[blockIdx.x] + tscan[threadIdx.x] - cnt;
     const double inc = 2 * len / facet;
+    if (ctr) {  // centre of grid index k, the same expression as uu / vv below
+        const unsigned k = blockIdx.x * kCompactBS + threadIdx.x;
+        if (k < facet) ctr[k] = (k + 0.5) * inc - len;
+    }
     for (int k = 0; k < kCompactPT; k++) {
         unsigned long p = base + k;
         if (p >= ncell) break;
@@ -732,6 +797,7 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
             unsigned j = (unsigned)(p / facet), i = (unsigned)(p % facet);
             uu[n] = (i + 0.5) * inc - len;
             vv[n] = (j + 0.5) * inc - len;
+            if (cid) cid[n] = (j << 16) | i;
             n++;
             acc_re[p] = 0.;
             acc_im[p] = 0.;
@@ -742,15 +808,17 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
 }
 
 void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
-                    double *vv, double *dre, double *dim, double *s2, unsigned *n_out,
-                    double *sum_w_dense, unsigned *scratch)
+                    double *vv, double *dre, double *dim, double *s2, uint32_t *cid, double *ctr,
+                    unsigned *n_out, double *sum_w_dense, unsigned *scratch)
 {
     unsigned long ncell = (unsigned long)facet * facet;
     unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    // ctr needs a thread per grid index: nb * kCompactBS = ceil(facet^2 / 4096) * 256 >= facet
+    // for every facet size
     compact_count_kernel<<<nb, kCompactBS, 0, s>>>(acc + 3 * ncell, ncell, scratch);
     compact_scan_kernel<<<1, 1024, 0, s>>>(scratch, nb, n_out);
     compact_write_kernel<<<nb, kCompactBS, 0, s>>>(acc, facet, ncell, len, scratch, uu, vv, dre,
-                                                   dim, s2, sum_w_dense);
+                                                   dim, s2, cid, ctr, sum_w_dense);
 }
 
 // ------------------------------------------------------------------ likelihood (K3)
@@ -1349,16 +1417,17 @@ __device__ double marginalise_posterior(int num, const double *x, double *y, Qag
     return log(sump) + ymax;
 }
 
-// Wave-cooperative variants: one 64-lane work-group per item.  Lanes 0..40 evaluate the
-// integrand at the 41 Gauss-Kronrod nodes; every lane then forms the QUADPACK sums from
-// the shared values in the sequential order, so all lanes follow the same control flow and
-// the result is bit-identical to the one-thread version above.
+// Wave-cooperative variants: one work-group of two waves per item.  Lanes 0..40 of a wave
+// evaluate the integrand at the 41 Gauss-Kronrod nodes; every lane then forms the QUADPACK
+// sums from the shared values in the sequential order, so all lanes follow the same
+// control flow and the result is bit-identical to the one-thread version above.  Each
+// bisection integrates its two halves on the two waves at once (fvs: 64 values per wave).
 __device__ inline void qk41_wave(const Spline &sp, double a, double b, double *fvs,
                                  double *result, double *abserr, double *resabs,
                                  double *resasc)
 {
     const int n = 21;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const double center = 0.5 * (a + b);
     const double half_length = 0.5 * (b - a);
     const double abs_half_length = fabs(half_length);
@@ -1422,7 +1491,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
     }
     int size = 0, nrmax = 0, imax = 0;
     *result = 0;
-    qk41_wave(sp, a, b, fvs, &result0, &abserr0, &resabs0, &resasc0);
+    qk41_wave(sp, a, b, fvs + 64 * (threadIdx.x >> 6), &result0, &abserr0, &resabs0, &resasc0);
     size = 1;
     if (writer) {
         w.rlist[0] = result0;
@@ -1449,8 +1518,29 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
         const int lev = w.level[imax];
         const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
         double area1, area2, error1, error2, resabs1, resabs2, resasc1, resasc2;
-        qk41_wave(sp, a1, b1, fvs, &area1, &error1, &resabs1, &resasc1);
-        qk41_wave(sp, a2, b2, fvs, &area2, &error2, &resabs2, &resasc2);
+        // the two halves at once: wave 0 integrates [a1, b1], wave 1 [a2, b2], each with
+        // its own node values; the results are exchanged through LDS
+        __shared__ double xch[8];
+        {
+            const int wv = threadIdx.x >> 6;
+            double r, e, ra, rc;
+            qk41_wave(sp, wv ? a2 : a1, wv ? b2 : b1, fvs + 64 * wv, &r, &e, &ra, &rc);
+            if ((threadIdx.x & 63) == 0) {
+                xch[4 * wv + 0] = r;
+                xch[4 * wv + 1] = e;
+                xch[4 * wv + 2] = ra;
+                xch[4 * wv + 3] = rc;
+            }
+        }
+        __syncthreads();
+        area1 = xch[0];
+        error1 = xch[1];
+        resabs1 = xch[2];
+        resasc1 = xch[3];
+        area2 = xch[4];
+        error2 = xch[5];
+        resabs2 = xch[6];
+        resasc2 = xch[7];
         const double area12 = area1 + area2;
         const double error12 = error1 + error2;
         errsum += (error12 - e_i);
@@ -1557,10 +1647,10 @@ __device__ double marginalise_posterior_wave(int num, const double *x, double *y
     return log(sump) + ymax;
 }
 
-// likelihood.c:126-146 for one item: L_e from its L_r, by every thread of the work-group
-// (at least 64); sp, fvs, x, y in LDS.
+// likelihood.c:126-146 for one item: L_e from its L_r, by every thread of the two-wave
+// work-group; sp, fvs (128 values), x, y and the QUADPACK lists in LDS.
 __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs, double *x,
-                           double *y)
+                           double *y, double *qd, int *qi)
 {
     const LikItem it = a.items[item];
     const FacetDev &F = a.facets[it.slot];
@@ -1586,14 +1676,12 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs
     double L_e;
     if (num > 4) {
         QagWs w;
-        double *base = a.qag_ws + (size_t)item * kQagDoubles;
-        int *ibase = a.qag_iws + (size_t)item * kQagInts;
-        w.alist = base;
-        w.blist = base + kQagLimit;
-        w.rlist = base + 2 * kQagLimit;
-        w.elist = base + 3 * kQagLimit;
-        w.order = ibase;
-        w.level = ibase + kQagLimit;
+        w.alist = qd;
+        w.blist = qd + kQagLimit;
+        w.rlist = qd + 2 * kQagLimit;
+        w.elist = qd + 3 * kQagLimit;
+        w.order = qi;
+        w.level = qi + kQagLimit;
         L_e = marginalise_posterior_wave(num, x, y, w, sp, fvs);
     } else {
         error = 1;
@@ -1609,12 +1697,23 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs
     }
 }
 
-__global__ __launch_bounds__(64) void rmarg_kernel(LikLaunch a)
+__device__ void tile_status(const LikLaunch &a);
+
+// The QUADPACK interval lists live in LDS (40 KB): lane 0 updates them and every lane
+// reads them back after each bisection, a round trip through L2 when they were in global
+// memory.
+__global__ __launch_bounds__(128) void rmarg_kernel(LikLaunch a)
 {
+    if (blockIdx.x == (unsigned)a.nitems) {  // the status group (tiled engine)
+        tile_status(a);
+        return;
+    }
     __shared__ Spline sp;
-    __shared__ double fvs[64];
+    __shared__ double fvs[128];
     __shared__ double x[kMaxR], y[kMaxR];
-    rmarg_item(a, blockIdx.x, sp, fvs, x, y);
+    __shared__ double qd[kQagDoubles];
+    __shared__ int qi[kQagInts];
+    rmarg_item(a, blockIdx.x, sp, fvs, x, y, qd, qi);
 }
 
 // ------------------------------------------------------------------ likelihood, tiled
@@ -1633,7 +1732,8 @@ __device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg)
     return lo;
 }
 
-// Work-groups [first, first + nr * chunks) of `item`.
+// Work-groups [first, first + nr * chunks) of `item`, chunk-major: the nr scalelength
+// tasks of one chunk are consecutive, so they read the same cells close together in time.
 __device__ inline void tile_range(const TileLaunch &t, int item, unsigned *first, unsigned *nch)
 {
     const int nr = t.lk.rg.numr - 1;
@@ -1641,6 +1741,48 @@ __device__ inline void tile_range(const TileLaunch &t, int item, unsigned *first
     const unsigned end = (item + 1 < t.lk.nitems) ? t.lk.items[item + 1].wg_first : t.nwg;
     *nch = (end - *first) / nr;
 }
+
+// Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD
+// has its own L2.  Block b takes logical work-group (b % 8) * per + b / 8, so every XCD
+// walks one contiguous eighth of the logical order: the tasks of a chunk run on one XCD
+// and share its L2.  Blocks past the logical count return at once.
+constexpr unsigned kXcds = 8;
+__device__ inline unsigned tile_logical(unsigned block, unsigned nwg)
+{
+    const unsigned per = (nwg + kXcds - 1) / kXcds;
+    return (block % kXcds) * per + block / kXcds;
+}
+inline unsigned tile_grid(unsigned nwg) { return (nwg + kXcds - 1) / kXcds * kXcds; }
+
+// Loads through the global address space: the facet arrays are reached through pointers
+// read from memory, which the compiler would otherwise address as flat (generic) memory,
+// and a flat load also holds the LDS counter every wait has to drain.
+#define RL2_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline T gload(const T *p, size_t i)
+{
+    return ((const RL2_GLOBAL T *)p)[i];
+}
+// HIP's vector structs through the matching builtin vector types (one 16-byte load)
+typedef double rl2_d2 __attribute__((ext_vector_type(2)));
+typedef float rl2_f4 __attribute__((ext_vector_type(4)));
+__device__ inline double2 gload(const double2 *p, size_t i)
+{
+    const rl2_d2 v = ((const RL2_GLOBAL rl2_d2 *)p)[i];
+    return make_double2(v.x, v.y);
+}
+__device__ inline float4 gload(const float4 *p, size_t i)
+{
+    const rl2_f4 v = ((const RL2_GLOBAL rl2_f4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+// Cells per thread whose loads are issued together before any of them is used: a thread
+// walks its cells beg + tid, beg + tid + 256, ... in batches of kTileU, so each memory
+// round trip serves kTileU cells.  Lanes past the chunk end load cell `beg` with zero data:
+// their terms are exactly zero, and every thread adds its cells in increasing order as
+// before.
+constexpr int kTileU = 4;
 
 // Chunk sums in FP64.  PASS 0: {S2, value, dx, dy, dxx, dyy, dxy} of the unnormalised
 // model at zero offset (likelihood.c:214-236); PASS 1: {value, dx, dy, dxx, dyy, dxy} at
@@ -1657,29 +1799,158 @@ __device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double 
     const double a1 = 1. + e1, a2 = 1. - e1;
 #pragma unroll
     for (int j = 0; j < 7; j++) s[j] = 0.;
-#pragma unroll 2
-    for (unsigned i = beg + threadIdx.x; i < end; i += kTileBS) {
-        const double uu = F.uu[i], vv = F.vv[i];
-        const double k1 = fma(e2, vv, a1 * uu);
-        const double k2 = fma(a2, vv, e2 * uu);
-        const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
-        const double M = rr * rr * rr;
-        const double hr = F.dre[i] * M;
-        const double hi = -F.dim[i] * M;
-        const double U = kTwoPi * uu, V = kTwoPi * vv;
-        if (PASS == 0) {
-            s[0] += M * M / F.s2[i];
+    for (unsigned base = beg + threadIdx.x; base < end; base += kTileU * kTileBS) {
+        double uu[kTileU], vv[kTileU], dr[kTileU], di[kTileU], s2[kTileU];
+#pragma unroll
+        for (int q = 0; q < kTileU; q++) {
+            const unsigned i = base + q * kTileBS;
+            const bool in = i < end;
+            const unsigned k = in ? i : beg;
+            uu[q] = gload(F.uu, k);
+            vv[q] = gload(F.vv, k);
+            dr[q] = in ? gload(F.dre, k) : 0.;
+            di[q] = in ? gload(F.dim, k) : 0.;
+            s2[q] = PASS == 0 ? gload(F.s2, k) : 1.;
+        }
+#pragma unroll
+        for (int q = 0; q < kTileU; q++) {
+            const double k1 = fma(e2, vv[q], a1 * uu[q]);
+            const double k2 = fma(a2, vv[q], e2 * uu[q]);
+            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = dr[q] * M;
+            const double hi = -di[q] * M;
+            const double U = kTwoPi * uu[q], V = kTwoPi * vv[q];
+            if (PASS == 0) {
+                if (base + q * kTileBS < end) s[0] += M * M / s2[q];
+                s[1] += hr;
+                s[2] = fma(-hi, U, s[2]);
+                s[3] = fma(-hi, V, s[3]);
+                const double hU = hr * U;
+                s[4] = fma(-hU, U, s[4]);
+                s[5] = fma(-hr * V, V, s[5]);
+                s[6] = fma(-hU, V, s[6]);
+            } else {
+                // phase xo U + yo V = 2 pi (xo u + yo v): evaluated in revolutions
+                double sn, cs;
+                sincos_rev64(fma(xo, uu[q], yo * vv[q]), sn, cs);
+                const double re = fma(hr, cs, -hi * sn);
+                const double im = fma(hr, sn, hi * cs);
+                s[0] += re;
+                s[1] = fma(-im, U, s[1]);
+                s[2] = fma(-im, V, s[2]);
+                const double rU = re * U;
+                s[3] = fma(-rU, U, s[3]);
+                s[4] = fma(-re * V, V, s[4]);
+                s[5] = fma(-rU, V, s[5]);
+            }
+        }
+    }
+}
+
+// PASS 0 sums of tile_sums64 on a facet with grid structure: the cell centres from the
+// LDS copy of ctr (sC) by cell id, so a cell costs 28 bytes of vector memory instead of 40.
+template <int U>
+__device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it, double ro,
+                                           const double *sC, unsigned beg, unsigned end,
+                                           double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    for (unsigned base = beg + threadIdx.x; base < end; base += U * kTileBS) {
+        uint32_t c[U];
+        double dr[U], di[U], s2[U];
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const unsigned i = base + q * kTileBS;
+            const bool in = i < end;
+            const unsigned k = in ? i : beg;
+            c[q] = gload(F.cid, k);
+            dr[q] = in ? gload(F.dre, k) : 0.;
+            di[q] = in ? gload(F.dim, k) : 0.;
+            s2[q] = gload(F.s2, k);
+        }
+#pragma unroll
+        for (int q = 0; q < U; q++) {
+            const double uu = sC[c[q] & 0xffffu], vv = sC[c[q] >> 16];
+            const double k1 = fma(e2, vv, a1 * uu);
+            const double k2 = fma(a2, vv, e2 * uu);
+            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = dr[q] * M;
+            const double hi = -di[q] * M;
+            const double U_ = kTwoPi * uu, V = kTwoPi * vv;
+            if (base + q * kTileBS < end) s[0] += M * M / s2[q];
             s[1] += hr;
-            s[2] = fma(-hi, U, s[2]);
+            s[2] = fma(-hi, U_, s[2]);
             s[3] = fma(-hi, V, s[3]);
-            const double hU = hr * U;
-            s[4] = fma(-hU, U, s[4]);
+            const double hU = hr * U_;
+            s[4] = fma(-hU, U_, s[4]);
             s[5] = fma(-hr * V, V, s[5]);
             s[6] = fma(-hU, V, s[6]);
-        } else {
-            // phase xo U + yo V = 2 pi (xo u + yo v): evaluated in revolutions
-            double sn, cs;
-            sincos_rev64(fma(xo, uu, yo * vv), sn, cs);
+        }
+    }
+}
+
+// PASS 1 sums of tile_sums64 on a facet with grid structure.  The Newton phase is
+// separable on the grid, 2 pi (xo u + yo v) with u = ctr[column], v = ctr[row], so
+// e^{i phase} is the product of the task's column entry X[column] = e^{2 pi i xo u} and row
+// entry Y[row] = e^{2 pi i yo v}, computed once per pass (lik_step_kernel) instead of a
+// sin/cos per cell.  The cell centres come from ctr, bit-identical to uu / vv.  The work-group
+// copies X and ctr to LDS first (sX, sC): per cell only the cell id and data then go
+// through the vector memory pipeline, which otherwise carried 68 bytes per cell and task.
+__device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
+                                       const double2 *X, const double2 *sX, const double *sC,
+                                       unsigned beg, unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+    const double2 *Y = X + F.grid_facet;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    for (unsigned base = beg + threadIdx.x; base < end; base += kTileU * kTileBS) {
+        uint32_t c[kTileU];
+        double dr[kTileU], di[kTileU];
+#pragma unroll
+        for (int q = 0; q < kTileU; q++) {
+            const unsigned i = base + q * kTileBS;
+            const bool in = i < end;
+            const unsigned k = in ? i : beg;
+            c[q] = gload(F.cid, k);
+            dr[q] = in ? gload(F.dre, k) : 0.;
+            di[q] = in ? gload(F.dim, k) : 0.;
+        }
+        double uu[kTileU], vv[kTileU];
+        double2 x[kTileU], y[kTileU];
+#pragma unroll
+        for (int q = 0; q < kTileU; q++) {
+            const unsigned ci = c[q] & 0xffffu, cj = c[q] >> 16;
+            uu[q] = sC[ci];
+            vv[q] = sC[cj];
+            x[q] = sX[ci];
+            y[q] = gload(Y, cj);  // the row is nearly uniform across a wave
+        }
+#pragma unroll
+        for (int q = 0; q < kTileU; q++) {
+            const double k1 = fma(e2, vv[q], a1 * uu[q]);
+            const double k2 = fma(a2, vv[q], e2 * uu[q]);
+            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = dr[q] * M;
+            const double hi = -di[q] * M;
+            const double U = kTwoPi * uu[q], V = kTwoPi * vv[q];
+            const double cs = fma(x[q].x, y[q].x, -(x[q].y * y[q].y));
+            const double sn = fma(x[q].y, y[q].x, x[q].x * y[q].y);
             const double re = fma(hr, cs, -hi * sn);
             const double im = fma(hr, sn, hi * cs);
             s[0] += re;
@@ -1714,7 +1985,7 @@ __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double 
         const unsigned stop = min(end, base + kFlush * kTileBS);
 #pragma unroll 2
         for (unsigned i = base; i < stop; i += kTileBS) {
-            const float4 p = F.pk[i];
+            const float4 p = gload(F.pk, i);
             const float k1 = fmaf(fe2, p.y, a1 * p.x);
             const float k2 = fmaf(a2, p.y, fe2 * p.x);
             const float rr = rsqrtf(fmaf(sw, fmaf(k1, k1, k2 * k2), 1.f));
@@ -1723,7 +1994,7 @@ __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double 
             const float hi = -p.w * M;
             const float U = tp * p.x, V = tp * p.y;
             if (PASS == 0) {
-                f[0] = fmaf(M * M, F.invs2[i], f[0]);
+                f[0] = fmaf(M * M, gload(F.invs2, i), f[0]);
                 f[1] += hr;
                 f[2] = fmaf(-hi, U, f[2]);
                 f[3] = fmaf(-hi, V, f[3]);
@@ -1750,8 +2021,8 @@ __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double 
     }
 }
 
-// The Newton loop of likelihood.c:284-350 one step at a time, by the last work-group of
-// a pass: returns 1 when the task has finished (L_r written).
+// The Newton loop of likelihood.c:284-350 one step at a time, by the step kernel after
+// each pass: returns 1 when the task has finished (L_r written).
 template <int PASS>
 __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
                                 const double (&v)[7])
@@ -1806,13 +2077,15 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
     const LikLaunch &a = t.lk;
-    const unsigned wg = blockIdx.x;
+    const unsigned wg = tile_logical(blockIdx.x, t.nwg);
+    if (wg >= t.nwg) return;
     const int item = tile_item(a.items, a.nitems, wg);
     unsigned first, nch;
     tile_range(t, item, &first, &nch);
+    const int nr = a.rg.numr - 1;
     const unsigned local = wg - first;
-    const int r = (int)(local / nch) + 1;
-    const unsigned chunk = local - (unsigned)(r - 1) * nch;
+    const unsigned chunk = local / (unsigned)nr;
+    const int r = (int)(local - chunk * (unsigned)nr) + 1;
     const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;  // finished task: the whole work-group leaves
     const LikItem it = a.items[item];
@@ -1821,7 +2094,25 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
     const unsigned end = min(F.n, beg + t.chunk_cells);
     const double xo = PASS == 1 ? T->xo : 0., yo = PASS == 1 ? T->yo : 0.;
     double s[7];
-    if (it.prec == 0)
+    if (PASS == 1 && it.prec == 0 && t.tab && F.cid) {
+        extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * (16 + 8) bytes
+        const unsigned gf = F.grid_facet;
+        const double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
+        double2 *sX = tab_lds;
+        double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
+        for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+            sX[k] = gload(X, k);
+            sC[k] = gload(F.ctr, k);
+        }
+        __syncthreads();
+        tile_sums64_tab(F, it, a.rg.ro[r], X, sX, sC, beg, end, s);
+    } else if (PASS == 0 && it.prec == 0 && t.tab && F.cid) {
+        extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
+        double *sC = reinterpret_cast<double *>(tab_lds);
+        for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
+        __syncthreads();
+        tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
+    } else if (it.prec == 0)
         tile_sums64<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
     else
         tile_sums32<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
@@ -1833,106 +2124,105 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
     }
 }
 
-// One wave per (item, r) task: the task's chunk sums added in a fixed order (lanes over
-// chunks, then a butterfly), then the Newton step or the end of the task.
+// One work-group per (item, r) task.  Wave 0 adds the task's chunk sums in a fixed order
+// (lanes over chunks, then a butterfly) and takes the Newton step or ends the task; when
+// the task goes on over a grid facet in FP64, the whole group then writes its phase tables
+// for the next pass.
 template <int PASS>
 __global__ __launch_bounds__(256) void lik_step_kernel(TileLaunch t)
 {
+    __shared__ int s_go;
+    __shared__ double s_off[2];
     const LikLaunch &a = t.lk;
     const int nr = a.rg.numr - 1;
-    const unsigned k = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (k >= (unsigned)a.nitems * nr) return;
+    const unsigned k = blockIdx.x;
     const int item = (int)(k / nr), r = (int)(k % nr) + 1;
     TileTask *T = t.tasks + (size_t)item * kMaxR + r;
-    if (PASS == 1 && !T->active) return;
-    unsigned first, nch;
-    tile_range(t, item, &first, &nch);
-    const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1) * nch) * 8;
-    double v[7] = {0., 0., 0., 0., 0., 0., 0.};
-    for (unsigned c = lane; c < nch; c += 64)
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        int go = 0;
+        if (PASS == 0 || T->active) {  // read by the whole wave before lane 0 updates it
+            unsigned first, nch;
+            tile_range(t, item, &first, &nch);
+            const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1)) * 8;
+            double v[7] = {0., 0., 0., 0., 0., 0., 0.};
+            for (unsigned c = lane; c < nch; c += 64)
 #pragma unroll
-        for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * 8 + j];
+                for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * nr * 8 + j];
 #pragma unroll
-    for (int j = 0; j < 7; j++)
-        for (int off = 32; off >= 1; off >>= 1) v[j] += __shfl_xor(v[j], off);
-    if (lane == 0) tile_step<PASS>(a, T, item, r, v);
-}
-
-__global__ void tile_init_kernel(TileLaunch t)
-{
-    const unsigned ntask = (unsigned)t.lk.nitems * kMaxR;
-    for (unsigned k = blockIdx.x * blockDim.x + threadIdx.x; k < ntask;
-         k += gridDim.x * blockDim.x) {
-        t.tasks[k].active = 0;
-        t.tasks[k].fragile = 0;
-        t.tasks[k].iter = 0;
+            for (int j = 0; j < 7; j++)
+                for (int off = 32; off >= 1; off >>= 1) v[j] += __shfl_xor(v[j], off);
+            if (lane == 0) {
+                go = !tile_step<PASS>(a, T, item, r, v);
+                s_off[0] = T->xo;
+                s_off[1] = T->yo;
+            }
+        }
+        if (lane == 0) s_go = go;
+    }
+    __syncthreads();
+    if (!s_go || !t.tab) return;
+    const LikItem &it = a.items[item];
+    const FacetDev &F = a.facets[it.slot];
+    if (it.prec != 0 || !F.cid) return;
+    const unsigned gf = F.grid_facet;
+    double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
+    const double xo = s_off[0], yo = s_off[1];
+    for (unsigned q = threadIdx.x; q < 2 * gf; q += blockDim.x) {
+        const bool col = q < gf;
+        double sn, cs;
+        sincos_rev64((col ? xo : yo) * F.ctr[col ? q : q - gf], sn, cs);
+        X[q] = make_double2(cs, sn);
     }
 }
 
+
 // status[0] = tasks still iterating, status[1] = largest Newton step count,
 // *flops = sum over tasks of n (37 + 25 N_it) (model + normalisation 19, zero-offset sums
-// 18, 25 per Newton pass; transcendentals, sqrt and division counted as one).
-__global__ __launch_bounds__(256) void tile_status_kernel(TileLaunch t)
+// 18, 25 per Newton pass; transcendentals, sqrt and division counted as one).  Run by the
+// first wave of one extra work-group of the r-marginalisation launch.
+__device__ void tile_status(const LikLaunch &a)
 {
-    __shared__ unsigned cnt[4], mx[4];
-    __shared__ double fl[4];
-    const int nr = t.lk.rg.numr - 1;
-    const unsigned ntask = (unsigned)t.lk.nitems * nr;
+    if (threadIdx.x >= 64) return;
+    const int nr = a.rg.numr - 1;
+    const unsigned ntask = (unsigned)a.nitems * nr;
     unsigned c = 0, m = 0;
     double f = 0.;
-    for (unsigned k = threadIdx.x; k < ntask; k += 256) {
+    for (unsigned k = threadIdx.x; k < ntask; k += 64) {
         const int item = (int)(k / nr);
-        const TileTask &T = t.tasks[(size_t)item * kMaxR + k % nr + 1];
+        const TileTask &T = a.tasks[(size_t)item * kMaxR + k % nr + 1];
         c += T.active ? 1u : 0u;
         m = max(m, (unsigned)T.iter);
-        f += (double)t.lk.facets[t.lk.items[item].slot].n * (37. + 25. * T.iter);
+        f += (double)a.facets[a.items[item].slot].n * (37. + 25. * T.iter);
     }
     for (int off = 32; off >= 1; off >>= 1) {
         c += __shfl_xor(c, off);
         m = max(m, (unsigned)__shfl_xor((int)m, off));
         f += __shfl_xor(f, off);
     }
-    if ((threadIdx.x & 63) == 0) {
-        cnt[threadIdx.x >> 6] = c;
-        mx[threadIdx.x >> 6] = m;
-        fl[threadIdx.x >> 6] = f;
-    }
-    __syncthreads();
     if (threadIdx.x == 0) {
-        t.status[0] = cnt[0] + cnt[1] + cnt[2] + cnt[3];
-        t.status[1] = max(max(mx[0], mx[1]), max(mx[2], mx[3]));
-        *t.flops = fl[0] + fl[1] + fl[2] + fl[3];
+        a.status[0] = c;
+        a.status[1] = m;
+        *a.flops = f;
     }
-}
-
-void launch_tile_init(hipStream_t s, const TileLaunch &t)
-{
-    const unsigned ntask = (unsigned)t.lk.nitems * kMaxR;
-    tile_init_kernel<<<(ntask + 255) / 256, 256, 0, s>>>(t);
 }
 
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned ntask = (unsigned)t.lk.nitems * (t.lk.rg.numr - 1);
     if (pass == 0) {
-        lik_tile_kernel<0><<<t.nwg, kTileBS, 0, s>>>(t);
-        lik_step_kernel<0><<<(ntask + 3) / 4, 256, 0, s>>>(t);
+        lik_tile_kernel<0><<<tile_grid(t.nwg), kTileBS, t.tab ? t.tab_lds_facet * 8 : 0, s>>>(t);
+        lik_step_kernel<0><<<ntask, 256, 0, s>>>(t);
     } else {
-        lik_tile_kernel<1><<<t.nwg, kTileBS, 0, s>>>(t);
-        lik_step_kernel<1><<<(ntask + 3) / 4, 256, 0, s>>>(t);
+        lik_tile_kernel<1><<<tile_grid(t.nwg), kTileBS, t.tab ? t.tab_lds_facet * 24 : 0, s>>>(t);
+        lik_step_kernel<1><<<ntask, 256, 0, s>>>(t);
     }
-}
-
-void launch_tile_status(hipStream_t s, const TileLaunch &t)
-{
-    tile_status_kernel<<<1, 256, 0, s>>>(t);
 }
 
 void launch_rmarg(hipStream_t s, const LikLaunch &a)
 {
     if (a.nitems <= 0) return;
-    rmarg_kernel<<<a.nitems, 64, 0, s>>>(a);
+    rmarg_kernel<<<a.nitems + (a.status ? 1 : 0), 128, 0, s>>>(a);
 }
 
 __global__ void marginalise_r_kernel(int num, const double *x, const double *y, double *out,
@@ -1973,14 +2263,9 @@ __global__ __launch_bounds__(256) void subtract_kernel(unsigned nch, unsigned lo
     const double p = ui * fit.l0 + vi * fit.m0 + wi * fit.n0;
     for (unsigned c = 0; c < nch; c++) {
         const double k = wn[c], k2 = k * k, spectra = spec[c];
-        double sn, cs;
-        sincos_rev64(p * inv_lambda[c], sn, cs);  // phase k p in revolutions
-        const double sr = model_shape_rsq(rnd, ui, vi, k2, spectra);
-        const double sfit = model_shape_rsq(fit, ui, vi, k2, spectra);
         const size_t idx = (size_t)c * nrow + i;
-        double2 r = load_res(res, PREC, idx);
-        r.x = (r.x - sfit * cs) + sr * cs;
-        r.y = (r.y - sfit * sn) + sr * sn;
+        const double2 r = subtract_vis(fit, rnd, p, ui, vi, k2, spectra, inv_lambda[c],
+                                       load_res(res, PREC, idx));
         store_res(res, PREC, idx, r);
     }
 }

@@ -21,6 +21,8 @@ struct FacetSlot {
     float4 *pk = nullptr;    // FP32 {u, v, D_re, D_im} for the mixed-precision likelihood
     float *invs2 = nullptr;
     unsigned *scan = nullptr;
+    uint32_t *cid = nullptr; // packed (row << 16 | column) per compacted cell
+    double *ctr = nullptr;   // cell-centre coordinate per grid index (fmax)
     unsigned facet = 0;
     unsigned n = 0;          // host copy of the compacted size (valid after a sync)
     double rprior[kMaxR] = {};
@@ -44,9 +46,12 @@ struct LikBuffers {
     TileTask *d_tasks = nullptr;                         // cap * kMaxR
     double *d_partials = nullptr;
     size_t partials_cap = 0;                             // work-groups
+    double2 *d_tab = nullptr;                            // Newton phase tables
+    size_t tab_cap = 0;                                  // entries
     int tile_passes = 6;     // Newton passes queued before the first status check
     bool ensure(int n);
     bool ensure_partials(size_t nwg);
+    bool ensure_tab(size_t entries);
     size_t out_bytes(int n) const;
     void release();
 };
@@ -95,8 +100,14 @@ void stats_add_work(int kernel, double flops, double bytes);
 
 GalaxyGeom make_geom(double l0, double m0, double flux, double scale, double e1, double e2);
 bool ensure_device();
+// A post-fit update (galaxy position, flux, scalelength, fitted e) held back to be made in
+// the same pass as the next extraction.
+struct PendingSub {
+    double l0, m0, flux, R_mu, e1, e2;
+};
 bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
-                 unsigned facet, bool want_sumw, double e1 = 0., double e2 = 0.);
+                 unsigned facet, bool want_sumw, double e1 = 0., double e2 = 0.,
+                 const PendingSub *sub = nullptr);
 bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2);
 bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const double *l,
                   const double *m, const double *flux, const double *scale,

@@ -195,6 +195,20 @@ bool LikBuffers::ensure_partials(size_t nwg)
     return true;
 }
 
+bool LikBuffers::ensure_tab(size_t entries)
+{
+    if (entries <= tab_cap) return true;
+    dfree(d_tab);
+    size_t c = 1 << 16;
+    while (c < entries) c *= 2;
+    if (!dmalloc(&d_tab, c)) {
+        tab_cap = 0;
+        return false;
+    }
+    tab_cap = c;
+    return true;
+}
+
 // ------------------------------------------------------------------ observation
 Obs::~Obs()
 {
@@ -220,12 +234,16 @@ Obs::~Obs()
         dfree(s.pk);
         dfree(s.invs2);
         dfree(s.scan);
+        dfree(s.cid);
+        dfree(s.ctr);
     }
     dfree(d_facets);
     dfree(d_feat);
     lik.release();
     dfree(lik.d_partials);
     lik.partials_cap = 0;
+    dfree(lik.d_tab);
+    lik.tab_cap = 0;
 }
 
 bool Obs::ensure_slots(int n)
@@ -238,7 +256,8 @@ bool Obs::ensure_slots(int n)
         FacetSlot &s = slots[i];
         if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
               dmalloc(&s.dre, ncell) && dmalloc(&s.dim, ncell) && dmalloc(&s.s2, ncell) &&
-              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
+              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) && dmalloc(&s.cid, ncell) &&
+              dmalloc(&s.ctr, (size_t)fmax) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
         if (!hip_ok(hipMemsetAsync(s.acc, 0, 4 * ncell * sizeof(double), g_stream), "memset"))
@@ -310,7 +329,7 @@ static double bytes_per_vis(const Obs &o, int kernel)
 bool ensure_device() { return ensure_init(); }
 
 bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
-                 unsigned facet, bool want_sumw, double e1, double e2)
+                 unsigned facet, bool want_sumw, double e1, double e2, const PendingSub *sub)
 {
     if (facet == 0 || facet > o.fmax) {
         set_error("facet size " + std::to_string(facet) + " outside the allocated facet (max " +
@@ -338,18 +357,37 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
     a.inc = 2 * o.len / facet;
     a.facet = facet;
     a.acc = s.acc;
+    if (sub) {  // the previous galaxy's post-fit update in the same pass
+        a.sub = 1;
+        a.sfit = make_geom(sub->l0, sub->m0, sub->flux, sub->R_mu, sub->e1, sub->e2);
+        a.srnd = make_geom(sub->l0, sub->m0, sub->flux, sub->R_mu, 0., 0.);
+        a.res_w = o.d_res;
+    }
     {
         KTimer t(RL2_K_EXTRACT);
         launch_grid(g_stream, a);
     }
-    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_EXTRACT) + o.nrow * 24.);
+    // read residual, sigma^2 and uvw; a fused pass also writes the residual back
+    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * (bytes_per_vis(o, RL2_K_EXTRACT) +
+                                                (sub ? (o.prec == 0 ? 16. : 8.) : 0.)) +
+                                          o.nrow * 24.);
     if (want_sumw && !s.sumw && !dmalloc(&s.sumw, (size_t)o.fmax * o.fmax)) return false;
     {
         KTimer t(RL2_K_COMPACT);
-        launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2,
-                       &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
+        launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
+                       s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
         launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
                           s.s2, s.pk, s.invs2);
+    }
+    // the facet now has grid structure: cell ids and centres for the phase tables (packed
+    // in 16-bit row / column fields)
+    FacetDev &f = o.h_facets[slot];
+    const unsigned gf = facet < 65536u ? facet : 0u;
+    if (f.cid != s.cid || f.grid_facet != gf) {
+        f.cid = gf ? s.cid : nullptr;
+        f.ctr = gf ? s.ctr : nullptr;
+        f.grid_facet = gf;
+        o.facet_upload(slot);
     }
     return hip_ok(hipGetLastError(), "extract launch");
 }
@@ -429,6 +467,21 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             total += nchunk * (unsigned)nr;
         }
         if (!b.ensure_partials(total)) return false;
+        // phase tables for FP64 items on grid facets: 2 * grid_facet entries per task
+        size_t ntab = 0;
+        unsigned tab_facet = 0;
+        for (int i = 0; i < n; i++) {
+            const FacetDev &f = o.h_facets[items[i].slot];
+            b.h_items[i].tab_first = (unsigned)ntab;
+            if (tuning().phase_tables && items[i].prec == 0 && f.cid && f.grid_facet) {
+                ntab += (size_t)nr * 2 * f.grid_facet;
+                tab_facet = std::max(tab_facet, f.grid_facet);
+            }
+        }
+        // offsets are 32-bit, and a work-group stages one task's column table (24 bytes per
+        // column) in LDS: beyond either limit the passes evaluate sin/cos per cell
+        if (ntab > 0xffffffffu || tab_facet * 24u > kTileLdsBytes) ntab = 0;
+        if (ntab && !b.ensure_tab(ntab)) return false;
         hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
                        g_stream);
         a.tasks = b.d_tasks;
@@ -436,11 +489,11 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         t.lk = a;
         t.tasks = b.d_tasks;
         t.partials = b.d_partials;
-        t.status = b.d_status;
-        t.flops = b.d_flops;
         t.nwg = total;
         t.chunk_cells = chunk;
-        launch_tile_init(g_stream, t);
+        t.tab = ntab ? b.d_tab : nullptr;
+        t.tab_lds_facet = ntab ? tab_facet : 0;
+        // every task's state is (re)initialised by the pass-0 step kernel
         {
             KTimer k(RL2_K_LIKELIHOOD);
             launch_tile_pass(g_stream, t, 0);
@@ -457,9 +510,11 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             passes += queued;
             {
                 KTimer k(RL2_K_RMARG);
-                launch_rmarg(g_stream, a);
+                LikLaunch ra = a;  // plus one work-group writing the status block
+                ra.status = b.d_status;
+                ra.flops = b.d_flops;
+                launch_rmarg(g_stream, ra);
             }
-            launch_tile_status(g_stream, t);
             copy_out();
             if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
             if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
@@ -631,6 +686,12 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
 {
     const int B = opt.batch > 0 ? opt.batch : 1;
     if (!o.ensure_slots(B)) return false;
+    // The last subtraction of a batch can ride on the first extraction of the next one
+    // (one pass over the residual instead of two): it is held here until then, and made on
+    // its own at the end of the list.
+    const bool fuse = tuning().fuse_subtract && tuning().grid_variant == 3;
+    PendingSub pend;
+    bool pending = false;
     o.lik_block = opt.lik_block > 0 ? opt.lik_block : 0;
     BatchEval eval = [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                          std::vector<int> &err) {
@@ -643,8 +704,10 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
             const unsigned gal = todo[b0 + k];
             prior_for(o, (int)k, flux[gal]);
             const unsigned facet = facet_size(scale[gal], o.len);
-            if (!obs_extract(o, (int)k, l[gal], m[gal], flux[gal], scale[gal], facet, false))
+            if (!obs_extract(o, (int)k, l[gal], m[gal], flux[gal], scale[gal], facet, false, 0.,
+                             0., pending ? &pend : nullptr))
                 return false;
+            pending = false;
             results[gal].facet = facet;
         }
         for (size_t k = 0; k < nb; k++) {
@@ -668,6 +731,10 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
             return false;
         double t2 = now_s();
         *t_fit += t2 - t1;
+        int last_good = -1;
+        for (size_t k = 0; k < nb; k++)
+            if (!results[todo[b0 + k]].error) last_good = (int)k;
+        const bool more = b0 + nb < todo.size();
         for (size_t k = 0; k < nb; k++) {
             const unsigned gal = todo[b0 + k];
             rl2_gal_result &r = results[gal];
@@ -679,7 +746,12 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
                     r.e2 = 0.;
                 }
             } else {
-                if (!obs_subtract(o, l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2)) return false;
+                if (fuse && more && (int)k == last_good) {
+                    pend = {l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2};
+                    pending = true;
+                } else if (!obs_subtract(o, l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2)) {
+                    return false;
+                }
                 r.subtracted = 1;
             }
             if (on_done) on_done(gal);
@@ -687,6 +759,8 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
         if (!hip_ok(hipStreamSynchronize(g_stream), "subtract sync")) return false;
         *t_sub += now_s() - t2;
     }
+    if (pending && !obs_subtract(o, pend.l0, pend.m0, pend.flux, pend.R_mu, pend.e1, pend.e2))
+        return false;
     return true;
 }
 
@@ -1017,6 +1091,13 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
     hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
     launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
                       s.invs2);
+    FacetDev &f = o.h_facets[slot];  // caller-given cells: no grid structure
+    if (f.cid || f.grid_facet) {
+        f.cid = nullptr;
+        f.ctr = nullptr;
+        f.grid_facet = 0;
+        o.facet_upload(slot);
+    }
     s.n = n;
     return hip_ok(hipStreamSynchronize(g_stream), "load_facet") ? RL2_OK : RL2_ERR_HIP;
 }
@@ -1165,6 +1246,14 @@ int rl2_tune_set(int knob, int value)
             break;
         tuning().tile_cells = (unsigned)value;
         return RL2_OK;
+    case RL2_TUNE_PHASE_TABLES:
+        if (value != 0 && value != 1) break;
+        tuning().phase_tables = value;
+        return RL2_OK;
+    case RL2_TUNE_FUSE_SUBTRACT:
+        if (value != 0 && value != 1) break;
+        tuning().fuse_subtract = value;
+        return RL2_OK;
     default:
         break;
     }
@@ -1179,6 +1268,8 @@ int rl2_tune_get(int knob, int *value)
     switch (knob) {
     case RL2_TUNE_GRID_VARIANT: *value = tuning().grid_variant; return RL2_OK;
     case RL2_TUNE_TILE_CELLS: *value = (int)tuning().tile_cells; return RL2_OK;
+    case RL2_TUNE_PHASE_TABLES: *value = tuning().phase_tables; return RL2_OK;
+    case RL2_TUNE_FUSE_SUBTRACT: *value = tuning().fuse_subtract; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

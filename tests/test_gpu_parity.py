@@ -75,6 +75,50 @@ def test_likelihood_matches_oracle(dev, case):
         assert abs(Le[k] - oLe) <= 1e-8, (k, Le[k], oLe)
 
 
+@pytest.mark.parametrize("g", [2, 9])
+def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
+    """A facet extracted on the device keeps its grid structure, so the FP64 Newton passes
+    take their phases from per-pass column x row tables.  On the device's own facet (copied
+    back and given to the oracle) L_r, the Newton step counts and L_e match the oracle, and
+    the per-cell sin/cos path (RL2_TUNE_PHASE_TABLES 0) gives the same values and fit."""
+    obs, cat, vs = case
+    facet = N.hip().facet_size(cat.scale[g], obs.len)
+    gv, gs, guu, gvv, gsw = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet,
+                                        slot=3)
+    params, keep = T.oracle_params(gv, gs, guu, gvv, cat.flux[g])
+    dev.set_prior(P.rprior_for(cat.flux[g], P.r_grid()), slot=3)
+    rng = np.random.default_rng(30 + g)
+    e = rng.uniform(-0.6, 0.6, size=(24, 2))
+    e = e[(e ** 2).sum(1) < 0.65]
+    print(f"extracted facet {facet} of galaxy {g}: {len(gv)} cells, {len(e)} points", flush=True)
+    Le, err, Lr, nit = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+    from oracle import oracle as O
+    import ctypes as C
+    for k in range(len(e)):
+        oLr = np.zeros(40)
+        onit = np.zeros(40, np.int32)
+        oerr = C.c_int()
+        oLe = O.lib().orl2_loglikelihood_detail(C.byref(params), e[k, 0], e[k, 1], C.byref(oerr),
+                                                O.ptr(oLr), onit.ctypes.data_as(O.c_int_p))
+        assert err[k] == oerr.value
+        np.testing.assert_array_equal(nit[k, 1:24], onit[1:24])
+        np.testing.assert_allclose(Lr[k, 1:24], oLr[1:24], rtol=0, atol=1e-8)
+        assert abs(Le[k] - oLe) <= 1e-8, (k, Le[k], oLe)
+    fit_t = dev.fit(slot=3)
+    H = N.hip()
+    N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 0), "rl2_tune_set")
+    try:
+        Le0, err0, Lr0, nit0 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+        fit_0 = dev.fit(slot=3)
+    finally:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 1), "rl2_tune_set")
+    np.testing.assert_array_equal(err0, err)
+    np.testing.assert_array_equal(nit0, nit)
+    np.testing.assert_allclose(Lr0[:, 1:24], Lr[:, 1:24], rtol=0, atol=1e-9)
+    assert fit_0.error == fit_t.error
+    assert abs(fit_0.e1 - fit_t.e1) <= 1e-8 and abs(fit_0.e2 - fit_t.e2) <= 1e-8
+
+
 @pytest.mark.parametrize("g", [0, 3, 7])
 def test_fit_matches_oracle(dev, case, g):
     obs, cat, vs = case
@@ -109,6 +153,33 @@ def test_process_matches_oracle(gpu, case, batch):
         assert abs(a.e1 - b.e1) <= 1e-6 and abs(a.e2 - b.e2) <= 1e-6, (g, a.e1, b.e1, a.e2, b.e2)
     scale = np.abs(ores_vis).max()
     assert np.abs(resid - ores_vis).max() <= 1e-10 * scale
+
+
+@pytest.mark.parametrize("batch", [1, 4])
+def test_fused_subtraction_matches_separate_passes(gpu, case, batch):
+    """A subtraction made in the same pass as the next extraction (RL2_TUNE_FUSE_SUBTRACT)
+    gives the results and residual of the separate subtract and extract passes; only the
+    gridding's atomic summation order differs between runs."""
+    obs, cat, vs = case
+    H = N.hip()
+    out = {}
+    try:
+        for fuse in (1, 0):
+            N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, fuse), "rl2_tune_set")
+            print(f"process batch {batch} fused {fuse}", flush=True)
+            d = P.DeviceObservation(obs, vis_sky=vs)
+            res, st = d.process(cat, batch=batch)
+            out[fuse] = (res, st, d.residual())
+            d.close()
+    finally:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1), "rl2_tune_set")
+    (ra, sa, va), (rb, sb, vb) = out[1], out[0]
+    assert (sa.n_bad_first, sa.n_bad_final) == (sb.n_bad_first, sb.n_bad_final)
+    for g in range(len(cat)):
+        a, b = ra[g], rb[g]
+        assert (a.error, a.pass_, a.subtracted) == (b.error, b.pass_, b.subtracted), g
+        assert abs(a.e1 - b.e1) <= 1e-9 and abs(a.e2 - b.e2) <= 1e-9, (g, a.e1, b.e1)
+    assert np.abs(va - vb).max() <= 1e-11 * np.abs(vb).max()
 
 
 def test_mixed_likelihood_tracks_fp64(dev, case):

@@ -210,3 +210,28 @@ def test_kernel_cell_index_equals_reference_division():
                                 fast.ctypes.data_as(N.c_uint_p), exact.ctypes.data_as(N.c_uint_p))
     assert bad == 0
     assert (fast == exact).all()
+
+
+def test_tune_knobs_validate_and_round_trip():
+    """rl2_tune_set accepts exactly the documented values and rl2_tune_get reads them back;
+    host-only, so it runs without a device."""
+    H = N.hip()
+    v = C.c_int()
+    cases = [(N.RL2_TUNE_GRID_VARIANT, [0, 1, 2, 3, 4], [-1, 5]),
+             (N.RL2_TUNE_TILE_CELLS, [0, 2048, 4096, 65536], [1024, 3000, 131072, -2048]),
+             (N.RL2_TUNE_PHASE_TABLES, [0, 1], [2, -1]),
+             (N.RL2_TUNE_FUSE_SUBTRACT, [0, 1], [2, -1])]
+    try:
+        for knob, good, bad in cases:
+            for x in good:
+                assert H.rl2_tune_set(knob, x) == 0
+                assert H.rl2_tune_get(knob, C.byref(v)) == 0 and v.value == x
+            for x in bad:
+                assert H.rl2_tune_set(knob, x) == N.RL2_ERR_ARG
+                assert v.value == good[-1]
+        assert H.rl2_tune_set(99, 0) == N.RL2_ERR_ARG
+    finally:  # defaults
+        H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 3)
+        H.rl2_tune_set(N.RL2_TUNE_TILE_CELLS, 0)
+        H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 1)
+        H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
