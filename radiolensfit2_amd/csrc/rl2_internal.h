@@ -100,6 +100,8 @@ struct Tuning {
     unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
     int phase_tables = 1;      // RL2_TUNE_PHASE_TABLES: separable Newton phases on grid facets
     int fuse_subtract = 1;     // RL2_TUNE_FUSE_SUBTRACT: a subtraction rides on the next extraction
+    unsigned tile_blocks = 0;  // RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass
+                               // (multiple of 8; 0: one block per chunk)
 };
 Tuning &tuning();
 
@@ -167,6 +169,7 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
+constexpr int kInlItems = 8;
 constexpr unsigned kTileLdsBytes = 40960;  // column-table staging per work-group (<= 1706 columns)
 
 struct TileLaunch {
@@ -182,6 +185,12 @@ struct TileLaunch {
     // passes evaluate sin/cos per cell.
     double2 *tab;
     unsigned tab_lds_facet;    // widest grid_facet of the launch's table items (LDS staging)
+    // Up to kInlItems items travel in the kernel arguments instead of a host-to-device
+    // copy: the pass-0 tile kernel reads them from here and its block 0 stores them to
+    // items_w (= lk.items) for the kernels after it.
+    unsigned n_inl;
+    LikItem *items_w;
+    LikItem inl[kInlItems];
 };
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
 

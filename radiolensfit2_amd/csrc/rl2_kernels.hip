@@ -1517,7 +1517,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
         const double r_i = w.rlist[imax], e_i = w.elist[imax];
         const int lev = w.level[imax];
         const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
-        double area1, area2, error1, error2, resabs1, resabs2, resasc1, resasc2;
+        double area1, area2, error1, error2, resasc1, resasc2;  // |f| integrals unused by qag
         // the two halves at once: wave 0 integrates [a1, b1], wave 1 [a2, b2], each with
         // its own node values; the results are exchanged through LDS
         __shared__ double xch[8];
@@ -1535,11 +1535,9 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
         __syncthreads();
         area1 = xch[0];
         error1 = xch[1];
-        resabs1 = xch[2];
         resasc1 = xch[3];
         area2 = xch[4];
         error2 = xch[5];
-        resabs2 = xch[6];
         resasc2 = xch[7];
         const double area12 = area1 + area2;
         const double error12 = error1 + error2;
@@ -1658,6 +1656,11 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs
     int num = 0;
     int nRo = 1;
     const int numR = a.rg.numr;
+    if (a.tasks) {  // tiled engine: an item with a task still iterating waits for the re-run
+        const int lane = threadIdx.x & 63;  // one task per lane (numR <= kMaxR <= 64)
+        const bool act = lane >= 1 && lane < numR && a.tasks[(size_t)item * kMaxR + lane].active;
+        if (__any(act)) return;
+    }
     while (nRo < numR && (L_r[nRo] <= kBad)) nRo++;
     if (nRo < numR) {
         for (nRo = 1; nRo < numR; nRo++) {
@@ -1734,25 +1737,25 @@ __device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg)
 
 // Work-groups [first, first + nr * chunks) of `item`, chunk-major: the nr scalelength
 // tasks of one chunk are consecutive, so they read the same cells close together in time.
-__device__ inline void tile_range(const TileLaunch &t, int item, unsigned *first, unsigned *nch)
+__device__ inline void tile_range(const TileLaunch &t, const LikItem *items, int item,
+                                  unsigned *first, unsigned *nch)
 {
     const int nr = t.lk.rg.numr - 1;
-    *first = t.lk.items[item].wg_first;
-    const unsigned end = (item + 1 < t.lk.nitems) ? t.lk.items[item + 1].wg_first : t.nwg;
+    *first = items[item].wg_first;
+    const unsigned end = (item + 1 < t.lk.nitems) ? items[item + 1].wg_first : t.nwg;
     *nch = (end - *first) / nr;
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD
-// has its own L2.  Block b takes logical work-group (b % 8) * per + b / 8, so every XCD
-// walks one contiguous eighth of the logical order: the tasks of a chunk run on one XCD
-// and share its L2.  Blocks past the logical count return at once.
+// has its own L2.  The blocks of XCD x walk the logical work-groups x * per ... (x + 1) *
+// per - 1 (per = ceil(nwg / 8)), so the tasks of a chunk run on one XCD and share its L2.
 constexpr unsigned kXcds = 8;
-__device__ inline unsigned tile_logical(unsigned block, unsigned nwg)
+// Blocks of a pass: one per logical work-group, or at most `cap` (a multiple of 8).
+inline unsigned tile_grid(unsigned nwg, unsigned cap)
 {
-    const unsigned per = (nwg + kXcds - 1) / kXcds;
-    return (block % kXcds) * per + block / kXcds;
+    const unsigned g = (nwg + kXcds - 1) / kXcds * kXcds;
+    return (cap && g > cap) ? cap : g;
 }
-inline unsigned tile_grid(unsigned nwg) { return (nwg + kXcds - 1) / kXcds * kXcds; }
 
 // Loads through the global address space: the facet arrays are reached through pointers
 // read from memory, which the compiler would otherwise address as flat (generic) memory,
@@ -1916,24 +1919,39 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     const double sw = sf * (4 * kPi * kPi);
     const double a1 = 1. + e1, a2 = 1. - e1;
     const double2 *Y = X + F.grid_facet;
+    constexpr int kTabU = 2;
 #pragma unroll
     for (int j = 0; j < 7; j++) s[j] = 0.;
-    for (unsigned base = beg + threadIdx.x; base < end; base += kTileU * kTileBS) {
-        uint32_t c[kTileU];
-        double dr[kTileU], di[kTileU];
+    // software-pipelined: the cell loads of the next batch are issued before the current
+    // batch is evaluated, so their memory latency overlaps its arithmetic
+    uint32_t cn[kTabU];
+    double drn[kTabU], din[kTabU];
+    auto fetch = [&](unsigned base) {
 #pragma unroll
-        for (int q = 0; q < kTileU; q++) {
+        for (int q = 0; q < kTabU; q++) {
             const unsigned i = base + q * kTileBS;
             const bool in = i < end;
             const unsigned k = in ? i : beg;
-            c[q] = gload(F.cid, k);
-            dr[q] = in ? gload(F.dre, k) : 0.;
-            di[q] = in ? gload(F.dim, k) : 0.;
+            cn[q] = gload(F.cid, k);
+            drn[q] = in ? gload(F.dre, k) : 0.;
+            din[q] = in ? gload(F.dim, k) : 0.;
         }
-        double uu[kTileU], vv[kTileU];
-        double2 x[kTileU], y[kTileU];
+    };
+    if (beg + threadIdx.x < end) fetch(beg + threadIdx.x);
+    for (unsigned base = beg + threadIdx.x; base < end; base += kTabU * kTileBS) {
+        uint32_t c[kTabU];
+        double dr[kTabU], di[kTabU];
 #pragma unroll
-        for (int q = 0; q < kTileU; q++) {
+        for (int q = 0; q < kTabU; q++) {
+            c[q] = cn[q];
+            dr[q] = drn[q];
+            di[q] = din[q];
+        }
+        if (base + kTabU * kTileBS < end) fetch(base + kTabU * kTileBS);
+        double uu[kTabU], vv[kTabU];
+        double2 x[kTabU], y[kTabU];
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
             const unsigned ci = c[q] & 0xffffu, cj = c[q] >> 16;
             uu[q] = sC[ci];
             vv[q] = sC[cj];
@@ -1941,7 +1959,7 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             y[q] = gload(Y, cj);  // the row is nearly uniform across a wave
         }
 #pragma unroll
-        for (int q = 0; q < kTileU; q++) {
+        for (int q = 0; q < kTabU; q++) {
             const double k1 = fma(e2, vv[q], a1 * uu[q]);
             const double k2 = fma(a2, vv[q], e2 * uu[q]);
             const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
@@ -2073,22 +2091,42 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
 }
 
 template <int PASS>
-__global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
+__device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red);
+
+// A block takes the logical work-groups q, q + nbx, q + 2 nbx, ... of its XCD's eighth of
+// the logical order (q = block / 8, nbx = blocks per XCD): with one block per logical
+// work-group that is exactly one, and a launch capped at the resident block count walks
+// the whole pass without further dispatches.
+template <int PASS>
+__global__ __launch_bounds__(kTileBS, 4) void lik_tile_kernel(TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
+    if (PASS == 0 && blockIdx.x == 0 && threadIdx.x < t.n_inl)
+        t.items_w[threadIdx.x] = t.inl[threadIdx.x];
+    const unsigned per = (t.nwg + kXcds - 1) / kXcds;
+    const unsigned nbx = gridDim.x / kXcds;
+    for (unsigned j = blockIdx.x / kXcds; j < per; j += nbx) {
+        const unsigned wg = (blockIdx.x % kXcds) * per + j;
+        if (wg >= t.nwg) break;
+        tile_chunk<PASS>(t, wg, red);
+    }
+}
+
+template <int PASS>
+__device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
+{
     const LikLaunch &a = t.lk;
-    const unsigned wg = tile_logical(blockIdx.x, t.nwg);
-    if (wg >= t.nwg) return;
-    const int item = tile_item(a.items, a.nitems, wg);
+    const LikItem *items = (PASS == 0 && t.n_inl) ? t.inl : a.items;
+    const int item = tile_item(items, a.nitems, wg);
     unsigned first, nch;
-    tile_range(t, item, &first, &nch);
+    tile_range(t, items, item, &first, &nch);
     const int nr = a.rg.numr - 1;
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)nr;
     const int r = (int)(local - chunk * (unsigned)nr) + 1;
     const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
-    if (PASS == 1 && !T->active) return;  // finished task: the whole work-group leaves
-    const LikItem it = a.items[item];
+    if (PASS == 1 && !T->active) return;  // finished task: the whole work-group moves on
+    const LikItem it = items[item];
     const FacetDev &F = a.facets[it.slot];
     const unsigned beg = chunk * t.chunk_cells;
     const unsigned end = min(F.n, beg + t.chunk_cells);
@@ -2126,10 +2164,11 @@ __global__ __launch_bounds__(kTileBS) void lik_tile_kernel(TileLaunch t)
 
 // One work-group per (item, r) task.  Wave 0 adds the task's chunk sums in a fixed order
 // (lanes over chunks, then a butterfly) and takes the Newton step or ends the task; when
-// the task goes on over a grid facet in FP64, the whole group then writes its phase tables
-// for the next pass.
+// the task goes on over a grid facet in FP64, the whole group (16 waves, so each lane
+// writes at most 3 entries of the 2 * facet) then writes its phase tables for the next pass.
+constexpr int kStepBS = 1024;
 template <int PASS>
-__global__ __launch_bounds__(256) void lik_step_kernel(TileLaunch t)
+__global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
 {
     __shared__ int s_go;
     __shared__ double s_off[2];
@@ -2143,7 +2182,7 @@ __global__ __launch_bounds__(256) void lik_step_kernel(TileLaunch t)
         int go = 0;
         if (PASS == 0 || T->active) {  // read by the whole wave before lane 0 updates it
             unsigned first, nch;
-            tile_range(t, item, &first, &nch);
+            tile_range(t, a.items, item, &first, &nch);
             const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1)) * 8;
             double v[7] = {0., 0., 0., 0., 0., 0., 0.};
             for (unsigned c = lane; c < nch; c += 64)
@@ -2211,11 +2250,11 @@ void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned ntask = (unsigned)t.lk.nitems * (t.lk.rg.numr - 1);
     if (pass == 0) {
-        lik_tile_kernel<0><<<tile_grid(t.nwg), kTileBS, t.tab ? t.tab_lds_facet * 8 : 0, s>>>(t);
-        lik_step_kernel<0><<<ntask, 256, 0, s>>>(t);
+        lik_tile_kernel<0><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 8 : 0, s>>>(t);
+        lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
     } else {
-        lik_tile_kernel<1><<<tile_grid(t.nwg), kTileBS, t.tab ? t.tab_lds_facet * 24 : 0, s>>>(t);
-        lik_step_kernel<1><<<ntask, 256, 0, s>>>(t);
+        lik_tile_kernel<1><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 24 : 0, s>>>(t);
+        lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
     }
 }
 

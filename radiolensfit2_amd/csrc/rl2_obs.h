@@ -34,7 +34,8 @@ struct LikBuffers {
     LikItem *d_items = nullptr, *h_items = nullptr;
     double *d_Lr = nullptr, *h_Lr = nullptr;
     int *d_nit = nullptr, *h_nit = nullptr;
-    // results block, copied back in one piece: status[2] u32 | FLOPs f64 | L_e[cap] | err[cap]
+    // results block in pinned host memory, written by the kernels through its device view
+    // d_out: status[2] u32 | FLOPs f64 | L_e[cap] | err[cap]
     char *d_out = nullptr, *h_out = nullptr;
     unsigned *d_status = nullptr, *h_status = nullptr;   // tasks still active, max steps
     double *d_flops = nullptr, *h_flops = nullptr;
@@ -52,7 +53,6 @@ struct LikBuffers {
     bool ensure(int n);
     bool ensure_partials(size_t nwg);
     bool ensure_tab(size_t entries);
-    size_t out_bytes(int n) const;
     void release();
 };
 

@@ -136,12 +136,15 @@ bool LikBuffers::ensure(int n)
     while (c < n) c *= 2;
     const size_t ob = 16 + 12 * (size_t)c;  // status[2] u32, FLOPs f64, L_e[c] f64, err[c] i32
     bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
-              dmalloc(&d_nit, (size_t)c * kMaxR) && dmalloc(&d_out, ob) &&
+              dmalloc(&d_nit, (size_t)c * kMaxR) &&
               dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts) &&
               dmalloc(&d_tasks, (size_t)c * kMaxR);
     ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * sizeof(TileTask), g_stream),
                       "memset");
+    // the results block lives in pinned host memory that the kernels write directly: the
+    // host reads it after the evaluation's one synchronisation, with no copy launch
     ok = ok && hip_ok(hipHostMalloc((void **)&h_out, ob), "hipHostMalloc") &&
+         hip_ok(hipHostGetDevicePointer((void **)&d_out, h_out, 0), "hipHostGetDevicePointer") &&
          hip_ok(hipHostMalloc((void **)&h_items, c * sizeof(LikItem)), "hipHostMalloc") &&
          hip_ok(hipHostMalloc((void **)&h_Lr, (size_t)c * kMaxR * sizeof(double)), "hipHostMalloc") &&
          hip_ok(hipHostMalloc((void **)&h_nit, (size_t)c * kMaxR * sizeof(int)), "hipHostMalloc");
@@ -157,13 +160,12 @@ bool LikBuffers::ensure(int n)
     h_err = (int *)(h_out + 16 + 8 * (size_t)c);
     return true;
 }
-size_t LikBuffers::out_bytes(int n) const { return 16 + 8 * (size_t)cap + 4 * (size_t)n; }
 void LikBuffers::release()
 {
     dfree(d_items);
     dfree(d_Lr);
     dfree(d_nit);
-    dfree(d_out);
+    d_out = nullptr;  // device view of h_out
     dfree(d_ws);
     dfree(d_iws);
     dfree(d_tasks);
@@ -435,8 +437,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         if (o.h_facets[items[i].slot].mod) tiled = false;  // given model: one group per (e, r)
     const bool want_detail = Lr || nit || (g_stats_on && !tiled);
     // one copy back per evaluation: status, FLOP count, L_e and the error flags
-    auto copy_out = [&]() {
-        hipMemcpyAsync(b.h_out, b.d_out, b.out_bytes(n), hipMemcpyDeviceToHost, g_stream);
+    auto copy_out = [&]() {  // the results block itself is written to host memory
         if (want_detail) {
             hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double),
                            hipMemcpyDeviceToHost, g_stream);
@@ -482,8 +483,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // column) in LDS: beyond either limit the passes evaluate sin/cos per cell
         if (ntab > 0xffffffffu || tab_facet * 24u > kTileLdsBytes) ntab = 0;
         if (ntab && !b.ensure_tab(ntab)) return false;
-        hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
-                       g_stream);
+        if (n > kInlItems)  // few items travel with the first kernel's arguments instead
+            hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
+                           g_stream);
         a.tasks = b.d_tasks;
         TileLaunch t;
         t.lk = a;
@@ -493,6 +495,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         t.chunk_cells = chunk;
         t.tab = ntab ? b.d_tab : nullptr;
         t.tab_lds_facet = ntab ? tab_facet : 0;
+        t.items_w = b.d_items;
+        t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
+        for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
         // every task's state is (re)initialised by the pass-0 step kernel
         {
             KTimer k(RL2_K_LIKELIHOOD);
@@ -1254,6 +1259,10 @@ int rl2_tune_set(int knob, int value)
         if (value != 0 && value != 1) break;
         tuning().fuse_subtract = value;
         return RL2_OK;
+    case RL2_TUNE_TILE_BLOCKS:
+        if (value < 0 || value % 8 || value > (1 << 20)) break;
+        tuning().tile_blocks = (unsigned)value;
+        return RL2_OK;
     default:
         break;
     }
@@ -1270,6 +1279,7 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_TILE_CELLS: *value = (int)tuning().tile_cells; return RL2_OK;
     case RL2_TUNE_PHASE_TABLES: *value = tuning().phase_tables; return RL2_OK;
     case RL2_TUNE_FUSE_SUBTRACT: *value = tuning().fuse_subtract; return RL2_OK;
+    case RL2_TUNE_TILE_BLOCKS: *value = (int)tuning().tile_blocks; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }
