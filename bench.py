@@ -255,6 +255,8 @@ def main():
                          "sin/cos per cell (0)")
     ap.add_argument("--tile-blocks", type=int, default=0,
                     help="cap on the blocks of a likelihood pass (multiple of 8; 0: one per chunk)")
+    ap.add_argument("--tile-streams", type=int, default=1, choices=[1, 2],
+                    help="scalelength task groups whose likelihood passes run on their own stream")
     ap.add_argument("--fuse-subtract", type=int, default=1, choices=[0, 1],
                     help="make each subtraction in the same residual pass as the next "
                          "extraction (1) or in a pass of its own (0)")
@@ -294,6 +296,7 @@ def main():
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, args.phase_tables), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, args.fuse_subtract), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, args.tile_blocks), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, args.tile_streams), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)

@@ -186,9 +186,11 @@ int rl2_synchronize(void);
  * RL2_TUNE_FUSE_SUBTRACT: 1 (default) makes the last subtraction of a batch in the same pass
  *   over the residual as the next extraction (gridding variant 3); 0 keeps them apart.
  * RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass, a multiple of 8 (each block
- *   then walks several chunks); 0 (default): one block per chunk. */
+ *   then walks several chunks); 0 (default): one block per chunk.
+ * RL2_TUNE_TILE_STREAMS: 2 splits the scalelength tasks of an evaluation in two groups
+ *   whose Newton passes run on two streams; 1 (default) runs them all on one. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
-       RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4 };
+       RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

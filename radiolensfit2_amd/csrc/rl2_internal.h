@@ -39,7 +39,7 @@ struct LikItem {
     int slot;
     int prec;                  // 0: FP64 kernel, 1: mixed-precision kernel
     double e1, e2;
-    unsigned wg_first;         // tiled engine: first work-group of this item
+    unsigned chunk_first;      // tiled engine: chunks of the items before this one
     unsigned tab_first;        // tiled engine: first phase-table entry of this item
 };
 
@@ -102,6 +102,7 @@ struct Tuning {
     int fuse_subtract = 1;     // RL2_TUNE_FUSE_SUBTRACT: a subtraction rides on the next extraction
     unsigned tile_blocks = 0;  // RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass
                                // (multiple of 8; 0: one block per chunk)
+    int tile_streams = 1;      // RL2_TUNE_TILE_STREAMS: task groups (streams) of the passes
 };
 Tuning &tuning();
 
@@ -177,8 +178,12 @@ struct TileLaunch {
                                // outputs and QAG workspace
     TileTask *tasks;           // nitems * kMaxR
     double *partials;          // 8 doubles per work-group
-    unsigned nwg;              // work-groups of a pass (nr * chunks summed over items)
+    unsigned nwg;              // work-groups of a pass (nr_g * chunks summed over items)
     unsigned chunk_cells;      // facet cells per work-group
+    // The launch covers the scalelength tasks r_lo .. r_lo + nr_g - 1 of every item: the
+    // tasks can be split in two groups whose passes run on two streams, so that one group's
+    // step kernel and launch gaps overlap the other group's sums.
+    int r_lo, nr_g;
     // Phase tables of the FP64 Newton passes on grid facets: per (item, r) task,
     // e^{2 pi i xo ctr[k]} for the columns then e^{2 pi i yo ctr[k]} for the rows
     // (2 * grid_facet entries from items[i].tab_first + (r - 1) * 2 * grid_facet); null: the

@@ -1722,12 +1722,12 @@ __global__ __launch_bounds__(128) void rmarg_kernel(LikLaunch a)
 // ------------------------------------------------------------------ likelihood, tiled
 // Largest item whose first work-group is <= wg (first work-groups are non-decreasing and
 // every item has at least one work-group).
-__device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg)
+__device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg, int nr_g)
 {
     int lo = 0, hi = nitems;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (items[mid].wg_first <= wg)
+        if (items[mid].chunk_first * (unsigned)nr_g <= wg)
             lo = mid;
         else
             hi = mid;
@@ -1735,15 +1735,16 @@ __device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg)
     return lo;
 }
 
-// Work-groups [first, first + nr * chunks) of `item`, chunk-major: the nr scalelength
+// Work-groups [first, first + nr_g * chunks) of `item`, chunk-major: the nr_g scalelength
 // tasks of one chunk are consecutive, so they read the same cells close together in time.
 __device__ inline void tile_range(const TileLaunch &t, const LikItem *items, int item,
                                   unsigned *first, unsigned *nch)
 {
-    const int nr = t.lk.rg.numr - 1;
-    *first = items[item].wg_first;
-    const unsigned end = (item + 1 < t.lk.nitems) ? items[item + 1].wg_first : t.nwg;
-    *nch = (end - *first) / nr;
+    const unsigned c0 = items[item].chunk_first;
+    const unsigned c1 = (item + 1 < t.lk.nitems) ? items[item + 1].chunk_first
+                                                  : t.nwg / (unsigned)t.nr_g;
+    *first = c0 * (unsigned)t.nr_g;
+    *nch = c1 - c0;
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD
@@ -2117,13 +2118,12 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
 {
     const LikLaunch &a = t.lk;
     const LikItem *items = (PASS == 0 && t.n_inl) ? t.inl : a.items;
-    const int item = tile_item(items, a.nitems, wg);
+    const int item = tile_item(items, a.nitems, wg, t.nr_g);
     unsigned first, nch;
     tile_range(t, items, item, &first, &nch);
-    const int nr = a.rg.numr - 1;
     const unsigned local = wg - first;
-    const unsigned chunk = local / (unsigned)nr;
-    const int r = (int)(local - chunk * (unsigned)nr) + 1;
+    const unsigned chunk = local / (unsigned)t.nr_g;
+    const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
     const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;  // finished task: the whole work-group moves on
     const LikItem it = items[item];
@@ -2173,9 +2173,9 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     __shared__ int s_go;
     __shared__ double s_off[2];
     const LikLaunch &a = t.lk;
-    const int nr = a.rg.numr - 1;
+    const int nr = t.nr_g;
     const unsigned k = blockIdx.x;
-    const int item = (int)(k / nr), r = (int)(k % nr) + 1;
+    const int item = (int)(k / nr), r = (int)(k % nr) + t.r_lo;
     TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
@@ -2183,7 +2183,7 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
         if (PASS == 0 || T->active) {  // read by the whole wave before lane 0 updates it
             unsigned first, nch;
             tile_range(t, a.items, item, &first, &nch);
-            const double *P0 = t.partials + (size_t)(first + (unsigned)(r - 1)) * 8;
+            const double *P0 = t.partials + (size_t)(first + (unsigned)(r - t.r_lo)) * 8;
             double v[7] = {0., 0., 0., 0., 0., 0., 0.};
             for (unsigned c = lane; c < nch; c += 64)
 #pragma unroll
@@ -2248,7 +2248,7 @@ __device__ void tile_status(const LikLaunch &a)
 
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
-    const unsigned ntask = (unsigned)t.lk.nitems * (t.lk.rg.numr - 1);
+    const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
     if (pass == 0) {
         lik_tile_kernel<0><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 8 : 0, s>>>(t);
         lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);

@@ -330,6 +330,17 @@ static double bytes_per_vis(const Obs &o, int kernel)
 
 bool ensure_device() { return ensure_init(); }
 
+// Second stream (default priority) and the events that order it against the library stream.
+static hipStream_t g_stream2 = nullptr;
+static hipEvent_t g_fork = nullptr, g_join = nullptr;
+static bool ensure_stream2()
+{
+    if (g_stream2) return true;
+    return hip_ok(hipStreamCreate(&g_stream2), "hipStreamCreate") &&
+           hip_ok(hipEventCreateWithFlags(&g_fork, hipEventDisableTiming), "hipEventCreate") &&
+           hip_ok(hipEventCreateWithFlags(&g_join, hipEventDisableTiming), "hipEventCreate");
+}
+
 bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
                  unsigned facet, bool want_sumw, double e1, double e2, const PendingSub *sub)
 {
@@ -460,13 +471,13 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             chunk = kTileMinCells;
             while (chunk < kTileMaxCells && groups(2 * chunk) >= kTileTargetGroups) chunk *= 2;
         }
-        unsigned total = 0;
+        unsigned chunks = 0;
         for (int i = 0; i < n; i++) {
             const unsigned cells = o.slots[items[i].slot].n;
-            const unsigned nchunk = std::max(1u, (cells + chunk - 1) / chunk);
-            b.h_items[i].wg_first = total;
-            total += nchunk * (unsigned)nr;
+            b.h_items[i].chunk_first = chunks;
+            chunks += std::max(1u, (cells + chunk - 1) / chunk);
         }
+        const unsigned total = chunks * (unsigned)nr;
         if (!b.ensure_partials(total)) return false;
         // phase tables for FP64 items on grid facets: 2 * grid_facet entries per task
         size_t ntab = 0;
@@ -487,31 +498,50 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
                            g_stream);
         a.tasks = b.d_tasks;
-        TileLaunch t;
-        t.lk = a;
-        t.tasks = b.d_tasks;
-        t.partials = b.d_partials;
-        t.nwg = total;
-        t.chunk_cells = chunk;
-        t.tab = ntab ? b.d_tab : nullptr;
-        t.tab_lds_facet = ntab ? tab_facet : 0;
-        t.items_w = b.d_items;
-        t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
-        for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
-        // every task's state is (re)initialised by the pass-0 step kernel
-        {
-            KTimer k(RL2_K_LIKELIHOOD);
-            launch_tile_pass(g_stream, t, 0);
+        // the scalelength tasks in one or two groups: the second group's passes run on a
+        // second stream, so one group's step kernels and launch gaps overlap the other's sums
+        const int ng = (tuning().tile_streams == 2 && nr >= 2 && ensure_stream2()) ? 2 : 1;
+        TileLaunch tg[2];
+        for (int g = 0; g < ng; g++) {
+            TileLaunch &t = tg[g];
+            t.lk = a;
+            t.tasks = b.d_tasks;
+            t.r_lo = g == 0 ? 1 : 1 + (nr + 1) / 2;
+            t.nr_g = ng == 1 ? nr : (g == 0 ? (nr + 1) / 2 : nr / 2);
+            t.nwg = chunks * (unsigned)t.nr_g;
+            t.partials = b.d_partials + (g == 0 ? 0 : (size_t)tg[0].nwg * 8);
+            t.chunk_cells = chunk;
+            t.tab = ntab ? b.d_tab : nullptr;
+            t.tab_lds_facet = ntab ? tab_facet : 0;
+            t.items_w = b.d_items;
+            t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
+            for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
         }
+        // Passes `first` .. `first + count - 1` (pass 0: zero-offset sums; every task's state
+        // is (re)initialised by the pass-0 step kernel).  The second stream starts after
+        // everything queued so far, and the r-marginalisation waits for both.
+        auto passes_of = [&](int first, int count) {
+            if (ng == 2) {
+                hipEventRecord(g_fork, g_stream);
+                hipStreamWaitEvent(g_stream2, g_fork, 0);
+            }
+            for (int p = first; p < first + count; p++) {
+                KTimer k(RL2_K_LIKELIHOOD);
+                launch_tile_pass(g_stream, tg[0], p == 0 ? 0 : 1);
+                if (ng == 2) launch_tile_pass(g_stream2, tg[1], p == 0 ? 0 : 1);
+            }
+            if (ng == 2) {
+                hipEventRecord(g_join, g_stream2);
+                hipStreamWaitEvent(g_stream, g_join, 0);
+            }
+        };
         // Newton passes are queued ahead of the status check (the previous evaluation's
         // largest step count + 1); a pass leaves finished tasks at once.  The
         // r-marginalisation is queued with them and repeated if tasks were still iterating.
-        int queued = b.tile_passes, passes = 0;
+        int queued = b.tile_passes, passes = 0, first = 0;
         while (true) {
-            for (int p = 0; p < queued; p++) {
-                KTimer k(RL2_K_LIKELIHOOD);
-                launch_tile_pass(g_stream, t, 1);
-            }
+            passes_of(first, first == 0 ? queued + 1 : queued);
+            first = 1;
             passes += queued;
             {
                 KTimer k(RL2_K_RMARG);
@@ -1263,6 +1293,10 @@ int rl2_tune_set(int knob, int value)
         if (value < 0 || value % 8 || value > (1 << 20)) break;
         tuning().tile_blocks = (unsigned)value;
         return RL2_OK;
+    case RL2_TUNE_TILE_STREAMS:
+        if (value != 1 && value != 2) break;
+        tuning().tile_streams = value;
+        return RL2_OK;
     default:
         break;
     }
@@ -1280,6 +1314,7 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_PHASE_TABLES: *value = tuning().phase_tables; return RL2_OK;
     case RL2_TUNE_FUSE_SUBTRACT: *value = tuning().fuse_subtract; return RL2_OK;
     case RL2_TUNE_TILE_BLOCKS: *value = (int)tuning().tile_blocks; return RL2_OK;
+    case RL2_TUNE_TILE_STREAMS: *value = tuning().tile_streams; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

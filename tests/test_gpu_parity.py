@@ -117,6 +117,17 @@ def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
     np.testing.assert_allclose(Lr0[:, 1:24], Lr[:, 1:24], rtol=0, atol=1e-9)
     assert fit_0.error == fit_t.error
     assert abs(fit_0.e1 - fit_t.e1) <= 1e-8 and abs(fit_0.e2 - fit_t.e2) <= 1e-8
+    # the scalelength tasks in two groups on two streams: every task's chunk sums are added
+    # in the same order, so the values are identical
+    N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 2), "rl2_tune_set")
+    try:
+        Le1, err1, Lr1, nit1 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+    finally:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 1), "rl2_tune_set")
+    np.testing.assert_array_equal(err1, err)
+    np.testing.assert_array_equal(nit1, nit)
+    np.testing.assert_array_equal(Lr1[:, 1:24], Lr[:, 1:24])
+    np.testing.assert_array_equal(Le1, Le)
 
 
 @pytest.mark.parametrize("g", [0, 3, 7])

@@ -221,7 +221,8 @@ def test_tune_knobs_validate_and_round_trip():
              (N.RL2_TUNE_TILE_CELLS, [0, 2048, 4096, 65536], [1024, 3000, 131072, -2048]),
              (N.RL2_TUNE_PHASE_TABLES, [0, 1], [2, -1]),
              (N.RL2_TUNE_FUSE_SUBTRACT, [0, 1], [2, -1]),
-             (N.RL2_TUNE_TILE_BLOCKS, [8, 1024, 0], [4, -8, 1001])]
+             (N.RL2_TUNE_TILE_BLOCKS, [8, 1024, 0], [4, -8, 1001]),
+             (N.RL2_TUNE_TILE_STREAMS, [2, 1], [0, 3])]
     try:
         for knob, good, bad in cases:
             for x in good:
@@ -237,3 +238,4 @@ def test_tune_knobs_validate_and_round_trip():
         H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 1)
         H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
         H.rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, 0)
+        H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 1)
