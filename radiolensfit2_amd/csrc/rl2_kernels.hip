@@ -1197,11 +1197,16 @@ struct QagWs {
     int *order, *level;
 };
 
+// QUADPACK's error rescaling (GSL 2.5 err.c).  (200 err / asc)^1.5 is formed as x sqrt(x):
+// within 1.5 ulp of the correctly rounded power (sqrt is correctly rounded), an order of
+// magnitude fewer instructions than pow on the r-marginalisation's serial chain, and the
+// value only feeds comparisons with tolerances.
 __device__ inline double rescale_error(double err, double result_abs, double result_asc)
 {
     err = fabs(err);
     if (result_asc != 0 && err != 0) {
-        double scale = pow((200 * err / result_asc), 1.5);
+        const double x = 200 * err / result_asc;
+        double scale = x * sqrt(x);
         err = (scale < 1) ? result_asc * scale : result_asc;
     }
     if (result_abs > kDblMin / (50 * kDblEps)) {
@@ -1417,54 +1422,46 @@ __device__ double marginalise_posterior(int num, const double *x, double *y, Qag
     return log(sump) + ymax;
 }
 
-// Wave-cooperative variants: one work-group of two waves per item.  Lanes 0..40 of a wave
-// evaluate the integrand at the 41 Gauss-Kronrod nodes; every lane then forms the QUADPACK
-// sums from the shared values in the sequential order, so all lanes follow the same
-// control flow and the result is bit-identical to the one-thread version above.  Each
-// bisection integrates its two halves on the two waves at once (fvs: 64 values per wave).
-__device__ inline void qk41_wave(const Spline &sp, double a, double b, double *fvs,
-                                 double *result, double *abserr, double *resabs,
-                                 double *resasc)
+// Wave-cooperative variants: one work-group of two waves per item.  Lane j of a wave
+// evaluates the integrand at Gauss-Kronrod node j (0..19 left of the centre, 20..39 right,
+// 40 the centre) and contributes its weighted terms to the Kronrod, Gauss, |f| and
+// |f - mean| sums, which butterfly reductions complete; every lane ends with the same
+// bit-identical sums (the pairwise additions are commutative), so all lanes follow the
+// same control flow.  The sums differ from QUADPACK's sequential order only by rounding.
+// Each bisection integrates its two halves on the two waves at once.
+__device__ inline double wave_sum(double v)
 {
-    const int n = 21;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ inline void qk41_wave(const Spline &sp, double a, double b, double *result,
+                                 double *abserr, double *resabs, double *resasc)
+{
     const int lane = threadIdx.x & 63;
     const double center = 0.5 * (a + b);
     const double half_length = 0.5 * (b - a);
     const double abs_half_length = fabs(half_length);
+    const int k = lane < 20 ? lane : lane - 20;  // node index of lanes 0..39
+    double f = 0., wk = 0., wg = 0.;
     if (lane < 41) {
         double x;
         if (lane < 20)
-            x = center - half_length * kXgk41[lane];
+            x = center - half_length * kXgk41[k];
         else if (lane < 40)
-            x = center + half_length * kXgk41[lane - 20];
+            x = center + half_length * kXgk41[k];
         else
             x = center;
-        fvs[lane] = spline_exp(sp, x);
+        f = spline_exp(sp, x);
+        wk = lane < 40 ? kWgk41[k] : kWgk41[20];
+        wg = (lane < 40 && (k & 1)) ? kWg41[k >> 1] : 0.;
     }
-    __syncthreads();
-    const double f_center = fvs[40];
-    double result_gauss = 0;
-    double result_kronrod = f_center * kWgk41[n - 1];
-    double result_abs = fabs(result_kronrod);
-    for (int j = 0; j < (n - 1) / 2; j++) {
-        const int jtw = j * 2 + 1;
-        const double f1 = fvs[jtw], f2 = fvs[20 + jtw];
-        const double fsum = f1 + f2;
-        result_gauss += kWg41[j] * fsum;
-        result_kronrod += kWgk41[jtw] * fsum;
-        result_abs += kWgk41[jtw] * (fabs(f1) + fabs(f2));
-    }
-    for (int j = 0; j < n / 2; j++) {
-        const int jtwm1 = j * 2;
-        const double f1 = fvs[jtwm1], f2 = fvs[20 + jtwm1];
-        result_kronrod += kWgk41[jtwm1] * (f1 + f2);
-        result_abs += kWgk41[jtwm1] * (fabs(f1) + fabs(f2));
-    }
+    double result_kronrod = wave_sum(wk * f);
+    const double result_gauss = wave_sum(wg * f);
+    double result_abs = wave_sum(wk * fabs(f));
     const double mean = result_kronrod * 0.5;
-    double result_asc = kWgk41[n - 1] * fabs(f_center - mean);
-    for (int j = 0; j < n - 1; j++)
-        result_asc += kWgk41[j] * (fabs(fvs[j] - mean) + fabs(fvs[20 + j] - mean));
-    __syncthreads();  // fvs is reused by the next call
+    double result_asc = wave_sum(lane < 41 ? wk * fabs(f - mean) : 0.);
     const double err = (result_kronrod - result_gauss) * half_length;
     result_kronrod *= half_length;
     result_abs *= abs_half_length;
@@ -1476,7 +1473,7 @@ __device__ inline void qk41_wave(const Spline &sp, double a, double b, double *f
 }
 
 __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, int limit,
-                          QagWs &w, double *fvs, double *result)
+                          QagWs &w, double *result)
 {
     double result0, abserr0, resabs0, resasc0;
     int roundoff_type1 = 0, roundoff_type2 = 0, error_type = 0;
@@ -1491,7 +1488,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
     }
     int size = 0, nrmax = 0, imax = 0;
     *result = 0;
-    qk41_wave(sp, a, b, fvs + 64 * (threadIdx.x >> 6), &result0, &abserr0, &resabs0, &resasc0);
+    qk41_wave(sp, a, b, &result0, &abserr0, &resabs0, &resasc0);
     size = 1;
     if (writer) {
         w.rlist[0] = result0;
@@ -1524,7 +1521,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
         {
             const int wv = threadIdx.x >> 6;
             double r, e, ra, rc;
-            qk41_wave(sp, wv ? a2 : a1, wv ? b2 : b1, fvs + 64 * wv, &r, &e, &ra, &rc);
+            qk41_wave(sp, wv ? a2 : a1, wv ? b2 : b1, &r, &e, &ra, &rc);
             if ((threadIdx.x & 63) == 0) {
                 xch[4 * wv + 0] = r;
                 xch[4 * wv + 1] = e;
@@ -1607,7 +1604,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
 
 // marginalise_r.c:81-223 for the wave: (x, y) in LDS, y shifted in place.
 __device__ double marginalise_posterior_wave(int num, const double *x, double *y, QagWs &w,
-                                             Spline &sp, double *fvs)
+                                             Spline &sp)
 {
     double ymax = -10.0;
     int ymaxpos = 0;
@@ -1634,21 +1631,21 @@ __device__ double marginalise_posterior_wave(int num, const double *x, double *y
     __syncthreads();
     if (st_init) return NAN;
     double sump = 0.;
-    int st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
+    int st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
     if (st) {
         __syncthreads();
         if (threadIdx.x == 0) spline_init_ws(sp, x, y, num, 1, spline_ws);
         __syncthreads();
-        st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, fvs, &sump);
+        st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
     }
     if (st) return kBad;
     return log(sump) + ymax;
 }
 
 // likelihood.c:126-146 for one item: L_e from its L_r, by every thread of the two-wave
-// work-group; sp, fvs (128 values), x, y and the QUADPACK lists in LDS.
-__device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs, double *x,
-                           double *y, double *qd, int *qi)
+// work-group; sp, x, y and the QUADPACK lists in LDS.
+__device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *x, double *y,
+                           double *qd, int *qi)
 {
     const LikItem it = a.items[item];
     const FacetDev &F = a.facets[it.slot];
@@ -1685,7 +1682,7 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *fvs
         w.elist = qd + 3 * kQagLimit;
         w.order = qi;
         w.level = qi + kQagLimit;
-        L_e = marginalise_posterior_wave(num, x, y, w, sp, fvs);
+        L_e = marginalise_posterior_wave(num, x, y, w, sp);
     } else {
         error = 1;
         L_e = kBad;
@@ -1712,11 +1709,10 @@ __global__ __launch_bounds__(128) void rmarg_kernel(LikLaunch a)
         return;
     }
     __shared__ Spline sp;
-    __shared__ double fvs[128];
     __shared__ double x[kMaxR], y[kMaxR];
     __shared__ double qd[kQagDoubles];
     __shared__ int qi[kQagInts];
-    rmarg_item(a, blockIdx.x, sp, fvs, x, y, qd, qi);
+    rmarg_item(a, blockIdx.x, sp, x, y, qd, qi);
 }
 
 // ------------------------------------------------------------------ likelihood, tiled
