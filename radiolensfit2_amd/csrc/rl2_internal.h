@@ -22,6 +22,7 @@ struct FacetDev {
     const double *mod;         // optional given model (drop-in cross_correlation); else null
     const float4 *pk;          // FP32 copy {u, v, D_re, D_im} for the mixed-precision kernel
     const float *invs2;        // FP32 1 / sigma^2
+    const double *invs2_64;    // FP64 1 / sigma^2
     // Grid structure of a facet written by extraction (null / 0 for a caller-given facet):
     // cell k sits at row cid[k] >> 16, column cid[k] & 0xffff, and its centre is
     // (ctr[column], ctr[row]), bit-identical to uu[k], vv[k].
@@ -119,7 +120,7 @@ size_t compact_scratch_words(unsigned facet);
 // FP32 packing of a compacted facet (count read from the device descriptor).
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
                        const double *vv, const double *dre, const double *dim, const double *s2,
-                       float4 *pk, float *invs2);
+                       float4 *pk, float *invs2, double *invs2_64);
 
 struct TileTask {
     double xo, yo;             // position offset of the current Newton iterate

@@ -1172,21 +1172,23 @@ void launch_likelihood(hipStream_t s, const LikLaunch &a)
 
 __global__ void pack_facet_kernel(const FacetDev *dfacet, const double *uu, const double *vv,
                                   const double *dre, const double *dim, const double *s2,
-                                  float4 *pk, float *invs2)
+                                  float4 *pk, float *invs2, double *invs2_64)
 {
     const unsigned n = dfacet->n;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         pk[i] = make_float4((float)uu[i], (float)vv[i], (float)dre[i], (float)dim[i]);
-        invs2[i] = (float)(1.0 / s2[i]);
+        const double w = 1.0 / s2[i];
+        invs2[i] = (float)w;
+        invs2_64[i] = w;
     }
 }
 
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
                        const double *vv, const double *dre, const double *dim, const double *s2,
-                       float4 *pk, float *invs2)
+                       float4 *pk, float *invs2, double *invs2_64)
 {
     unsigned blocks = std::min(2048u, (cap + 255) / 256 + 1);
-    pack_facet_kernel<<<blocks, 256, 0, s>>>(dfacet, uu, vv, dre, dim, s2, pk, invs2);
+    pack_facet_kernel<<<blocks, 256, 0, s>>>(dfacet, uu, vv, dre, dim, s2, pk, invs2, invs2_64);
 }
 
 // ------------------------------------------------------------------ r-marginalisation (K4)
@@ -1849,7 +1851,9 @@ __device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double 
 }
 
 // PASS 0 sums of tile_sums64 on a facet with grid structure: the cell centres from the
-// LDS copy of ctr (sC) by cell id, so a cell costs 28 bytes of vector memory instead of 40.
+// LDS copy of ctr (sC) by cell id, so a cell costs 28 bytes of vector memory instead of 40,
+// and M^2 / sigma^2 as M^2 times the packed 1 / sigma^2 (one rounding apart from the
+// quotient; the division sequence was a fifth of the pass's instructions).
 template <int U>
 __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it, double ro,
                                            const double *sC, unsigned beg, unsigned end,
@@ -1874,7 +1878,7 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
             c[q] = gload(F.cid, k);
             dr[q] = in ? gload(F.dre, k) : 0.;
             di[q] = in ? gload(F.dim, k) : 0.;
-            s2[q] = gload(F.s2, k);
+            s2[q] = gload(F.invs2_64, k);  // 1 / sigma^2
         }
 #pragma unroll
         for (int q = 0; q < U; q++) {
@@ -1886,7 +1890,7 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
             const double hr = dr[q] * M;
             const double hi = -di[q] * M;
             const double U_ = kTwoPi * uu, V = kTwoPi * vv;
-            if (base + q * kTileBS < end) s[0] += M * M / s2[q];
+            if (base + q * kTileBS < end) s[0] = fma(M * M, s2[q], s[0]);
             s[1] += hr;
             s[2] = fma(-hi, U_, s[2]);
             s[3] = fma(-hi, V, s[3]);
@@ -2036,25 +2040,27 @@ __device__ inline void tile_sums32(const FacetDev &F, const LikItem &it, double 
     }
 }
 
-// The Newton loop of likelihood.c:284-350 one step at a time, by the step kernel after
-// each pass: returns 1 when the task has finished (L_r written).
+// The Newton loop of likelihood.c:284-350 one step at a time: the task's state T advances
+// by the step from the sums v (PASS 0: the zero-offset sums, which also start the state);
+// returns 1 when the task has finished, with ho and the normalised determinant for L_r.
 template <int PASS>
-__device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
-                                const double (&v)[7])
+__device__ int newton_update(TileTask &T, const double (&v)[7], double *ho_out,
+                             double *det_out)
 {
     double value, dx, dy, dxx, dyy, dxy;
     if (PASS == 0) {
-        T->S2 = v[0];
+        T.S2 = v[0];
         value = v[1];
         dx = v[2];
         dy = v[3];
         dxx = v[4];
         dyy = v[5];
         dxy = v[6];
-        T->xo = 0.;
-        T->yo = 0.;
-        T->norm = 1.;
-        T->iter = 0;
+        T.xo = 0.;
+        T.yo = 0.;
+        T.norm = 1.;
+        T.iter = 0;
+        T.fragile = 0;
     } else {
         value = v[0];
         dx = v[1];
@@ -2064,27 +2070,42 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
         dxy = v[5];
     }
     const double det = dxx * dyy - dxy * dxy;
-    if (det > 0 && T->norm > 1.e-7 && T->iter < 50) {
+    if (det > 0 && T.norm > 1.e-7 && T.iter < 50) {
         const double incx = (dx * dyy - dy * dxy) / det;
         const double incy = (dy * dxx - dx * dxy) / det;
-        T->xo -= incx;
-        T->yo -= incy;
-        T->iter += 1;
-        T->norm = sqrt(incx * incx + incy * incy);
-        T->active = 1;
+        T.xo -= incx;
+        T.yo -= incy;
+        T.iter += 1;
+        T.norm = sqrt(incx * incx + incy * incy);
+        T.active = 1;
         return 0;
     }
-    T->active = 0;
-    const double S2 = T->S2;
+    T.active = 0;
+    const double S2 = T.S2;
     // branches an FP32 sum could decide differently: the determinant sign, the Newton
     // step cap, the position-marginalisation cut x <= Delta (likelihood.c:395)
     const double ho = value / sqrt(S2);
     const double scale = fabs(dxx * dyy) + dxy * dxy;
     const double B = ho * ho * 0.5, Delta = 0.5 * 5.991;
-    T->fragile = (fabs(det) <= 1e-5 * scale) || T->iter >= 50 ||
-                 fabs(B - Delta) <= 1e-4 * fabs(B) || !isfinite(det);
-    finish_lr(a, item, r, ho, det / S2, T->iter);
+    T.fragile = (fabs(det) <= 1e-5 * scale) || T.iter >= 50 ||
+                fabs(B - Delta) <= 1e-4 * fabs(B) || !isfinite(det);
+    *ho_out = ho;
+    *det_out = det / S2;
     return 1;
+}
+
+// newton_update on the task's state in memory, writing L_r when the task finishes (the
+// step kernel of the two-kernel passes).
+template <int PASS>
+__device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
+                         const double (&v)[7])
+{
+    TileTask S = *T;
+    double ho, det;
+    const int done = newton_update<PASS>(S, v, &ho, &det);
+    *T = S;
+    if (done) finish_lr(a, item, r, ho, det, S.iter);
+    return done;
 }
 
 template <int PASS>
@@ -2140,7 +2161,7 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
         }
         __syncthreads();
         tile_sums64_tab(F, it, a.rg.ro[r], X, sX, sC, beg, end, s);
-    } else if (PASS == 0 && it.prec == 0 && t.tab && F.cid) {
+    } else if (PASS == 0 && it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet) {
         extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
         double *sC = reinterpret_cast<double *>(tab_lds);
         for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
@@ -2246,7 +2267,7 @@ void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
     if (pass == 0) {
-        lik_tile_kernel<0><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 8 : 0, s>>>(t);
+        lik_tile_kernel<0><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab_lds_facet * 8, s>>>(t);
         lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
     } else {
         lik_tile_kernel<1><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 24 : 0, s>>>(t);

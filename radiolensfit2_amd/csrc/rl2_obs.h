@@ -20,6 +20,7 @@ struct FacetSlot {
     double *sumw = nullptr;  // dense weights, allocated on first request
     float4 *pk = nullptr;    // FP32 {u, v, D_re, D_im} for the mixed-precision likelihood
     float *invs2 = nullptr;
+    double *invs2_64 = nullptr;
     unsigned *scan = nullptr;
     uint32_t *cid = nullptr; // packed (row << 16 | column) per compacted cell
     double *ctr = nullptr;   // cell-centre coordinate per grid index (fmax)

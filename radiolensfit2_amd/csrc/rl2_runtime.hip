@@ -235,6 +235,7 @@ Obs::~Obs()
         dfree(s.sumw);
         dfree(s.pk);
         dfree(s.invs2);
+        dfree(s.invs2_64);
         dfree(s.scan);
         dfree(s.cid);
         dfree(s.ctr);
@@ -258,7 +259,8 @@ bool Obs::ensure_slots(int n)
         FacetSlot &s = slots[i];
         if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
               dmalloc(&s.dre, ncell) && dmalloc(&s.dim, ncell) && dmalloc(&s.s2, ncell) &&
-              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) && dmalloc(&s.cid, ncell) &&
+              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
+              dmalloc(&s.invs2_64, ncell) && dmalloc(&s.cid, ncell) &&
               dmalloc(&s.ctr, (size_t)fmax) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
@@ -285,6 +287,7 @@ bool Obs::ensure_slots(int n)
         f.mod = nullptr;
         f.pk = slots[i].pk;
         f.invs2 = slots[i].invs2;
+        f.invs2_64 = slots[i].invs2_64;
         f.n = 0;
         // full descriptor once (n = 0 until a compaction or load writes it)
         if (!hip_ok(hipMemcpyAsync(d_facets + i, &f, sizeof(FacetDev), hipMemcpyHostToDevice,
@@ -390,7 +393,7 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
         launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
                        s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
         launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
-                          s.s2, s.pk, s.invs2);
+                          s.s2, s.pk, s.invs2, s.invs2_64);
     }
     // the facet now has grid structure: cell ids and centres for the phase tables (packed
     // in 16-bit row / column fields)
@@ -1125,7 +1128,7 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
     hipMemcpyAsync(s.vv, vv, n * 8, hipMemcpyHostToDevice, g_stream);
     hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
     launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
-                      s.invs2);
+                      s.invs2, s.invs2_64);
     FacetDev &f = o.h_facets[slot];  // caller-given cells: no grid structure
     if (f.cid || f.grid_facet) {
         f.cid = nullptr;
