@@ -286,14 +286,14 @@ def main():
                     help="extraction gridding: 0 per-visibility LDS hash, 1 wave-aggregated, "
                          "2 / 3 / 4 per-thread runs of 8 / 4 / 4 rows, tables of 2048 / "
                          "1024 / 2048 slots")
-    ap.add_argument("--phase-tables", type=int, default=1, choices=[0, 1],
-                    help="FP64 Newton phases from per-pass column x row tables (1) or a "
-                         "sin/cos per cell (0)")
+    ap.add_argument("--phase-tables", type=int, default=2, choices=[0, 1, 2],
+                    help="FP64 Newton phases from per-pass column x row tables with the rows "
+                         "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")
     ap.add_argument("--tile-blocks", type=int, default=0,
                     help="cap on the blocks of a likelihood pass (multiple of 8; 0: one per chunk)")
     ap.add_argument("--tile-streams", type=int, default=1, choices=[1, 2],
                     help="scalelength task groups whose likelihood passes run on their own stream")
-    ap.add_argument("--fuse-subtract", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--fuse-subtract", type=int, default=1, choices=[0, 1, 2],
                     help="make each subtraction in the same residual pass as the next "
                          "extraction (1) or in a pass of its own (0)")
     ap.add_argument("--no-variants", dest="variants", action="store_false",

@@ -181,8 +181,10 @@ int rl2_synchronize(void);
  *   thread (LDS tables of 2048 / 1024 / 2048 slots); default 3.
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
  *   in [2048, 65536], or 0 (default) to choose per launch.
- * RL2_TUNE_PHASE_TABLES: 1 (default) evaluates the FP64 Newton phases on extracted facets
- *   as products of per-pass column and row tables; 0 evaluates sin/cos per cell.
+ * RL2_TUNE_PHASE_TABLES: 2 (default) evaluates the FP64 Newton phases on extracted facets
+ *   as products of per-pass column and row tables, both staged in LDS when they fit (40
+ *   bytes per grid index within 40 KB); 1 stages only the column table; 0 evaluates
+ *   sin/cos per cell.  All three give the same values up to rounding (1 and 2 identical).
  * RL2_TUNE_FUSE_SUBTRACT: 1 (default) makes the last subtraction of a batch in the same pass
  *   over the residual as the next extraction (gridding variant 3); 0 keeps them apart.
  * RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass, a multiple of 8 (each block

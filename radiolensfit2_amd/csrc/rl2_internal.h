@@ -99,7 +99,7 @@ struct Tuning {
     int grid_variant = 3;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
                                // 1 wave-aggregated cells, 2 / 3 per-thread runs of 8 / 4
     unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
-    int phase_tables = 1;      // RL2_TUNE_PHASE_TABLES: separable Newton phases on grid facets
+    int phase_tables = 2;      // RL2_TUNE_PHASE_TABLES: separable Newton phases on grid facets
     int fuse_subtract = 1;     // RL2_TUNE_FUSE_SUBTRACT: a subtraction rides on the next extraction
     unsigned tile_blocks = 0;  // RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass
                                // (multiple of 8; 0: one block per chunk)
@@ -191,6 +191,8 @@ struct TileLaunch {
     // passes evaluate sin/cos per cell.
     double2 *tab;
     unsigned tab_lds_facet;    // widest grid_facet of the launch's table items (LDS staging)
+    unsigned tab_y_lds;        // 1: the row tables are staged in LDS too
+    unsigned grid;             // 1: every item is FP64 on an extracted facet with tables
     // Up to kInlItems items travel in the kernel arguments instead of a host-to-device
     // copy: the pass-0 tile kernel reads them from here and its block 0 stores them to
     // items_w (= lk.items) for the kernels after it.

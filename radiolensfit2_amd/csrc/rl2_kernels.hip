@@ -40,6 +40,16 @@ __device__ inline double model_shape(const GalaxyGeom &g, double u, double v, do
     return shape * (spectra * g.flux);
 }
 
+// 1 / sqrt(a) for a >= 1: the hardware estimate refined once, r + r e (1/2 + 3/8 e) with
+// e = 1 - a r^2 (bit-identical to rsqrt(), which adds a class test of the estimate for
+// zero, infinite and denormal arguments that a >= 1 never has).
+__device__ inline double rsqrt_ge1(double a)
+{
+    const double r = __builtin_amdgcn_rsq(a);
+    const double e = fma(-a * r, r, 1.);
+    return fma(r * e, fma(e, 0.375, 0.5), r);
+}
+
 // The same model with (1 + s w |k|^2)^(-3/2) as rsqrt(den)^3 (one reciprocal square
 // root instead of a square root and a division; a few ulp from the reference's form).
 __device__ inline double model_shape_rsq(const GalaxyGeom &g, double u, double v, double wn2,
@@ -47,7 +57,7 @@ __device__ inline double model_shape_rsq(const GalaxyGeom &g, double u, double v
 {
     const double k1 = fma(g.e2, v, (1. + g.e1) * u);
     const double k2 = fma(1. - g.e1, v, g.e2 * u);
-    const double rr = rsqrt(fma(g.sf * wn2, fma(k1, k1, k2 * k2), 1.));
+    const double rr = rsqrt_ge1(fma(g.sf * wn2, fma(k1, k1, k2 * k2), 1.));
     return rr * rr * rr * (spectra * g.flux);
 }
 
@@ -780,6 +790,14 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
     }
     unsigned n = boff[blockIdx.x] + tscan[threadIdx.x] - cnt;
     const double inc = 2 * len / facet;
+    if (cid && blockIdx.x == 0 && threadIdx.x == 0) {
+        // zero pad cell after the last one (the arrays hold facet^2 + 1 entries): lanes past
+        // the end of a chunk read it instead of masking their data
+        const unsigned nt = boff[gridDim.x];
+        dre[nt] = 0.;
+        dim[nt] = 0.;
+        cid[nt] = 0;
+    }
     if (ctr) {  // centre of grid index k, the same expression as uu / vv below
         const unsigned k = blockIdx.x * kCompactBS + threadIdx.x;
         if (k < facet) ctr[k] = (k + 0.5) * inc - len;
@@ -1181,6 +1199,7 @@ __global__ void pack_facet_kernel(const FacetDev *dfacet, const double *uu, cons
         invs2[i] = (float)w;
         invs2_64[i] = w;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) invs2_64[n] = 0.;  // the zero pad cell
 }
 
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
@@ -1810,15 +1829,16 @@ __device__ inline void tile_sums64(const FacetDev &F, const LikItem &it, double 
             const unsigned k = in ? i : beg;
             uu[q] = gload(F.uu, k);
             vv[q] = gload(F.vv, k);
-            dr[q] = in ? gload(F.dre, k) : 0.;
-            di[q] = in ? gload(F.dim, k) : 0.;
+            const double r_ = gload(F.dre, k), m_ = gload(F.dim, k);
+            dr[q] = in ? r_ : 0.;
+            di[q] = in ? m_ : 0.;
             s2[q] = PASS == 0 ? gload(F.s2, k) : 1.;
         }
 #pragma unroll
         for (int q = 0; q < kTileU; q++) {
             const double k1 = fma(e2, vv[q], a1 * uu[q]);
             const double k2 = fma(a2, vv[q], e2 * uu[q]);
-            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
             const double M = rr * rr * rr;
             const double hr = dr[q] * M;
             const double hi = -di[q] * M;
@@ -1873,11 +1893,10 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
 #pragma unroll
         for (int q = 0; q < U; q++) {
             const unsigned i = base + q * kTileBS;
-            const bool in = i < end;
-            const unsigned k = in ? i : beg;
+            const unsigned k = i < end ? i : F.n;  // past the end: the zero pad cell
             c[q] = gload(F.cid, k);
-            dr[q] = in ? gload(F.dre, k) : 0.;
-            di[q] = in ? gload(F.dim, k) : 0.;
+            dr[q] = gload(F.dre, k);
+            di[q] = gload(F.dim, k);
             s2[q] = gload(F.invs2_64, k);  // 1 / sigma^2
         }
 #pragma unroll
@@ -1885,12 +1904,12 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
             const double uu = sC[c[q] & 0xffffu], vv = sC[c[q] >> 16];
             const double k1 = fma(e2, vv, a1 * uu);
             const double k2 = fma(a2, vv, e2 * uu);
-            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
             const double M = rr * rr * rr;
             const double hr = dr[q] * M;
             const double hi = -di[q] * M;
             const double U_ = kTwoPi * uu, V = kTwoPi * vv;
-            if (base + q * kTileBS < end) s[0] = fma(M * M, s2[q], s[0]);
+            s[0] = fma(M * M, s2[q], s[0]);
             s[1] += hr;
             s[2] = fma(-hi, U_, s[2]);
             s[3] = fma(-hi, V, s[3]);
@@ -1907,11 +1926,14 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
 // e^{i phase} is the product of the task's column entry X[column] = e^{2 pi i xo u} and row
 // entry Y[row] = e^{2 pi i yo v}, computed once per pass (lik_step_kernel) instead of a
 // sin/cos per cell.  The cell centres come from ctr, bit-identical to uu / vv.  The work-group
-// copies X and ctr to LDS first (sX, sC): per cell only the cell id and data then go
-// through the vector memory pipeline, which otherwise carried 68 bytes per cell and task.
+// copies X, ctr and (YL) Y to LDS first (sX, sC, sY): per cell only the cell id and data then
+// go through the vector memory pipeline, which otherwise carried 68 bytes per cell and task.
+// Without YL the row entry is a global load, nearly uniform across a wave.
+template <bool YL>
 __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
-                                       const double2 *X, const double2 *sX, const double *sC,
-                                       unsigned beg, unsigned end, double (&s)[7])
+                                       const double2 *Yg, const double2 *sX, const double2 *sY,
+                                       const double *sC, unsigned beg, unsigned end,
+                                       double (&s)[7])
 {
     const double e1 = it.e1, e2 = it.e2;
     const double detA = 1. - e1 * e1 - e2 * e2;
@@ -1919,26 +1941,31 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     const double sf = (scale * scale) / (detA * detA);
     const double sw = sf * (4 * kPi * kPi);
     const double a1 = 1. + e1, a2 = 1. - e1;
-    const double2 *Y = X + F.grid_facet;
     constexpr int kTabU = 2;
 #pragma unroll
     for (int j = 0; j < 7; j++) s[j] = 0.;
     // software-pipelined: the cell loads of the next batch are issued before the current
-    // batch is evaluated, so their memory latency overlaps its arithmetic
+    // batch is evaluated, so their memory latency overlaps its arithmetic.  The loads are
+    // unconditional, in the loop body's own block: lanes past the end of the chunk read the
+    // zero pad cell after the facet's last one.
     uint32_t cn[kTabU];
     double drn[kTabU], din[kTabU];
     auto fetch = [&](unsigned base) {
 #pragma unroll
         for (int q = 0; q < kTabU; q++) {
             const unsigned i = base + q * kTileBS;
-            const bool in = i < end;
-            const unsigned k = in ? i : beg;
+            const unsigned k = i < end ? i : F.n;
             cn[q] = gload(F.cid, k);
-            drn[q] = in ? gload(F.dre, k) : 0.;
-            din[q] = in ? gload(F.dim, k) : 0.;
+            drn[q] = gload(F.dre, k);
+            din[q] = gload(F.dim, k);
         }
     };
-    if (beg + threadIdx.x < end) fetch(beg + threadIdx.x);
+    fetch(beg + threadIdx.x);
+    // the first batch's values pass through an empty asm: the loop-carried values then
+    // are not loads on every path, and the compiler keeps them in registers across the
+    // iteration instead of folding them back into loads at the point of use
+#pragma unroll
+    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(cn[q]), "+v"(drn[q]), "+v"(din[q]));
     for (unsigned base = beg + threadIdx.x; base < end; base += kTabU * kTileBS) {
         uint32_t c[kTabU];
         double dr[kTabU], di[kTabU];
@@ -1948,7 +1975,7 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             dr[q] = drn[q];
             di[q] = din[q];
         }
-        if (base + kTabU * kTileBS < end) fetch(base + kTabU * kTileBS);
+        fetch(base + kTabU * kTileBS);
         double uu[kTabU], vv[kTabU];
         double2 x[kTabU], y[kTabU];
 #pragma unroll
@@ -1957,13 +1984,13 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             uu[q] = sC[ci];
             vv[q] = sC[cj];
             x[q] = sX[ci];
-            y[q] = gload(Y, cj);  // the row is nearly uniform across a wave
+            y[q] = YL ? sY[cj] : gload(Yg, cj);
         }
 #pragma unroll
         for (int q = 0; q < kTabU; q++) {
             const double k1 = fma(e2, vv[q], a1 * uu[q]);
             const double k2 = fma(a2, vv[q], e2 * uu[q]);
-            const double rr = rsqrt(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
             const double M = rr * rr * rr;
             const double hr = dr[q] * M;
             const double hi = -di[q] * M;
@@ -2108,15 +2135,27 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
     return done;
 }
 
-template <int PASS>
+template <int PASS, bool GRID>
 __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red);
+
+// Waves per SIMD the tile kernels are compiled for.  The generic kernel carries every path
+// (the FP32 sums and the per-cell sin/cos among them) in 128 VGPRs; the grid kernel only
+// the FP64 paths of extracted facets (pass 0 with the cell centres in LDS, Newton passes
+// from the phase tables), which fit a smaller register budget, so more waves stay resident
+// to cover the memory latency.
+constexpr int kTileWaves = 4;
+#ifndef RL2_TILE_GRID_WAVES
+#define RL2_TILE_GRID_WAVES 4
+#endif
+constexpr int kTileGridWaves = RL2_TILE_GRID_WAVES;
 
 // A block takes the logical work-groups q, q + nbx, q + 2 nbx, ... of its XCD's eighth of
 // the logical order (q = block / 8, nbx = blocks per XCD): with one block per logical
 // work-group that is exactly one, and a launch capped at the resident block count walks
 // the whole pass without further dispatches.
-template <int PASS>
-__global__ __launch_bounds__(kTileBS, 4) void lik_tile_kernel(TileLaunch t)
+template <int PASS, bool GRID>
+__global__ __launch_bounds__(kTileBS, GRID ? kTileGridWaves : kTileWaves) void lik_tile_kernel(
+    TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
     if (PASS == 0 && blockIdx.x == 0 && threadIdx.x < t.n_inl)
@@ -2126,11 +2165,13 @@ __global__ __launch_bounds__(kTileBS, 4) void lik_tile_kernel(TileLaunch t)
     for (unsigned j = blockIdx.x / kXcds; j < per; j += nbx) {
         const unsigned wg = (blockIdx.x % kXcds) * per + j;
         if (wg >= t.nwg) break;
-        tile_chunk<PASS>(t, wg, red);
+        tile_chunk<PASS, GRID>(t, wg, red);
     }
 }
 
-template <int PASS>
+// GRID: every item of the launch is an FP64 item on an extracted facet with phase tables
+// (the host checks this when it picks the kernel).
+template <int PASS, bool GRID>
 __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
 {
     const LikLaunch &a = t.lk;
@@ -2147,30 +2188,47 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
     const FacetDev &F = a.facets[it.slot];
     const unsigned beg = chunk * t.chunk_cells;
     const unsigned end = min(F.n, beg + t.chunk_cells);
-    const double xo = PASS == 1 ? T->xo : 0., yo = PASS == 1 ? T->yo : 0.;
     double s[7];
-    if (PASS == 1 && it.prec == 0 && t.tab && F.cid) {
-        extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * (16 + 8) bytes
+    if (PASS == 1 && (GRID || (it.prec == 0 && t.tab && F.cid))) {
+        // t.tab_lds_facet * (16 + 8) bytes, + 16 with the rows: X, ctr, Y
+        extern __shared__ double2 tab_lds[];
         const unsigned gf = F.grid_facet;
         const double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
+        const double2 *Y = X + gf;
         double2 *sX = tab_lds;
         double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
-        for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
-            sX[k] = gload(X, k);
-            sC[k] = gload(F.ctr, k);
+        double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
+        if (t.tab_y_lds) {
+            for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+                sX[k] = gload(X, k);
+                sC[k] = gload(F.ctr, k);
+                sY[k] = gload(Y, k);
+            }
+            __syncthreads();
+            tile_sums64_tab<true>(F, it, a.rg.ro[r], Y, sX, sY, sC, beg, end, s);
+        } else {
+            for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+                sX[k] = gload(X, k);
+                sC[k] = gload(F.ctr, k);
+            }
+            __syncthreads();
+            tile_sums64_tab<false>(F, it, a.rg.ro[r], Y, sX, sY, sC, beg, end, s);
         }
-        __syncthreads();
-        tile_sums64_tab(F, it, a.rg.ro[r], X, sX, sC, beg, end, s);
-    } else if (PASS == 0 && it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet) {
+    } else if (PASS == 0 &&
+               (GRID || (it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet))) {
         extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
         double *sC = reinterpret_cast<double *>(tab_lds);
         for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
         __syncthreads();
         tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
+    } else if (GRID) {
+        return;  // not reached: the host launches GRID only for grid items
     } else if (it.prec == 0)
-        tile_sums64<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
+        tile_sums64<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0., PASS == 1 ? T->yo : 0.,
+                          beg, end, s);
     else
-        tile_sums32<PASS>(F, it, a.rg.ro[r], xo, yo, beg, end, s);
+        tile_sums32<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0., PASS == 1 ? T->yo : 0.,
+                          beg, end, s);
     block_allreduce<kTileBS, 7>(s, red);
     if (threadIdx.x == 0) {
         double *P = t.partials + (size_t)wg * 8;
@@ -2267,10 +2325,19 @@ void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
     if (pass == 0) {
-        lik_tile_kernel<0><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab_lds_facet * 8, s>>>(t);
+        const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
+        if (t.grid)
+            lik_tile_kernel<0, true><<<g, kTileBS, t.tab_lds_facet * 8, s>>>(t);
+        else
+            lik_tile_kernel<0, false><<<g, kTileBS, t.tab_lds_facet * 8, s>>>(t);
         lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
     } else {
-        lik_tile_kernel<1><<<tile_grid(t.nwg, tuning().tile_blocks), kTileBS, t.tab ? t.tab_lds_facet * 24 : 0, s>>>(t);
+        const unsigned lds = t.tab ? t.tab_lds_facet * (t.tab_y_lds ? 40 : 24) : 0;
+        const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
+        if (t.grid)
+            lik_tile_kernel<1, true><<<g, kTileBS, lds, s>>>(t);
+        else
+            lik_tile_kernel<1, false><<<g, kTileBS, lds, s>>>(t);
         lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
     }
 }

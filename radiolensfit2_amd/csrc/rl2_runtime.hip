@@ -257,10 +257,11 @@ bool Obs::ensure_slots(int n)
     slots.resize(n);
     for (int i = old; i < n; i++) {
         FacetSlot &s = slots[i];
+        // the cell arrays of the grid paths hold a zero pad cell after the last cell
         if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
-              dmalloc(&s.dre, ncell) && dmalloc(&s.dim, ncell) && dmalloc(&s.s2, ncell) &&
-              dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
-              dmalloc(&s.invs2_64, ncell) && dmalloc(&s.cid, ncell) &&
+              dmalloc(&s.dre, ncell + 1) && dmalloc(&s.dim, ncell + 1) &&
+              dmalloc(&s.s2, ncell) && dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
+              dmalloc(&s.invs2_64, ncell + 1) && dmalloc(&s.cid, ncell + 1) &&
               dmalloc(&s.ctr, (size_t)fmax) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
@@ -485,12 +486,14 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // phase tables for FP64 items on grid facets: 2 * grid_facet entries per task
         size_t ntab = 0;
         unsigned tab_facet = 0;
+        int ntab_items = 0;
         for (int i = 0; i < n; i++) {
             const FacetDev &f = o.h_facets[items[i].slot];
             b.h_items[i].tab_first = (unsigned)ntab;
             if (tuning().phase_tables && items[i].prec == 0 && f.cid && f.grid_facet) {
                 ntab += (size_t)nr * 2 * f.grid_facet;
                 tab_facet = std::max(tab_facet, f.grid_facet);
+                ntab_items++;
             }
         }
         // offsets are 32-bit, and a work-group stages one task's column table (24 bytes per
@@ -516,6 +519,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             t.chunk_cells = chunk;
             t.tab = ntab ? b.d_tab : nullptr;
             t.tab_lds_facet = ntab ? tab_facet : 0;
+            t.tab_y_lds = ntab && tuning().phase_tables == 2 && tab_facet * 40u <= kTileLdsBytes;
+            // the grid kernels when every item has tables (FP64 on an extracted facet)
+            t.grid = ntab && ntab_items == n;
             t.items_w = b.d_items;
             t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
             for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
@@ -1285,7 +1291,7 @@ int rl2_tune_set(int knob, int value)
         tuning().tile_cells = (unsigned)value;
         return RL2_OK;
     case RL2_TUNE_PHASE_TABLES:
-        if (value != 0 && value != 1) break;
+        if (value < 0 || value > 2) break;
         tuning().phase_tables = value;
         return RL2_OK;
     case RL2_TUNE_FUSE_SUBTRACT:

@@ -106,12 +106,25 @@ def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
         assert abs(Le[k] - oLe) <= 1e-8, (k, Le[k], oLe)
     fit_t = dev.fit(slot=3)
     H = N.hip()
+    mode = C.c_int()
+    N.check(H.rl2_tune_get(N.RL2_TUNE_PHASE_TABLES, C.byref(mode)), "rl2_tune_get")
+    # the row tables read from LDS (2) or from global memory (1): the same table entries,
+    # so the same values
+    N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 3 - mode.value), "rl2_tune_set")
+    try:
+        Le2, err2, Lr2, nit2 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+    finally:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, mode.value), "rl2_tune_set")
+    np.testing.assert_array_equal(err2, err)
+    np.testing.assert_array_equal(nit2, nit)
+    np.testing.assert_array_equal(Lr2[:, 1:24], Lr[:, 1:24])
+    np.testing.assert_array_equal(Le2, Le)
     N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 0), "rl2_tune_set")
     try:
         Le0, err0, Lr0, nit0 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
         fit_0 = dev.fit(slot=3)
     finally:
-        N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 1), "rl2_tune_set")
+        N.check(H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, mode.value), "rl2_tune_set")
     np.testing.assert_array_equal(err0, err)
     np.testing.assert_array_equal(nit0, nit)
     np.testing.assert_allclose(Lr0[:, 1:24], Lr[:, 1:24], rtol=0, atol=1e-9)

@@ -219,7 +219,7 @@ def test_tune_knobs_validate_and_round_trip():
     v = C.c_int()
     cases = [(N.RL2_TUNE_GRID_VARIANT, [0, 1, 2, 3, 4], [-1, 5]),
              (N.RL2_TUNE_TILE_CELLS, [0, 2048, 4096, 65536], [1024, 3000, 131072, -2048]),
-             (N.RL2_TUNE_PHASE_TABLES, [0, 1], [2, -1]),
+             (N.RL2_TUNE_PHASE_TABLES, [0, 1, 2], [3, -1]),
              (N.RL2_TUNE_FUSE_SUBTRACT, [0, 1], [2, -1]),
              (N.RL2_TUNE_TILE_BLOCKS, [8, 1024, 0], [4, -8, 1001]),
              (N.RL2_TUNE_TILE_STREAMS, [2, 1], [0, 3])]
@@ -235,7 +235,7 @@ def test_tune_knobs_validate_and_round_trip():
     finally:  # defaults
         H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 3)
         H.rl2_tune_set(N.RL2_TUNE_TILE_CELLS, 0)
-        H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 1)
+        H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 2)
         H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
         H.rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, 0)
         H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 1)
