@@ -1929,6 +1929,13 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
 // copies X, ctr and (YL) Y to LDS first (sX, sC, sY): per cell only the cell id and data then
 // go through the vector memory pipeline, which otherwise carried 68 bytes per cell and task.
 // Without YL the row entry is a global load, nearly uniform across a wave.
+// One batch of cells of the table path: cell ids and data.
+constexpr int kTabU = 2;
+struct TabCells {
+    uint32_t c[kTabU];
+    double dr[kTabU], di[kTabU];
+};
+
 template <bool YL>
 __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
                                        const double2 *Yg, const double2 *sX, const double2 *sY,
@@ -1941,46 +1948,31 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     const double sf = (scale * scale) / (detA * detA);
     const double sw = sf * (4 * kPi * kPi);
     const double a1 = 1. + e1, a2 = 1. - e1;
-    constexpr int kTabU = 2;
 #pragma unroll
     for (int j = 0; j < 7; j++) s[j] = 0.;
-    // software-pipelined: the cell loads of the next batch are issued before the current
-    // batch is evaluated, so their memory latency overlaps its arithmetic.  The loads are
-    // unconditional, in the loop body's own block: lanes past the end of the chunk read the
-    // zero pad cell after the facet's last one.
-    uint32_t cn[kTabU];
-    double drn[kTabU], din[kTabU];
-    auto fetch = [&](unsigned base) {
+    // The facet fields in registers (the scheduling barriers below would otherwise make the
+    // compiler reload them from the descriptor every iteration).  Lanes past the end of the
+    // chunk read the zero pad cell after the facet's last one, so the loads are
+    // unconditional and no lane masks its data.
+    const uint32_t *cid = F.cid;
+    const double *dre = F.dre, *dim = F.dim;
+    const unsigned pad = F.n;
+    auto fetch = [&](TabCells &b, unsigned base) {
 #pragma unroll
         for (int q = 0; q < kTabU; q++) {
             const unsigned i = base + q * kTileBS;
-            const unsigned k = i < end ? i : F.n;
-            cn[q] = gload(F.cid, k);
-            drn[q] = gload(F.dre, k);
-            din[q] = gload(F.dim, k);
+            const unsigned k = i < end ? i : pad;
+            b.c[q] = gload(cid, k);
+            b.dr[q] = gload(dre, k);
+            b.di[q] = gload(dim, k);
         }
     };
-    fetch(beg + threadIdx.x);
-    // the first batch's values pass through an empty asm: the loop-carried values then
-    // are not loads on every path, and the compiler keeps them in registers across the
-    // iteration instead of folding them back into loads at the point of use
-#pragma unroll
-    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(cn[q]), "+v"(drn[q]), "+v"(din[q]));
-    for (unsigned base = beg + threadIdx.x; base < end; base += kTabU * kTileBS) {
-        uint32_t c[kTabU];
-        double dr[kTabU], di[kTabU];
-#pragma unroll
-        for (int q = 0; q < kTabU; q++) {
-            c[q] = cn[q];
-            dr[q] = drn[q];
-            di[q] = din[q];
-        }
-        fetch(base + kTabU * kTileBS);
+    auto eval = [&](const TabCells &b) {
         double uu[kTabU], vv[kTabU];
         double2 x[kTabU], y[kTabU];
 #pragma unroll
         for (int q = 0; q < kTabU; q++) {
-            const unsigned ci = c[q] & 0xffffu, cj = c[q] >> 16;
+            const unsigned ci = b.c[q] & 0xffffu, cj = b.c[q] >> 16;
             uu[q] = sC[ci];
             vv[q] = sC[cj];
             x[q] = sX[ci];
@@ -1992,8 +1984,8 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             const double k2 = fma(a2, vv[q], e2 * uu[q]);
             const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
             const double M = rr * rr * rr;
-            const double hr = dr[q] * M;
-            const double hi = -di[q] * M;
+            const double hr = b.dr[q] * M;
+            const double hi = -b.di[q] * M;
             const double U = kTwoPi * uu[q], V = kTwoPi * vv[q];
             const double cs = fma(x[q].x, y[q].x, -(x[q].y * y[q].y));
             const double sn = fma(x[q].y, y[q].x, x[q].x * y[q].y);
@@ -2007,6 +1999,27 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             s[4] = fma(-re * V, V, s[4]);
             s[5] = fma(-rU, V, s[5]);
         }
+    };
+    // Software-pipelined over two register sets: the loads of the next batch are issued
+    // before the current batch is evaluated, so their latency overlaps its arithmetic.  A
+    // set is loaded only after its previous batch has been used, so the loop needs no
+    // register copies (a copy of a just-issued load waits for it at once), and the
+    // scheduling barriers keep the loads above the arithmetic.  The first batch passes
+    // through an empty asm so that the compiler does not fold the loop-carried set back
+    // into loads at the point of use.  Cells are still added in increasing order.
+    constexpr unsigned S = kTabU * kTileBS;
+    TabCells A, B;
+    fetch(A, beg + threadIdx.x);
+#pragma unroll
+    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.c[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
+    for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
+        fetch(B, base + S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(A);
+        if (base + S >= end) break;
+        fetch(A, base + 2 * S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(B);
     }
 }
 
