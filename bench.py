@@ -244,11 +244,14 @@ def roofline(kstats, lik_peak):
     tr = measured_traffic()
     lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
     ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
-    r_lik = {"kernel": "likelihood_kernel", "bound": "valu", "achieved": round(lik_tf, 4),
+    r_lik = {"kernel": "lik_tile_kernel + lik_step_kernel (one likelihood pass)", "bound": "valu",
+             "achieved": round(lik_tf, 4),
              "peak": lik_peak, "unit": "TFLOP/s", "frac": round(lik_tf / lik_peak, 5),
              "traffic": tr.get("likelihood_launch_bytes"), "launches": lik["launches"],
-             "avg_launch_us": round(1e3 * lik["ms"] / max(1, lik["launches"]), 3)}
-    r_ext = {"kernel": "grid_kernel (extraction)", "bound": "hbm", "achieved": round(ext_gbs, 2),
+             "avg_launch_us": round(1e3 * lik["ms"] / max(1, lik["launches"]), 3),
+             "valu_busy": tr.get("likelihood_tile_valu_busy")}
+    r_ext = {"kernel": "grid_run_kernel (extraction)", "bound": "hbm",
+             "achieved": round(ext_gbs, 2),
              "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(ext_gbs / PEAK_HBM_GBS, 5),
              "traffic": tr.get("extraction_launch_bytes"), "launches": ext["launches"],
              "avg_launch_us": round(1e3 * ext["ms"] / max(1, ext["launches"]), 3)}

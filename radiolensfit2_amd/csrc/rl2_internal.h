@@ -192,10 +192,11 @@ struct TileLaunch {
     double2 *tab;
     unsigned tab_lds_facet;    // widest grid_facet of the launch's table items (LDS staging)
     unsigned tab_y_lds;        // 1: the row tables are staged in LDS too
-    unsigned grid;             // 1: every item is FP64 on an extracted facet with tables
+    unsigned grid;             // 1: every item is FP64 on an extracted facet with tables;
+                               // 2: the same with the items in the kernel arguments
     // Up to kInlItems items travel in the kernel arguments instead of a host-to-device
-    // copy: the pass-0 tile kernel reads them from here and its block 0 stores them to
-    // items_w (= lk.items) for the kernels after it.
+    // copy: the tile kernels read them from here, and block 0 of the pass-0 tile kernel
+    // stores them to items_w (= lk.items) for the step kernels and the r-marginalisation.
     unsigned n_inl;
     LikItem *items_w;
     LikItem inl[kInlItems];

@@ -1446,15 +1446,43 @@ __device__ double marginalise_posterior(int num, const double *x, double *y, Qag
 // Wave-cooperative variants: one work-group of two waves per item.  Lane j of a wave
 // evaluates the integrand at Gauss-Kronrod node j (0..19 left of the centre, 20..39 right,
 // 40 the centre) and contributes its weighted terms to the Kronrod, Gauss, |f| and
-// |f - mean| sums, which butterfly reductions complete; every lane ends with the same
-// bit-identical sums (the pairwise additions are commutative), so all lanes follow the
-// same control flow.  The sums differ from QUADPACK's sequential order only by rounding.
-// Each bisection integrates its two halves on the two waves at once.
+// |f - mean| sums, which wave_sum completes; every lane ends with the same sums (lane
+// 63's, read into scalar registers), so all lanes follow the same control flow.  The sums
+// differ from QUADPACK's sequential order only by rounding.  Each bisection integrates its
+// two halves on the two waves at once.
+
+// Sum of v over the 64 lanes of a wave, in lane 63, by data-parallel-primitive moves (VALU
+// lane shifts, no LDS round trips): shifts by 1, 2, 4 and 8 within each row of 16 lanes
+// leave each row's total in its lane 15; row_bcast:15 adds row 0's total into row 1 and
+// row 2's into row 3, row_bcast:31 adds lane 31's into rows 2 and 3.  Lanes whose source
+// lies outside the row or is masked off add 0.  A fixed order, so the sum is deterministic.
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ inline double dpp_f64(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, BANK_MASK, false);
+    const int rhi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, BANK_MASK, false);
+    return __hiloint2double(rhi, rlo);
+}
+__device__ inline double wave_sum_last(double v)
+{
+    v += dpp_f64<0x111, 0xf, 0xf>(v);  // row_shr:1
+    v += dpp_f64<0x112, 0xf, 0xf>(v);  // row_shr:2
+    v += dpp_f64<0x114, 0xf, 0xf>(v);  // row_shr:4
+    v += dpp_f64<0x118, 0xf, 0xf>(v);  // row_shr:8
+    v += dpp_f64<0x142, 0xa, 0xf>(v);  // row_bcast:15 into rows 1 and 3
+    v += dpp_f64<0x143, 0xc, 0xf>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
+// The wave's sum in every lane: wave_sum_last, then lane 63's value read into scalar
+// registers (every lane gets the same value, so the wave takes the same branches after it).
 __device__ inline double wave_sum(double v)
 {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    v = wave_sum_last(v);
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
 }
 
 __device__ inline void qk41_wave(const Spline &sp, double a, double b, double *result,
@@ -1938,9 +1966,9 @@ struct TabCells {
 
 template <bool YL>
 __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
-                                       const double2 *Yg, const double2 *sX, const double2 *sY,
-                                       const double *sC, unsigned beg, unsigned end,
-                                       double (&s)[7])
+                                       const double2 *X, const double2 *Yg, unsigned gf,
+                                       double2 *sX, double2 *sY, double *sC, unsigned beg,
+                                       unsigned end, double (&s)[7])
 {
     const double e1 = it.e1, e2 = it.e2;
     const double detA = 1. - e1 * e1 - e2 * e2;
@@ -2009,7 +2037,16 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     // into loads at the point of use.  Cells are still added in increasing order.
     constexpr unsigned S = kTabU * kTileBS;
     TabCells A, B;
+    // the first cells are requested before the tables are staged in LDS, so the two
+    // memory round trips at the start of the work-group overlap
     fetch(A, beg + threadIdx.x);
+    const double *ctr = F.ctr;
+    for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+        sX[k] = gload(X, k);
+        sC[k] = gload(ctr, k);
+        if (YL) sY[k] = gload(Yg, k);
+    }
+    __syncthreads();
 #pragma unroll
     for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.c[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
     for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
@@ -2148,7 +2185,29 @@ __device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
     return done;
 }
 
-template <int PASS, bool GRID>
+// The chunk sums of a tile work-group into P[0..N-1]: each wave's total from lane 63, then
+// thread j adds the waves' value j in wave order.
+template <int N>
+__device__ inline void tile_reduce(double (&v)[N], double *lds, double *P)
+{
+    constexpr int NW = kTileBS / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = wave_sum_last(v[j]);
+    if (lane == 63)
+#pragma unroll
+        for (int j = 0; j < N; j++) lds[wave * N + j] = v[j];
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double t = 0.;
+#pragma unroll
+        for (int q = 0; q < NW; q++) t += lds[q * N + threadIdx.x];
+        P[threadIdx.x] = t;
+    }
+    __syncthreads();  // lds is reused by the block's next logical work-group
+}
+
+template <int PASS, int KIND>
 __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red);
 
 // Waves per SIMD the tile kernels are compiled for.  The generic kernel carries every path
@@ -2166,8 +2225,8 @@ constexpr int kTileGridWaves = RL2_TILE_GRID_WAVES;
 // the logical order (q = block / 8, nbx = blocks per XCD): with one block per logical
 // work-group that is exactly one, and a launch capped at the resident block count walks
 // the whole pass without further dispatches.
-template <int PASS, bool GRID>
-__global__ __launch_bounds__(kTileBS, GRID ? kTileGridWaves : kTileWaves) void lik_tile_kernel(
+template <int PASS, int KIND>
+__global__ __launch_bounds__(kTileBS, KIND ? kTileGridWaves : kTileWaves) void lik_tile_kernel(
     TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
@@ -2178,17 +2237,23 @@ __global__ __launch_bounds__(kTileBS, GRID ? kTileGridWaves : kTileWaves) void l
     for (unsigned j = blockIdx.x / kXcds; j < per; j += nbx) {
         const unsigned wg = (blockIdx.x % kXcds) * per + j;
         if (wg >= t.nwg) break;
-        tile_chunk<PASS, GRID>(t, wg, red);
+        tile_chunk<PASS, KIND>(t, wg, red);
     }
 }
 
-// GRID: every item of the launch is an FP64 item on an extracted facet with phase tables
-// (the host checks this when it picks the kernel).
-template <int PASS, bool GRID>
+// KIND 0: any items (the generic kernel).  KIND 1 and 2 (the grid kernels): every item of
+// the launch is an FP64 item on an extracted facet with phase tables, which the host checks
+// when it picks the kernel; KIND 2 when the items travel in the kernel arguments.
+template <int PASS, int KIND>
 __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
 {
+    constexpr bool GRID = KIND != 0;
     const LikLaunch &a = t.lk;
-    const LikItem *items = (PASS == 0 && t.n_inl) ? t.inl : a.items;
+    // KIND 2: the items from the kernel arguments (scalar loads that stay in the scalar
+    // cache instead of a chain of dependent vector loads for the search).  Pass 0 reads them
+    // there whenever they travel there: its block 0 is what stores them to a.items.
+    const LikItem *items =
+        KIND == 2 ? t.inl : ((PASS == 0 && t.n_inl) ? t.inl : a.items);
     const int item = tile_item(items, a.nitems, wg, t.nr_g);
     unsigned first, nch;
     tile_range(t, items, item, &first, &nch);
@@ -2198,60 +2263,69 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
     const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;  // finished task: the whole work-group moves on
     const LikItem it = items[item];
-    const FacetDev &F = a.facets[it.slot];
     const unsigned beg = chunk * t.chunk_cells;
-    const unsigned end = min(F.n, beg + t.chunk_cells);
     double s[7];
-    if (PASS == 1 && (GRID || (it.prec == 0 && t.tab && F.cid))) {
-        // t.tab_lds_facet * (16 + 8) bytes, + 16 with the rows: X, ctr, Y
-        extern __shared__ double2 tab_lds[];
-        const unsigned gf = F.grid_facet;
-        const double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
-        const double2 *Y = X + gf;
-        double2 *sX = tab_lds;
-        double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
-        double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
-        if (t.tab_y_lds) {
-            for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
-                sX[k] = gload(X, k);
-                sC[k] = gload(F.ctr, k);
-                sY[k] = gload(Y, k);
+    if constexpr (GRID) {
+        auto grid_sums = [&](const FacetDev &F) {
+            const unsigned end = min(F.n, beg + t.chunk_cells);
+            extern __shared__ double2 tab_lds[];
+            if (PASS == 1) {
+                // t.tab_lds_facet * (16 + 8) bytes, + 16 with the rows: X, ctr, Y
+                const unsigned gf = F.grid_facet;
+                const double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
+                const double2 *Y = X + gf;
+                double2 *sX = tab_lds;
+                double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
+                double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
+                if (t.tab_y_lds)
+                    tile_sums64_tab<true>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+                else
+                    tile_sums64_tab<false>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end,
+                                           s);
+            } else {
+                double *sC = reinterpret_cast<double *>(tab_lds);  // tab_lds_facet * 8 bytes
+                for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS)
+                    sC[k] = gload(F.ctr, k);
+                __syncthreads();
+                tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
             }
-            __syncthreads();
-            tile_sums64_tab<true>(F, it, a.rg.ro[r], Y, sX, sY, sC, beg, end, s);
-        } else {
-            for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
-                sX[k] = gload(X, k);
+        };
+        grid_sums(a.facets[it.slot]);
+    } else {
+        const FacetDev &F = a.facets[it.slot];
+        const unsigned end = min(F.n, beg + t.chunk_cells);
+        if (PASS == 1 && it.prec == 0 && t.tab && F.cid) {
+            extern __shared__ double2 tab_lds[];
+            const unsigned gf = F.grid_facet;
+            const double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
+            const double2 *Y = X + gf;
+            double2 *sX = tab_lds;
+            double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
+            double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
+            if (t.tab_y_lds)
+                tile_sums64_tab<true>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+            else
+                tile_sums64_tab<false>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+        } else if (PASS == 0 && it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet) {
+            extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
+            double *sC = reinterpret_cast<double *>(tab_lds);
+            for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS)
                 sC[k] = gload(F.ctr, k);
-            }
             __syncthreads();
-            tile_sums64_tab<false>(F, it, a.rg.ro[r], Y, sX, sY, sC, beg, end, s);
-        }
-    } else if (PASS == 0 &&
-               (GRID || (it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet))) {
-        extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
-        double *sC = reinterpret_cast<double *>(tab_lds);
-        for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
-        __syncthreads();
-        tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
-    } else if (GRID) {
-        return;  // not reached: the host launches GRID only for grid items
-    } else if (it.prec == 0)
-        tile_sums64<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0., PASS == 1 ? T->yo : 0.,
-                          beg, end, s);
-    else
-        tile_sums32<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0., PASS == 1 ? T->yo : 0.,
-                          beg, end, s);
-    block_allreduce<kTileBS, 7>(s, red);
-    if (threadIdx.x == 0) {
-        double *P = t.partials + (size_t)wg * 8;
-#pragma unroll
-        for (int j = 0; j < 7; j++) P[j] = s[j];
+            tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
+        } else if (it.prec == 0)
+            tile_sums64<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0.,
+                              PASS == 1 ? T->yo : 0., beg, end, s);
+        else
+            tile_sums32<PASS>(F, it, a.rg.ro[r], PASS == 1 ? T->xo : 0.,
+                              PASS == 1 ? T->yo : 0., beg, end, s);
     }
+    tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
 }
 
 // One work-group per (item, r) task.  Wave 0 adds the task's chunk sums in a fixed order
-// (lanes over chunks, then a butterfly) and takes the Newton step or ends the task; when
+// (lanes over chunks, then across lanes into lane 63), and lane 63 takes the Newton step or
+// ends the task; when
 // the task goes on over a grid facet in FP64, the whole group (16 waves, so each lane
 // writes at most 3 entries of the 2 * facet) then writes its phase tables for the next pass.
 constexpr int kStepBS = 1024;
@@ -2268,7 +2342,7 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         int go = 0;
-        if (PASS == 0 || T->active) {  // read by the whole wave before lane 0 updates it
+        if (PASS == 0 || T->active) {  // read by the whole wave before lane 63 updates it
             unsigned first, nch;
             tile_range(t, a.items, item, &first, &nch);
             const double *P0 = t.partials + (size_t)(first + (unsigned)(r - t.r_lo)) * 8;
@@ -2277,15 +2351,14 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
 #pragma unroll
                 for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * nr * 8 + j];
 #pragma unroll
-            for (int j = 0; j < 7; j++)
-                for (int off = 32; off >= 1; off >>= 1) v[j] += __shfl_xor(v[j], off);
-            if (lane == 0) {
+            for (int j = 0; j < 7; j++) v[j] = wave_sum_last(v[j]);  // totals in lane 63
+            if (lane == 63) {
                 go = !tile_step<PASS>(a, T, item, r, v);
                 s_off[0] = T->xo;
                 s_off[1] = T->yo;
             }
         }
-        if (lane == 0) s_go = go;
+        if (lane == 63) s_go = go;
     }
     __syncthreads();
     if (!s_go || !t.tab) return;
@@ -2339,18 +2412,23 @@ void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
     const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
     if (pass == 0) {
         const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
-        if (t.grid)
-            lik_tile_kernel<0, true><<<g, kTileBS, t.tab_lds_facet * 8, s>>>(t);
+        const unsigned lds = t.tab_lds_facet * 8;
+        if (t.grid == 2)
+            lik_tile_kernel<0, 2><<<g, kTileBS, lds, s>>>(t);
+        else if (t.grid == 1)
+            lik_tile_kernel<0, 1><<<g, kTileBS, lds, s>>>(t);
         else
-            lik_tile_kernel<0, false><<<g, kTileBS, t.tab_lds_facet * 8, s>>>(t);
+            lik_tile_kernel<0, 0><<<g, kTileBS, lds, s>>>(t);
         lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
     } else {
         const unsigned lds = t.tab ? t.tab_lds_facet * (t.tab_y_lds ? 40 : 24) : 0;
         const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
-        if (t.grid)
-            lik_tile_kernel<1, true><<<g, kTileBS, lds, s>>>(t);
+        if (t.grid == 2)
+            lik_tile_kernel<1, 2><<<g, kTileBS, lds, s>>>(t);
+        else if (t.grid == 1)
+            lik_tile_kernel<1, 1><<<g, kTileBS, lds, s>>>(t);
         else
-            lik_tile_kernel<1, false><<<g, kTileBS, lds, s>>>(t);
+            lik_tile_kernel<1, 0><<<g, kTileBS, lds, s>>>(t);
         lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
     }
 }

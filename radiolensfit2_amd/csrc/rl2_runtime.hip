@@ -520,8 +520,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             t.tab = ntab ? b.d_tab : nullptr;
             t.tab_lds_facet = ntab ? tab_facet : 0;
             t.tab_y_lds = ntab && tuning().phase_tables == 2 && tab_facet * 40u <= kTileLdsBytes;
-            // the grid kernels when every item has tables (FP64 on an extracted facet)
-            t.grid = ntab && ntab_items == n;
+            // the grid kernels when every item has tables (FP64 on an extracted facet); 2:
+            // with the items in the kernel arguments
+            t.grid = (ntab && ntab_items == n) ? (n <= kInlItems ? 2u : 1u) : 0u;
             t.items_w = b.d_items;
             t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
             for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
