@@ -2339,6 +2339,13 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     const unsigned k = blockIdx.x;
     const int item = (int)(k / nr), r = (int)(k % nr) + t.r_lo;
     TileTask *T = t.tasks + (size_t)item * kMaxR + r;
+    // the item and its facet fields are requested by every wave at once, so their latency
+    // overlaps the reduction instead of following the barrier
+    const LikItem it = a.items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const uint32_t *fcid = F.cid;
+    const double *fctr = F.ctr;
+    const unsigned gf = F.grid_facet;
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         int go = 0;
@@ -2362,16 +2369,13 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     }
     __syncthreads();
     if (!s_go || !t.tab) return;
-    const LikItem &it = a.items[item];
-    const FacetDev &F = a.facets[it.slot];
-    if (it.prec != 0 || !F.cid) return;
-    const unsigned gf = F.grid_facet;
+    if (it.prec != 0 || !fcid) return;
     double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
     const double xo = s_off[0], yo = s_off[1];
     for (unsigned q = threadIdx.x; q < 2 * gf; q += blockDim.x) {
         const bool col = q < gf;
         double sn, cs;
-        sincos_rev64((col ? xo : yo) * F.ctr[col ? q : q - gf], sn, cs);
+        sincos_rev64((col ? xo : yo) * fctr[col ? q : q - gf], sn, cs);
         X[q] = make_double2(cs, sn);
     }
 }
