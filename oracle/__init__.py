@@ -1,1 +1,2 @@
-"""FP64 CPU oracle (test infrastructure): the checker for the HIP path, never a fallback."""
+"""FP64 CPU oracle (test infrastructure): the reference the HIP path is compared against,
+never a fallback."""

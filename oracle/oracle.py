@@ -1,8 +1,8 @@
 """ctypes bindings for the FP64 CPU oracle (liborl2.so) and the reference gridding build.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg, where it is the checker.  The product (radiolensfit2_amd) never imports
-this module.
+cpu_baseline leg, which compare the HIP path against it.  The product (radiolensfit2_amd)
+never imports this module.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * rl2_oracle.h -- FP64 CPU oracle for the per-galaxy shape-measurement path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product path
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference that the HIP product path
  * (radiolensfit2_amd/, librl2hip.so) is compared against.  Only tests/, the smoke()
  * entry point and bench.py's cpu_baseline leg may load it.  Nothing in the product
  * links, calls or falls back to it.
