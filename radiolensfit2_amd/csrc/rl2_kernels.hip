@@ -2014,18 +2014,19 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
             const double M = rr * rr * rr;
             const double hr = b.dr[q] * M;
             const double hi = -b.di[q] * M;
-            const double U = kTwoPi * uu[q], V = kTwoPi * vv[q];
             const double cs = fma(x[q].x, y[q].x, -(x[q].y * y[q].y));
             const double sn = fma(x[q].y, y[q].x, x[q].x * y[q].y);
             const double re = fma(hr, cs, -hi * sn);
             const double im = fma(hr, sn, hi * cs);
+            // the gradient and Hessian sums over u, v; their 2 pi and 4 pi^2 factors are
+            // applied once to the thread's totals instead of to every cell's U = 2 pi u
             s[0] += re;
-            s[1] = fma(-im, U, s[1]);
-            s[2] = fma(-im, V, s[2]);
-            const double rU = re * U;
-            s[3] = fma(-rU, U, s[3]);
-            s[4] = fma(-re * V, V, s[4]);
-            s[5] = fma(-rU, V, s[5]);
+            s[1] = fma(-im, uu[q], s[1]);
+            s[2] = fma(-im, vv[q], s[2]);
+            const double rU = re * uu[q];
+            s[3] = fma(-rU, uu[q], s[3]);
+            s[4] = fma(-re * vv[q], vv[q], s[4]);
+            s[5] = fma(-rU, vv[q], s[5]);
         }
     };
     // Software-pipelined over two register sets: the loads of the next batch are issued
@@ -2058,6 +2059,12 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
         __builtin_amdgcn_sched_barrier(0);
         eval(B);
     }
+    s[1] *= kTwoPi;
+    s[2] *= kTwoPi;
+    constexpr double kTwoPi2 = kTwoPi * kTwoPi;
+    s[3] *= kTwoPi2;
+    s[4] *= kTwoPi2;
+    s[5] *= kTwoPi2;
 }
 
 // The same sums from the FP32 facet copy: model and terms in FP32, the per-thread FP32
