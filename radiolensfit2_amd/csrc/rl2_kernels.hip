@@ -1936,17 +1936,23 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
             const double M = rr * rr * rr;
             const double hr = dr[q] * M;
             const double hi = -di[q] * M;
-            const double U_ = kTwoPi * uu, V = kTwoPi * vv;
+            // sums over u, v: the 2 pi and 4 pi^2 factors are applied to the totals below
             s[0] = fma(M * M, s2[q], s[0]);
             s[1] += hr;
-            s[2] = fma(-hi, U_, s[2]);
-            s[3] = fma(-hi, V, s[3]);
-            const double hU = hr * U_;
-            s[4] = fma(-hU, U_, s[4]);
-            s[5] = fma(-hr * V, V, s[5]);
-            s[6] = fma(-hU, V, s[6]);
+            s[2] = fma(-hi, uu, s[2]);
+            s[3] = fma(-hi, vv, s[3]);
+            const double hU = hr * uu;
+            s[4] = fma(-hU, uu, s[4]);
+            s[5] = fma(-hr * vv, vv, s[5]);
+            s[6] = fma(-hU, vv, s[6]);
         }
     }
+    s[2] *= kTwoPi;
+    s[3] *= kTwoPi;
+    constexpr double kTwoPi2 = kTwoPi * kTwoPi;
+    s[4] *= kTwoPi2;
+    s[5] *= kTwoPi2;
+    s[6] *= kTwoPi2;
 }
 
 // PASS 1 sums of tile_sums64 on a facet with grid structure.  The Newton phase is
