@@ -105,21 +105,34 @@ def run_path(dev, cat, n_gal, warmup_gal, opts, timed=True):
     return res, st, elapsed, kstats
 
 
-def cpu_baseline(obs, cat, resid, n):
-    """FP64 oracle on galaxies [0, n) from the same initial residual."""
+def cpu_baseline(obs, cat, resid, n, batch=1):
+    """FP64 oracle: the whole loop (extraction, fit, subtraction, bad-list re-fit) over the
+    catalog's first n galaxies from the same initial residual, `batch` galaxies per residual
+    update (data_processing-mpi.c semantics for batch > 1)."""
     from oracle import oracle as O
     L = O.lib()
     sub = subset(cat, 0, n)
     vs = np.zeros_like(resid)
+    vd = resid.copy()
     res = (O.GalResult * n)()
     st = O.RunStats()
     t0 = time.time()
-    L.orl2_data_processing(n, 1, obs.len, obs.nch, O.ptr(obs.wn), O.ptr(obs.spec), obs.nrow,
+    L.orl2_data_processing(n, batch, obs.len, obs.nch, O.ptr(obs.wn), O.ptr(obs.spec), obs.nrow,
                            O.ptr(obs.u), O.ptr(obs.v), O.ptr(obs.w),
-                           O.ptr(obs.sigma2, O.c_float_p), None, O.cptr(resid), O.cptr(vs),
+                           O.ptr(obs.sigma2, O.c_float_p), None, O.cptr(vd), O.cptr(vs),
                            O.ptr(sub.l), O.ptr(sub.m), O.ptr(sub.flux), O.ptr(sub.scale), res, 1,
                            C.byref(st))
     return time.time() - t0, [(r.e1, r.e2, r.error) for r in res]
+
+
+def gate(res, ores):
+    """Galaxies whose ellipticity is within 1e-4 of the FP64 oracle's, error flag equal."""
+    n = len(ores)
+    de = np.array([max(abs(res[k].e1 - ores[k][0]), abs(res[k].e2 - ores[k][1]))
+                   for k in range(n)])
+    same = np.array([res[k].error == ores[k][2] for k in range(n)])
+    ok = (de <= 1e-4) & same
+    return ok, de, same
 
 
 def same_facet_acceptance(dev64, cat, n, opts):
@@ -204,6 +217,72 @@ def measured_traffic():
         return {}
 
 
+def run_compare(ngal=100, batch=16, log_prefix="[compare]"):
+    """The metric's comparison on the 64-dish array (the array the proposal network was trained
+    on): 480 times x 2016 baselines x 4 channels of 60 MHz, a 100-galaxy catalog (seed 1, shear
+    (0.02, -0.01)) simulated on the device.  Every path runs the whole loop with the bad-list
+    re-fit from the same residual; each is gated against the FP64 oracle run with the same
+    batch semantics (batch 1: data_processing.c; batch B: data_processing-mpi.c:65-216), and
+    galaxies/s inside the gate (|de| <= 1e-4, equal error flag) is reported beside the raw
+    rate."""
+    obs, cat = make_geometry_and_catalog(synth.ARRAY_64, ngal, seed=1, shear=(0.02, -0.01))
+    dev = make_device_observation(obs, cat, N.RL2_PREC_FP64)
+    resid0 = dev.residual()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    oracle = {}
+    for b in (1, batch):
+        tc, ores = cpu_baseline(obs, cat, resid0, ngal, batch=b)
+        oracle[b] = (tc, ores)
+        log(f"{log_prefix} oracle batch {b}: {ngal / tc:.2f} gal/s ({cores} threads)")
+    net = None
+    try:
+        from radiolensfit2_amd import proposal as PR
+        net = PR.Proposal(0)
+        net.load(os.path.join(ROOT, "radiolensfit2_amd", "proposal_c1.npy"))
+    except OSError as e:
+        log(f"{log_prefix} no proposal weights: {e}")
+        net = None
+    base = dict(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0, lik_precision=0)
+    paths = [("plain", dict(base)),
+             (f"batched{batch}", dict(base, batch=batch)),
+             (f"batched{batch}_mixed", dict(base, batch=batch, lik_precision=4))]
+    if net is not None:
+        paths += [(f"batched{batch}_learned", dict(base, batch=batch, proposal=net.h)),
+                  (f"batched{batch}_learned_mixed",
+                   dict(base, batch=batch, proposal=net.h, lik_precision=4))]
+    out = {"workload": f"64 dishes, {obs.nrow} rows x {obs.nch} ch ({obs.nvis / 1e6:.2f}M vis), "
+                       f"{ngal} galaxies, whole loop with re-fit, device-simulated data",
+           "oracle": {f"batch{b}_galaxies_per_s": round(ngal / oracle[b][0], 4) for b in oracle},
+           "oracle_threads": cores, "paths": {}}
+    plain_rate = None
+    for name, opts in paths:
+        dev.restore()
+        dev.process(subset(cat, 0, min(ngal, 2 * opts["batch"])), **opts)  # warm-up
+        dev.restore()
+        N.check(N.hip().rl2_synchronize(), "sync")
+        t0 = time.perf_counter()
+        res, st = dev.process(cat, **opts)
+        N.check(N.hip().rl2_synchronize(), "sync")
+        dt = time.perf_counter() - t0
+        ok, de, same = gate(res, oracle[opts["batch"]][1])
+        rate = ngal / dt
+        plain_rate = plain_rate or rate
+        out["paths"][name] = {
+            "galaxies_per_s": round(rate, 2), "speedup_vs_plain": round(rate / plain_rate, 3),
+            "within_1e-4_of_oracle": round(float(ok.mean()), 4),
+            "galaxies_per_s_within_gate": round(rate * float(ok.mean()), 2),
+            "gated_speedup_vs_plain": round(rate * float(ok.mean()) / plain_rate, 3),
+            "max_abs_de": float(de.max()), "median_abs_de": float(np.median(de)),
+            "error_flags_equal": int(same.sum()), "oracle_batch": opts["batch"],
+            "bad_fits": [st.n_bad_first, st.n_bad_final]}
+        log(f"{log_prefix} {name}: {rate:.1f} gal/s, within 1e-4 of the oracle "
+            f"{ok.mean():.3f} (max |de| {de.max():.2e})")
+    if net is not None:
+        net.close()
+    dev.close()
+    return out
+
+
 def run_config1(ngal=100):
     """The reference's CPU-runnable case: 64 dishes, 480 times x 2016 baselines x 4 channels
     of 60 MHz, 100 galaxies (seed 1, shear (0.02, -0.01)), the whole loop on the GPU (FP64)
@@ -272,11 +351,13 @@ def main():
                          "of near-threshold samples")
     ap.add_argument("--config", choices=["197dish", "64dish"], default="197dish")
     ap.add_argument("--ngal", type=int, default=1000)
-    ap.add_argument("--stride", type=int, default=10,
-                    help="timed galaxies are every stride-th of the flux-ordered catalog, so "
-                         "the sample spans the whole flux and size range")
-    ap.add_argument("--cpu-sample", type=int, default=1,
-                    help="galaxies re-run on the FP64 CPU oracle (0: skip)")
+    ap.add_argument("--stride", type=int, default=0,
+                    help="timed galaxies are every stride-th of the flux-ordered catalog "
+                         "(default: ngal // (steps * batch), so the sample spans the whole "
+                         "flux and size range)")
+    ap.add_argument("--cpu-sample", type=int, default=5,
+                    help="galaxies, spread over the timed set, run as one loop on the FP64 "
+                         "CPU oracle and on the GPU from the same residual (0: skip)")
     ap.add_argument("--compare-fp64", action="store_true",
                     help="re-run the timed galaxies with the FP64 GPU path (untimed)")
     ap.add_argument("--no-speculate", action="store_true")
@@ -299,8 +380,14 @@ def main():
     ap.add_argument("--fuse-subtract", type=int, default=1, choices=[0, 1, 2],
                     help="make each subtraction in the same residual pass as the next "
                          "extraction (1) or in a pass of its own (0)")
-    ap.add_argument("--no-variants", dest="variants", action="store_false",
-                    help="skip the batched, mixed-precision and learned-proposal variants")
+    ap.add_argument("--variants", action="store_true",
+                    help="also time the batched, mixed-precision and learned-proposal variants "
+                         "on the same galaxies")
+    ap.add_argument("--compare", action="store_true",
+                    help="only run the metric's comparison on the 64-dish array (plain vs "
+                         "batched / mixed / learned, each gated against the FP64 oracle) and "
+                         "print its JSON line")
+    ap.add_argument("--compare-batch", type=int, default=16)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -321,10 +408,16 @@ def main():
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes}), flush=True)
         return
 
+    if args.compare:
+        print(json.dumps({"comparison": run_compare(batch=args.compare_batch)}), flush=True)
+        return
+
     cfg = synth.ARRAY_197 if args.config == "197dish" else synth.ARRAY_64
     res_prec = N.RL2_PREC_FP64 if args.residual == "c128" else N.RL2_PREC_MIXED
     B = args.batch
     n_gal = args.steps * B
+    if args.stride <= 0:
+        args.stride = max(1, args.ngal // max(n_gal, args.warmup * B))
     if max(n_gal, args.warmup * B) * args.stride > args.ngal:
         sys.exit(f"steps*batch*stride = {n_gal * args.stride} galaxies, catalog has {args.ngal}")
     obs, full_cat = make_geometry_and_catalog(cfg, args.ngal)
@@ -397,16 +490,27 @@ def main():
 
     cpu = parity = None
     if args.cpu_sample > 0:
+        # galaxies spread evenly over the timed set (and so over the flux range), run as one
+        # loop on the oracle and on the GPU from the same initial residual
         ncpu = min(args.cpu_sample, n_gal)
-        dt, ores = cpu_baseline(obs, cat, resid0, ncpu)
+        idx = np.unique(np.linspace(0, n_gal - 1, ncpu).round().astype(int))
+        ncpu = len(idx)
+        scat = synth.Catalog(*(np.ascontiguousarray(getattr(cat, f)[idx]) for f in
+                               ("flux", "scale", "e1", "e2", "l", "m", "snr")))
+        dt, ores = cpu_baseline(obs, scat, resid0, ncpu, batch=B)
+        dev.restore()
+        gres, _ = dev.process(scat, **opts)
+        ok, de, same = gate(gres, ores)
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         cpu = {"value": round(ncpu / dt, 5), "unit": "galaxies/s", "cores": cores, "kind": "port",
-               "sample": f"first {ncpu} galaxies of the timed set (extraction, fit, subtraction, "
-                         f"bad-list re-fit) on the FP64 oracle, {dt:.1f}s"}
-        de = [max(abs(res[k].e1 - ores[k][0]), abs(res[k].e2 - ores[k][1])) for k in range(ncpu)
-              if res[k].error == ores[k][2] == 0]
-        parity = {"galaxies": ncpu, "max_abs_de_vs_fp64_oracle": (max(de) if de else None),
-                  "error_flags_equal": all(res[k].error == ores[k][2] for k in range(ncpu))}
+               "sample": f"{ncpu} galaxies spread over the timed set (positions "
+                         f"{idx.tolist()}, fluxes {scat.flux.min():.1f}-{scat.flux.max():.1f} "
+                         f"uJy) as one loop (extraction, fit, subtraction, bad-list re-fit) on "
+                         f"the FP64 oracle, {dt:.1f}s"}
+        parity = {"galaxies": ncpu, "within_1e-4_of_fp64_oracle": int(ok.sum()),
+                  "max_abs_de_vs_fp64_oracle": float(de.max()),
+                  "error_flags_equal": bool(same.all()),
+                  "reference": "the same sub-loop on the GPU from the same residual"}
         log(f"[bench] cpu baseline {cpu['value']} gal/s; parity {parity}")
 
     out = {

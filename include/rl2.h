@@ -201,6 +201,9 @@ int rl2_tune_get(int knob, int *value);
  * batched evaluation rounds. */
 int rl2_fit_with_callback(double (*loglik)(double e1, double e2, int *error, void *ctx),
                           void *ctx, int nm_speculate, int line_chunk, rl2_gal_result *res);
+/* Philox4x32-10 of n counters (4 words each) under one key (2 words), as the noise kernel
+ * evaluates it. */
+void rl2_test_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 /* sin(2 pi t), cos(2 pi t) as the FP64 kernels evaluate them. */
 void rl2_test_sincos_rev64(int n, const double *t, double *s, double *c);
 /* Cell indices by the kernels' reciprocal-product form and by the reference's division;

@@ -125,6 +125,7 @@ _HIP_SIGS = {
     "rl2_synchronize": (C.c_int, []),
     "rl2_tune_set": (C.c_int, [C.c_int, C.c_int]),
     "rl2_test_sincos_rev64": (None, [C.c_int, c_double_p, c_double_p, c_double_p]),
+    "rl2_test_philox4x32_10": (None, [C.c_int, c_uint_p, c_uint_p, c_uint_p]),
     "rl2_test_cell_index": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double, C.c_double,
                                       C.c_double, C.c_uint, c_uint_p, c_uint_p]),
     "rl2_tune_get": (C.c_int, [C.c_int, c_int_p]),

@@ -723,6 +723,12 @@ void rl2_test_sincos_rev64(int n, const double *t, double *s, double *c)
     for (int i = 0; i < n; i++) rl2::sincos_rev64(t[i], s[i], c[i]);
 }
 
+// The simulated noise's generator as the kernel evaluates it (n counters, one key).
+void rl2_test_philox4x32_10(int n, const uint32_t *ctr, const uint32_t *key, uint32_t *out)
+{
+    for (int i = 0; i < n; i++) rl2::philox4x32_10(ctr + 4 * i, key, out + 4 * i);
+}
+
 // Cell of each (u, v) by the kernels' reciprocal-product form and by the reference's
 // divisions; returns the number of disagreements.
 int rl2_test_cell_index(int n, const double *u, const double *v, double inv_lambda, double len,

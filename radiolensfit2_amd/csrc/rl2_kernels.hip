@@ -155,27 +155,17 @@ void launch_add_sources(hipStream_t s, int prec, unsigned nch, unsigned long nro
                                                     sign, res);
 }
 
-// Philox4x32-10 counter-based generator (Salmon et al., SC'11) -> Box-Muller.
-__device__ inline uint4 philox4x32(uint4 ctr, uint2 key)
-{
-    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
-    for (int r = 0; r < 10; r++) {
-        uint32_t hi0 = __umulhi(M0, ctr.x), lo0 = M0 * ctr.x;
-        uint32_t hi1 = __umulhi(M1, ctr.z), lo1 = M1 * ctr.z;
-        ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
-        key.x += W0;
-        key.y += W1;
-    }
-    return ctr;
-}
-
 template <int PREC>
 __global__ void add_noise_kernel(unsigned long nvis, double sigma, uint64_t seed, void *res)
 {
     unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
     if (i >= nvis) return;
-    uint4 r = philox4x32(make_uint4((uint32_t)i, (uint32_t)(i >> 32), 0x5eed, 0),
-                         make_uint2((uint32_t)seed, (uint32_t)(seed >> 32)));
+    // Philox4x32-10 (rl2_math.h) of counter (i, 0x5eed, 0) under the seed -> Box-Muller
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0x5eedu, 0u};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t rr[4];
+    philox4x32_10(ctr, key, rr);
+    const uint4 r = make_uint4(rr[0], rr[1], rr[2], rr[3]);
     // 53-bit uniforms in (0, 1]
     double u1 = ((double)(((uint64_t)r.x << 21) ^ (r.y >> 11)) + 1.0) * (1.0 / 9007199254740992.0);
     double u2 = ((double)(((uint64_t)r.z << 21) ^ (r.w >> 11)) + 0.5) * (1.0 / 9007199254740992.0);

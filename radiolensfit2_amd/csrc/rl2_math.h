@@ -130,6 +130,30 @@ RL2_HD bool cell_index_fast(double u, double v, double inv_lambda, double len, d
     return pc < (unsigned long long)facet * facet;
 }
 
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror and Shaw, SC'11): ten rounds of
+// two 32x32 -> 64-bit products with the Weyl key schedule.  The simulated noise draws its
+// uniforms from it, one counter per visibility.
+RL2_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
+RL2_HD void philox4x32_10(const uint32_t in[4], const uint32_t key_in[2], uint32_t out[4])
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+    uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3], k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = mulhi32(M0, c0), lo0 = M0 * c0;
+        const uint32_t hi1 = mulhi32(M1, c2), lo1 = M1 * c2;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
 // Exponential integral Ei(x), x > 0: series gamma + ln x + sum x^k/(k k!) for x <= 40,
 // asymptotic e^x/x sum k!/x^k above (replaces gsl_sf_expint_Ei_e, likelihood.c:409).
 RL2_HD double expint_ei(double x)

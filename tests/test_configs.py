@@ -1,0 +1,270 @@
+"""The HIP path at the benchmark configurations, against the FP64 oracle.
+
+C1, the reference's CPU-runnable case: 64 dishes (seed 1, log-uniform radius 20 m - 10 km),
+480 times x 2016 baselines x 4 channels of 60 MHz from 950 MHz (3.87M visibilities),
+100 catalog galaxies (seed 1, shear (0.02, -0.01)), simulated with noise on the host; the
+oracle's sky model; the whole loop (extraction, fit, subtraction, bad-list re-fit) on the
+GPU and on the oracle from the same arrays, every galaxy compared.
+
+C2, the full array shape: 197 dishes (dense 1 km core, log-radial to 150 km), 19,306
+baselines x 480 times x 16 channels of 15 MHz (148.3M visibilities), 1,000 catalog galaxies
+(shear (0.03, 0)) simulated on the device.  The device's sheared models, its sky model and
+its noise are checked against the oracle (sampled rows) and the noise distribution; then
+galaxies spread over the flux range are extracted and fitted on the device and on the oracle
+from the same residual.
+
+Also the device simulation at C1 (sheared models and sky model against the oracle on
+every row) and the science loop: a sheared catalog simulated, fitted and passed through the
+reference's shear estimator recovers the input shear.
+
+Tolerances: cell counts and coordinates bit-exact; gridded sums 1e-11 relative; fitted e
+1e-8 per galaxy on the same facet; whole loop: |de| <= 1e-4 with equal error flags (the
+estimator's bar; the loop carries rounding differences through every subtraction).
+"""
+import ctypes as C
+import math
+import os
+import tempfile
+import time
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from radiolensfit2_amd import _native as N
+from radiolensfit2_amd import pipeline as P
+from radiolensfit2_amd import shear as SH
+from radiolensfit2_amd import synth
+from tests import _common as T
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _device_obs(obs, prec=N.RL2_PREC_FP64):
+    """An rl2_obs over obs's geometry with a zero residual."""
+    H = N.hip()
+    N.check(H.rl2_init(0), "rl2_init")
+    h = C.c_void_p()
+    N.check(H.rl2_obs_create(C.byref(h), obs.nch, obs.nrow, N.dptr(obs.u), N.dptr(obs.v),
+                             N.dptr(obs.w), N.dptr(obs.wn), N.dptr(obs.spec), N.fptr(obs.sigma2),
+                             None, obs.len, prec), "rl2_obs_create")
+    dev = P.DeviceObservation.__new__(P.DeviceObservation)
+    dev.obs, dev.precision, dev.h = obs, prec, h
+    return dev
+
+
+def _oracle_models(obs, cat, rows, sheared):
+    """Sum over the catalog of data_galaxy_visibilities on the given rows (oracle)."""
+    L = O.lib()
+    u, v, w = (np.ascontiguousarray(x[rows]) for x in (obs.u, obs.v, obs.w))
+    out = np.zeros((obs.nch, len(rows)), np.complex128)
+    seg = np.zeros(len(rows), np.complex128)
+    for g in range(len(cat)):
+        e1, e2 = (cat.e1[g], cat.e2[g]) if sheared else (0.0, 0.0)
+        for c in range(obs.nch):
+            L.orl2_data_galaxy_vis(obs.spec[c], obs.wn[c], e1, e2, cat.scale[g], cat.flux[g],
+                                   cat.l[g], cat.m[g], len(rows), O.ptr(u), O.ptr(v), O.ptr(w),
+                                   O.cptr(seg))
+            out[c] += seg
+    return out
+
+
+# ---------------------------------------------------------------- C1
+
+@pytest.fixture(scope="module")
+def c1(gpu):
+    t0 = time.time()
+    cat = synth.make_catalog(100, seed=1, g=(0.02, -0.01))
+    obs = synth.make_observation(synth.ARRAY_64, cat)
+    vs = T.oracle_sky_model(obs, cat)
+    _log(f"C1 observation: {obs.nrow} rows x {obs.nch} ch, {len(cat)} galaxies, "
+         f"{time.time() - t0:.1f}s")
+    return obs, cat, vs
+
+
+def test_c1_whole_loop_matches_oracle(c1):
+    obs, cat, vs = c1
+    t0 = time.time()
+    ores, ost, ovis = T.oracle_process(obs, cat, vs)
+    t1 = time.time()
+    d = P.DeviceObservation(obs, vis_sky=vs)
+    res, st = d.process(cat)
+    resid = d.residual()
+    d.close()
+    t2 = time.time()
+    de = np.array([max(abs(res[g].e1 - ores[g].e1), abs(res[g].e2 - ores[g].e2))
+                   for g in range(len(cat))])
+    same = np.array([(res[g].error, res[g].pass_, res[g].subtracted) ==
+                     (ores[g].error, ores[g].pass_, ores[g].subtracted) for g in range(len(cat))])
+    _log(f"C1 loop: oracle {t1 - t0:.1f}s, GPU {t2 - t1:.2f}s; bad {ost.n_bad_first}/"
+         f"{ost.n_bad_final} vs {st.n_bad_first}/{st.n_bad_final}; max |de| {de.max():.3e}; "
+         f"flags equal {int(same.sum())}/{len(cat)}")
+    assert same.all()
+    assert (st.n_bad_first, st.n_bad_final) == (ost.n_bad_first, ost.n_bad_final)
+    assert de.max() <= 1e-4
+    assert np.median(de) <= 1e-9
+    assert np.abs(resid - ovis).max() <= 1e-8 * np.abs(ovis).max()
+
+
+def test_c1_device_sky_model_and_sheared_models_match_oracle(c1):
+    """rl2_obs_add_sources (add_sources_kernel) for round models (sky_model,
+    data_simulation.c:131-165) and sheared models (data_simulation.c:81-106) on every row."""
+    obs, cat, vs = c1
+    dev = _device_obs(obs)
+    try:
+        dev.add_sources(cat, -1.0, round_models=True)
+        sky = -dev.residual()
+        assert np.abs(sky - vs).max() <= 1e-12 * np.abs(vs).max()
+        dev.add_sources(cat, 1.0, round_models=True)
+        assert np.abs(dev.residual()).max() <= 1e-12 * np.abs(vs).max()
+        dev.add_sources(cat, 1.0)
+        got = dev.residual().reshape(obs.nch, obs.nrow)
+    finally:
+        dev.close()
+    rows = np.arange(0, obs.nrow, 7)
+    want = _oracle_models(obs, cat, rows, sheared=True)
+    assert np.abs(got[:, rows] - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_c1_device_noise_distribution(c1):
+    """rl2_obs_add_noise: complex Gaussian noise of the requested variance per component,
+    uncorrelated between components and channels, repeatable by seed."""
+    obs, cat, vs = c1
+    s2 = float(obs.sigma2[0])
+    dev = _device_obs(obs)
+    try:
+        dev.add_noise(s2, 99)
+        a = dev.residual()
+        dev.add_noise(s2, 99)
+        b = dev.residual() - a
+        dev.add_noise(s2, 100)
+        c = dev.residual() - a - b
+    finally:
+        dev.close()
+    np.testing.assert_array_equal(a, b)          # same seed, same stream
+    assert np.abs(c - a).min() > 0 or not np.array_equal(c, a)
+    n = a.size
+    for x in (a.real, a.imag, c.real):
+        z = x / math.sqrt(s2)
+        assert abs(z.mean()) <= 5.0 / math.sqrt(n)
+        assert abs(z.var() - 1.0) <= 5.0 * math.sqrt(2.0 / n)
+        assert abs(((z ** 4).mean()) - 3.0) <= 0.02          # Gaussian kurtosis
+    zr, zi = a.real / math.sqrt(s2), a.imag / math.sqrt(s2)
+    assert abs(np.mean(zr * zi)) <= 5.0 / math.sqrt(n)
+    ch = a.reshape(obs.nch, obs.nrow).real / math.sqrt(s2)
+    assert abs(np.mean(ch[0] * ch[1])) <= 5.0 / math.sqrt(obs.nrow)
+
+
+def test_shear_recovered_through_the_whole_loop(gpu):
+    """Catalog -> simulation with shear g -> the GPU loop -> the reference's shear estimator
+    (shear.py:135-145): the recovered shear matches g (orientation pairs cancel the intrinsic
+    shapes; what is left is measurement noise, estimated by the bootstrap)."""
+    g_in = (0.05, -0.04)
+    cfg = synth.ARRAY_64
+    u, v, w, wn, spec, sigma2, flag, len_l = synth.make_geometry(cfg)
+    obs = synth.Observation(cfg, u, v, w, wn, spec, sigma2, flag, len_l, np.zeros(0, np.complex128))
+    cat = synth.make_catalog(600, seed=11, fmin=40.0, fmax=200.0, g=g_in)
+    # SNR of each sheared model (data_simulation.c:108-125: sqrt(sum |V|^2 / sigma^2)), from
+    # every 8th row scaled by 8
+    a = cat.scale * (math.pi / 648000.0)
+    sub = slice(0, obs.nrow, 8)
+    snr = np.zeros(len(cat))
+    for k in range(len(cat)):
+        det = 1 - cat.e1[k] ** 2 - cat.e2[k] ** 2
+        k1 = (1 + cat.e1[k]) * u[sub] + cat.e2[k] * v[sub]
+        k2 = cat.e2[k] * u[sub] + (1 - cat.e1[k]) * v[sub]
+        p = 0.0
+        for c in range(obs.nch):
+            shape = cat.flux[k] * spec[c] * (1 + (a[k] / det) ** 2 * wn[c] ** 2 * (k1 ** 2 + k2 ** 2)) ** -1.5
+            p += np.sum(shape ** 2)
+        snr[k] = math.sqrt(8.0 * p / float(sigma2[0]))
+    dev = _device_obs(obs)
+    try:
+        dev.add_sources(cat, 1.0)
+        dev.add_noise(float(sigma2[0]), 4242)
+        dev.add_sources(cat, -1.0, round_models=True)
+        res, st = dev.process(cat)
+    finally:
+        dev.close()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "ellipticities.txt")
+        SH.write_ellipticities(path, cat.flux, cat.e1, cat.e2, res, snr, cat.l, cat.m)
+        ((g1, s1), (g2, s2), sigma), bad = SH.shear_from_file(path, bootnum=500, seed=1)
+    _log(f"shear: in {g_in}, out ({g1:.4f} +- {s1:.4f}, {g2:.4f} +- {s2:.4f}), shape noise "
+         f"{sigma:.3f}, rejected {bad}, bad fits {st.n_bad_first}/{st.n_bad_final}")
+    assert abs(g1 - g_in[0]) <= 4 * s1 + 0.2 * abs(g_in[0])
+    assert abs(g2 - g_in[1]) <= 4 * s2 + 0.2 * abs(g_in[1])
+
+
+# ---------------------------------------------------------------- C2
+
+@pytest.fixture(scope="module")
+def c2(gpu):
+    t0 = time.time()
+    cfg = synth.ARRAY_197
+    u, v, w, wn, spec, sigma2, flag, len_l = synth.make_geometry(cfg)
+    obs = synth.Observation(cfg, u, v, w, wn, spec, sigma2, flag, len_l, np.zeros(0, np.complex128))
+    cat = synth.make_catalog(1000, seed=1, g=(0.03, 0.0))
+    _log(f"C2 geometry: {obs.nrow} rows x {obs.nch} ch = {obs.nvis / 1e6:.1f}M vis, "
+         f"{time.time() - t0:.1f}s")
+    return obs, cat
+
+
+def test_c2_device_simulation_and_galaxies_match_oracle(c2):
+    obs, cat = c2
+    dev = _device_obs(obs)
+    rows = np.arange(0, obs.nrow, 1009)
+    try:
+        t0 = time.time()
+        dev.add_sources(cat, 1.0)
+        D = dev.residual().reshape(obs.nch, obs.nrow)
+        want = _oracle_models(obs, cat, rows, sheared=True)
+        assert np.abs(D[:, rows] - want).max() <= 1e-12 * np.abs(want).max()
+        _log(f"C2 sheared models on {len(rows)} rows x {obs.nch} ch match the oracle "
+             f"({time.time() - t0:.1f}s)")
+        dev.add_noise(float(obs.sigma2[0]), 12345)
+        Dn = dev.residual().reshape(obs.nch, obs.nrow)
+        z = (Dn - D).ravel() / math.sqrt(float(obs.sigma2[0]))
+        del D
+        assert abs(z.real.mean()) <= 5.0 / math.sqrt(z.size)
+        assert abs(z.real.var() - 1.0) <= 5.0 * math.sqrt(2.0 / z.size)
+        assert abs(z.imag.var() - 1.0) <= 5.0 * math.sqrt(2.0 / z.size)
+        del z
+        dev.add_sources(cat, -1.0, round_models=True)
+        R0 = dev.residual()
+        sky = _oracle_models(obs, cat, rows, sheared=False)
+        got_sky = Dn[:, rows] - R0.reshape(obs.nch, obs.nrow)[:, rows]
+        assert np.abs(got_sky - sky).max() <= 1e-11 * np.abs(sky).max()
+        del Dn
+        _log(f"C2 noise and sky model checked ({time.time() - t0:.1f}s)")
+        zeros = np.zeros_like(R0)
+        ro = P.r_grid()
+        for g in (0, 249, 499, 749, 999):
+            t1 = time.time()
+            facet = N.hip().facet_size(cat.scale[g], obs.len)
+            gv, gs, guu, gvv, gsw = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g],
+                                                facet, slot=0)
+            dev.set_prior(P.rprior_for(cat.flux[g], ro), slot=0)
+            r = dev.fit(slot=0)
+            fv, fs, uu, vv, sw = O.extract(obs, cat.l[g], cat.m[g], cat.flux[g], cat.scale[g],
+                                           facet, R0, zeros)
+            params, keep = T.oracle_params(fv, fs, uu, vv, cat.flux[g])
+            oerr, (e1, e2, v1, v2, od, maxL) = T.oracle_fit(params)
+            _log(f"C2 galaxy {g}: flux {cat.flux[g]:.1f} facet {facet} n {len(fv)}: GPU "
+                 f"({r.e1:.6f}, {r.e2:.6f}) err {r.error}, oracle ({e1:.6f}, {e2:.6f}) err {oerr} "
+                 f"({time.time() - t1:.1f}s)")
+            assert len(gv) == len(fv)
+            np.testing.assert_array_equal(gsw, sw)
+            np.testing.assert_array_equal(guu, uu)
+            np.testing.assert_array_equal(gvv, vv)
+            np.testing.assert_allclose(gs, fs, rtol=1e-12, atol=0)
+            assert np.abs(gv - fv).max() <= 1e-11 * np.abs(fv).max()
+            assert r.error == oerr
+            assert abs(r.e1 - e1) <= 1e-8 and abs(r.e2 - e2) <= 1e-8
+            assert abs(r.maxL - maxL) <= 1e-8 * max(1.0, abs(maxL))
+    finally:
+        dev.close()

@@ -239,3 +239,45 @@ def test_tune_knobs_validate_and_round_trip():
         H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
         H.rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, 0)
         H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 1)
+
+
+def _philox_py(ctr, key, rounds=10):
+    """Philox4x32 from its published definition (Salmon et al., SC'11, Table 2 constants):
+    each round maps (c0, c1, c2, c3) to (hi(M1 c2) ^ c1 ^ k0, lo(M1 c2), hi(M0 c0) ^ c3 ^ k1,
+    lo(M0 c0)); the key advances by the Weyl constants after every round."""
+    M0, M1, W0, W1 = 0xD2511F53, 0xCD9E8D57, 0x9E3779B9, 0xBB67AE85
+    c = list(ctr)
+    k0, k1 = key
+    mask = 0xFFFFFFFF
+    for _ in range(rounds):
+        p0, p1 = M0 * c[0], M1 * c[2]
+        c = [((p1 >> 32) ^ c[1] ^ k0) & mask, p1 & mask, ((p0 >> 32) ^ c[3] ^ k1) & mask,
+             p0 & mask]
+        k0, k1 = (k0 + W0) & mask, (k1 + W1) & mask
+    return c
+
+
+def test_noise_generator_philox_known_answers():
+    """The simulated noise's Philox4x32-10 (host evaluation of the kernel's function) against
+    the Random123 known-answer vectors and a restatement of the published algorithm."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    H = N.hip()
+    for ctr, key, want in kat:
+        c = np.array(ctr, np.uint32)
+        k = np.array(key, np.uint32)
+        out = np.zeros(4, np.uint32)
+        H.rl2_test_philox4x32_10(1, c.ctypes.data_as(N.c_uint_p), k.ctypes.data_as(N.c_uint_p),
+                                 out.ctypes.data_as(N.c_uint_p))
+        assert tuple(int(x) for x in out) == want
+        assert tuple(_philox_py(ctr, key)) == want
+    rng = np.random.default_rng(8)
+    ctr = rng.integers(0, 2 ** 32, (64, 4), dtype=np.uint64).astype(np.uint32)
+    key = np.array([12345, 0], np.uint32)
+    out = np.zeros((64, 4), np.uint32)
+    H.rl2_test_philox4x32_10(64, ctr.ctypes.data_as(N.c_uint_p), key.ctypes.data_as(N.c_uint_p),
+                             out.ctypes.data_as(N.c_uint_p))
+    for i in range(64):
+        assert [int(x) for x in out[i]] == _philox_py([int(x) for x in ctr[i]], (12345, 0))
