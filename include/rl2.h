@@ -176,9 +176,11 @@ int rl2_stats_get(int kernel, rl2_kernel_stat *out);
 int rl2_synchronize(void);
 
 /* ---- launch-shape knobs (set by an autotuner; process-wide) ----
- * RL2_TUNE_GRID_VARIANT: extraction gridding, 0 per-visibility LDS hash,
- *   1 wave-aggregated cells, 2 / 3 / 4 runs of 8 / 4 / 4 consecutive rows merged per
- *   thread (LDS tables of 2048 / 1024 / 2048 slots); default 3.
+ * RL2_TUNE_GRID_VARIANT: extraction gridding, runs of consecutive rows merged per thread
+ *   before one LDS table update: 0 / 1 / 2 / 3 / 4 runs of 2 / 2 / 8 / 4 / 4 rows with
+ *   tables of 512 / 1024 / 2048 / 1024 / 2048 slots; default 3.  Every variant accumulates
+ *   in integers, so the facet does not depend on the variant or on the order of the
+ *   work-groups.
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
  *   in [2048, 65536], or 0 (default) to choose per launch.
  * RL2_TUNE_PHASE_TABLES: 2 (default) evaluates the FP64 Newton phases on extracted facets

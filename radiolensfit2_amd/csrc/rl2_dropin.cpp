@@ -273,8 +273,8 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     DBuf<uint32_t> dperm(ncoords);
     DBuf<float> ds2(nvis), ds2f(nvis);
     DBuf<unsigned char> dfl(flag ? nvis : 1);
-    DBuf<double> acc(4 * ncell), cu(ncell), cv(ncell), cre(ncell), cim(ncell), cs2(ncell),
-        csw(ncell);
+    DBuf<double> acc(grid_acc_words(sizeg)), cu(ncell), cv(ncell), cre(ncell), cim(ncell),
+        cs2(ncell), csw(ncell);
     DBuf<unsigned> scan(compact_scratch_words(sizeg) + 1), dn(1);
     du.up(u, ncoords);
     dv.up(v, ncoords);
@@ -285,7 +285,14 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     ds2.up(s2.data(), nvis);
     if (flag) dfl.up((const unsigned char *)flag, nvis);
     if (mode == 1) dres.up((const double *)vis, 2 * nvis);
-    hipMemsetAsync(acc.p, 0, 4 * ncell * sizeof(double), stream());
+    hipMemsetAsync(acc.p, 0, grid_acc_words(sizeg) * sizeof(double), stream());
+    // scales of the integer accumulators from the largest gridded value and variance
+    double vb = 0., sb = 0.;
+    if (mode == 1)
+        for (size_t i = 0; i < nvis; i++)
+            vb = fmax(vb, fmax(fabs(vis[i].real), fabs(vis[i].imag)));
+    for (size_t i = 0; i < nvis; i++) sb = fmax(sb, (double)fabsf(s2[i]));
+    const AccScale sc = fx_scales(vb, sb);
     launch_pack_sigma(stream(), nch, ncoords, dperm.p, ds2.p, flag ? dfl.p : nullptr, ds2f.p);
     GridLaunch a = {};
     a.mode = mode;
@@ -304,10 +311,11 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     a.inc = 2 * len / sizeg;
     a.facet = sizeg;
     a.acc = acc.p;
+    a.fx_scale = sc.fx_scale;
+    a.fs_scale = sc.fs_scale;
     launch_grid(stream(), a);
     launch_compact(stream(), acc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
-                   dn.p, csw.p,
-                   scan.p);
+                   dn.p, csw.p, scan.p, &sc);
     unsigned n = 0;
     dn.down(&n, 1);
     if (!sync_ok()) {
@@ -367,8 +375,7 @@ static unsigned compact_host(unsigned sizeg, double len, const double *re, const
     DBuf<unsigned> scan(compact_scratch_words(sizeg) + 1), dn(1);
     dacc.up(acc.data(), 4 * ncell);
     launch_compact(stream(), dacc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
-                   dn.p, nullptr,
-                   scan.p);
+                   dn.p, nullptr, scan.p, nullptr);
     unsigned n = 0;
     dn.down(&n, 1);
     if (!sync_ok()) return 0;

@@ -69,7 +69,24 @@ void launch_add_sources(hipStream_t s, int prec, unsigned nch, unsigned long nro
 void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma, uint64_t seed,
                       void *res);
 
-// Pillbox gridding into dense accumulators acc[4][facet^2] (re, im, sigma2, weight).
+// Power-of-two scales of the gridding's integer accumulators: a value x is carried as
+// rint(x * fx_scale) (visibility components) or rint(x * fs_scale) (sigma^2), with
+// |x| * scale * (rows of a gridding work-group) <= 2^62 (fx_scales(), for the gridding
+// variant selected when it is called).
+struct AccScale {
+    double fx_scale = 1., fs_scale = 1.;
+};
+// The scales for visibility components bounded by vis_bound and variances by s2_bound.
+AccScale fx_scales(double vis_bound, double s2_bound);
+// Words (8 bytes) of a facet's gridding accumulators.
+size_t grid_acc_words(unsigned facet);
+// *out = max(*out, max over the residual of |re| + |im|), as the bits of a double (*out
+// must start at 0 or at an earlier bound's bits).
+void launch_abs_bound(hipStream_t s, int prec, unsigned long nvis, const void *res,
+                      unsigned long long *out);
+
+// Pillbox gridding into a facet's dense integer accumulators (grid_acc_words(facet) words,
+// zero before the first launch; compaction clears them again).
 // mode 0: extraction (round model of `g` added, phase-shifted to (l0,m0)), reads the
 //         residual `res` (prec selects complex128/complex64).
 // mode 1: plain gridding of complex128 `res` (drop-in gridding_visibilities).
@@ -85,7 +102,8 @@ struct GridLaunch {
     GalaxyGeom g;
     double len, inc;
     unsigned facet;
-    double *acc;               // 4 * facet^2
+    double *acc;               // grid_acc_words(facet) words
+    double fx_scale = 1., fs_scale = 1.;  // AccScale of the launch
     // mode 0, gridding variant 3 only: apply this post-fit update to the residual first
     // (written back through res_w), then grid the updated visibilities
     int sub = 0;
@@ -108,6 +126,8 @@ struct Tuning {
 Tuning &tuning();
 
 // Compaction of dense accumulators into the facet arrays; clears the accumulators.
+// fx: the integer accumulators of launch_grid with these scales; null: four double arrays
+// acc[4][facet^2] (re, im, sigma^2, weight) given by a caller.
 // n_out (device) receives the number of non-empty cells. sum_w_dense (optional, device)
 // receives the dense weights before clearing.
 // cid / ctr (optional): packed row and column of each compacted cell, and the centre
@@ -115,7 +135,7 @@ Tuning &tuning();
 void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
                     double *vv, double *dre, double *dim, double *s2, uint32_t *cid, double *ctr,
                     unsigned *n_out,
-                    double *sum_w_dense, unsigned *scratch_blocks);
+                    double *sum_w_dense, unsigned *scratch_blocks, const AccScale *fx);
 size_t compact_scratch_words(unsigned facet);
 // FP32 packing of a compacted facet (count read from the device descriptor).
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,

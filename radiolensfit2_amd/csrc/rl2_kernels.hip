@@ -5,6 +5,9 @@
 // complex64 (RL2_PREC_MIXED).  sigma^2 is one float per visibility with the sign bit
 // marking a flagged visibility (sigma^2 > 0 after defaulting, RadioLensfit2.cpp:266-269).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
 #include <hipcub/hipcub.hpp>
 
 #include "rl2_internal.h"
@@ -189,139 +192,33 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 }
 
 // ------------------------------------------------------------------ extraction + gridding
-// Variant 0.  One work-group owns RPT*256 consecutive (sorted) rows, thread t rows
-// t, t+256, ...  Per channel each thread adds its RPT visibilities into an LDS
-// open-addressing table keyed by cell index (at most RPT*256 distinct keys, table size
-// 2*RPT*256, so probing always terminates); the occupied slots (listed as they are
-// claimed) are then flushed to the global accumulators with one atomic per cell and
-// component.  Sorted rows make neighbouring threads hit the same cells, so hot cells near
-// the uv origin receive one global atomic per work-group instead of one per visibility.
-// The next channel's sigma^2 and residual are loaded before the current channel is
-// gridded, so their latency overlaps the arithmetic and the LDS traffic.
-template <int MODE, int PREC, int RPT>
-__global__ __launch_bounds__(256) void grid_direct_kernel(GridLaunch a)
+// Deterministic accumulation.  The gridded sums of a cell are kept as integers, so the
+// order in which work-groups and threads add them does not change a single bit:
+//  - a value x (a visibility component, or sigma^2) is carried as rint(x * 2^S), with S
+//    chosen per launch from a bound B >= |x| (the residual's bound plus the galaxy's flux;
+//    the largest sigma^2) so that B * 2^S * (rows of a work-group) <= 2^62;
+//  - a work-group adds its runs in LDS as int64 (the slot sums of a channel cover at most
+//    the work-group's rows, so they stay below 2^62);
+//  - the flush splits each slot sum into its high word (arithmetic shift by 32) and its low
+//    32 bits, added by integer atomics into two 64-bit accumulators per cell and component
+//    (no carries are lost: the low sums stay below 2^64 for fewer than 2^32 flushes);
+//  - compaction recombines hi * 2^32 + lo and scales by 2^-S.
+// A term's rounding (one per run of up to 4 visibilities) is at most B * 2^-53 for 1024-row
+// work-groups, and the sum itself is exact, so the facet agrees with the reference's sequential FP64 sums to
+// their own rounding and is bit-identical from run to run.
+// Accumulator layout: 7 arrays of facet^2 words: re hi, re lo, im hi, im lo, sigma^2 hi,
+// sigma^2 lo (uint64, two's complement), weight (double, integer-valued).
+enum { kAccReHi = 0, kAccReLo, kAccImHi, kAccImLo, kAccS2Hi, kAccS2Lo, kAccW, kAccWords };
+
+__device__ inline long long fx_round(double x, double scale)
 {
-    constexpr int BS = 256;
-    constexpr int HS = 2 * RPT * BS;
-    constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
-    static_assert((1 << HBITS) == HS, "hash size must be a power of two");
-    __shared__ uint32_t hkey[HS];
-    __shared__ double hre[HS], him[HS], hs2[HS];
-    __shared__ uint32_t hcnt[HS];
-    __shared__ uint32_t hlist[HS];
-    __shared__ uint32_t hn;
-    const int tid = threadIdx.x;
-    for (int s = tid; s < HS; s += BS) {
-        hkey[s] = kEmpty;
-        hre[s] = 0.;
-        him[s] = 0.;
-        hs2[s] = 0.;
-        hcnt[s] = 0;
-    }
-    if (tid == 0) hn = 0;
-    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS);
-    double ru[RPT], rv[RPT], rp[RPT];
-    bool rok[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        unsigned long i = row0 + tid + (unsigned long)k * BS;
-        rok[k] = i < a.nrow;
-        ru[k] = rok[k] ? a.u[i] : 0.;
-        rv[k] = rok[k] ? a.v[i] : 0.;
-        double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
-        rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
-    }
-    const unsigned long ncell = (unsigned long)a.facet * a.facet;
-    const double inv_inc = 1.0 / a.inc;
-    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
-           *acc_w = a.acc + 3 * ncell;
-    // channel-0 inputs
-    float ns2[RPT];
-    double2 nres[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const size_t idx = row0 + tid + (size_t)k * BS;
-        ns2[k] = rok[k] ? a.s2f[idx] : -1.f;
-        nres[k] = (rok[k] && MODE != 2) ? load_res(a.res, PREC, idx) : make_double2(0., 0.);
-    }
-    __syncthreads();
-    for (unsigned c = 0; c < a.nch; c++) {
-        const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
-        const double wn2 = wn * wn, spectra = a.spec[c];
-        float cs2[RPT];
-        double2 cres[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            cs2[k] = ns2[k];
-            cres[k] = nres[k];
-        }
-        if (c + 1 < a.nch) {
-#pragma unroll
-            for (int k = 0; k < RPT; k++) {
-                const size_t idx = (size_t)(c + 1) * a.nrow + row0 + tid + (size_t)k * BS;
-                ns2[k] = rok[k] ? a.s2f[idx] : -1.f;
-                nres[k] = (rok[k] && MODE != 2) ? load_res(a.res, PREC, idx)
-                                                : make_double2(0., 0.);
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            const float s2 = cs2[k];
-            if (!rok[k] || signbit(s2)) continue;  // outside the observation, or flagged
-            uint32_t key;
-            if (!cell_index_fast(ru[k], rv[k], inv_lambda, a.len, a.inc, inv_inc, a.facet, &key))
-                continue;
-            double xr = 0., xi = 0.;
-            if (MODE == 0) {
-                // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
-                // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
-                const double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
-                double sn, cs;
-                sincos_rev64(rp[k] * inv_lambda, sn, cs);
-                const double2 r = cres[k];
-                const double gre = shape * cs + r.x;
-                const double gim = shape * sn + r.y;
-                xr = gre * cs - gim * (-sn);
-                xi = gre * (-sn) + gim * cs;
-            } else if (MODE == 1) {
-                xr = cres[k].x;
-                xi = cres[k].y;
-            }
-            uint32_t h = (key * 2654435761u) >> (32 - HBITS);
-            while (true) {
-                const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
-                if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
-                if (prev == kEmpty || prev == key) break;
-                h = (h + 1) & (HS - 1);
-            }
-            if (MODE != 2) {
-                atomicAdd(&hre[h], xr);
-                atomicAdd(&him[h], xi);
-            }
-            atomicAdd(&hs2[h], (double)s2);
-            atomicAdd(&hcnt[h], 1u);
-        }
-        __syncthreads();
-        const uint32_t nocc = hn;
-        for (uint32_t j = tid; j < nocc; j += BS) {
-            const uint32_t s = hlist[j];
-            const uint32_t key = hkey[s];
-            if (MODE != 2) {
-                unsafeAtomicAdd(&acc_re[key], hre[s]);
-                unsafeAtomicAdd(&acc_im[key], him[s]);
-            }
-            unsafeAtomicAdd(&acc_s2[key], hs2[s]);
-            unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
-            hkey[s] = kEmpty;
-            hre[s] = 0.;
-            him[s] = 0.;
-            hs2[s] = 0.;
-            hcnt[s] = 0;
-        }
-        __syncthreads();
-        if (tid == 0) hn = 0;
-        __syncthreads();
-    }
+    return (long long)rint(x * scale);
+}
+
+__device__ inline void fx_flush(unsigned long long *hi, unsigned long long *lo, long long v)
+{
+    atomicAdd(hi, (unsigned long long)(v >> 32));
+    atomicAdd(lo, (unsigned long long)(v & 0xffffffffll));
 }
 
 // Variants 2-4.  Each thread owns RPT consecutive sorted rows and, per channel, adds runs
@@ -341,25 +238,26 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
     static_assert(HS >= RPT * BS, "a free slot must exist for every run of a channel");
     constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
     static_assert((1 << HBITS) == HS, "hash size must be a power of two");
+    static_assert(RPT * BS <= 4096, "slot sums of a channel must stay below 2^62");
     __shared__ uint32_t hkey[HS];
-    __shared__ double hre[HS], him[HS], hs2[HS];
+    __shared__ unsigned long long hre[HS], him[HS], hs2[HS];  // rint(x * 2^S), int64
     __shared__ uint32_t hcnt[HS];
     __shared__ uint32_t hlist[HS];
     __shared__ uint32_t hn;
     const int tid = threadIdx.x;
     for (int s = tid; s < HS; s += BS) {
         hkey[s] = kEmpty;
-        hre[s] = 0.;
-        him[s] = 0.;
-        hs2[s] = 0.;
+        hre[s] = 0ull;
+        him[s] = 0ull;
+        hs2[s] = 0ull;
         hcnt[s] = 0;
     }
     if (tid == 0) hn = 0;
     const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS) + (unsigned long)tid * RPT;
     const unsigned long ncell = (unsigned long)a.facet * a.facet;
     const double inv_inc = 1.0 / a.inc;
-    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
-           *acc_w = a.acc + 3 * ncell;
+    unsigned long long *acc = (unsigned long long *)a.acc;
+    double *acc_w = a.acc + kAccW * ncell;
     auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {
         uint32_t h = (key * 2654435761u) >> (32 - HBITS);
         while (true) {
@@ -369,10 +267,10 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
             h = (h + 1) & (HS - 1);
         }
         if (MODE != 2) {
-            atomicAdd(&hre[h], xr);
-            atomicAdd(&him[h], xi);
+            atomicAdd(&hre[h], (unsigned long long)fx_round(xr, a.fx_scale));
+            atomicAdd(&him[h], (unsigned long long)fx_round(xi, a.fx_scale));
         }
-        atomicAdd(&hs2[h], xs);
+        atomicAdd(&hs2[h], (unsigned long long)fx_round(xs, a.fs_scale));
         atomicAdd(&hcnt[h], n);
     };
     // the rows' u, v and phase numerator p = ul + vm + wn stay in registers for all
@@ -467,181 +365,18 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
             const uint32_t s = hlist[j];
             const uint32_t key = hkey[s];
             if (MODE != 2) {
-                unsafeAtomicAdd(&acc_re[key], hre[s]);
-                unsafeAtomicAdd(&acc_im[key], him[s]);
+                fx_flush(acc + kAccReHi * ncell + key, acc + kAccReLo * ncell + key,
+                         (long long)hre[s]);
+                fx_flush(acc + kAccImHi * ncell + key, acc + kAccImLo * ncell + key,
+                         (long long)him[s]);
             }
-            unsafeAtomicAdd(&acc_s2[key], hs2[s]);
-            unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+            fx_flush(acc + kAccS2Hi * ncell + key, acc + kAccS2Lo * ncell + key,
+                     (long long)hs2[s]);
+            unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);  // integer-valued: exact
             hkey[s] = kEmpty;
-            hre[s] = 0.;
-            him[s] = 0.;
-            hs2[s] = 0.;
-            hcnt[s] = 0;
-        }
-        __syncthreads();
-        if (tid == 0) hn = 0;
-        __syncthreads();
-    }
-}
-
-// Variant 1.  One work-group owns RPT*256 consecutive (sorted) rows, each thread RPT
-// consecutive rows.  Sorted rows are neighbours in uv, so most visibilities of a wave fall
-// in a few cells.
-// Per channel:
-//  1. each thread merges runs of equal cells among its own RPT visibilities;
-//  2. per row slot, the wave sums the values of lanes sharing a cell (one pass per
-//     distinct cell: the lowest pending lane's cell is broadcast, matching lanes are summed
-//     with cross-lane adds) and that lane alone inserts the sum into an LDS open-addressing
-//     table keyed by cell (HS >= RPT*256 slots, so a free slot always exists);
-//  3. the table's occupied slots are flushed to the global accumulators with one atomic
-//     per cell and component.
-// LDS atomics are therefore issued once per distinct cell per wave instead of once per
-// visibility (same-address LDS atomics serialise across the 64 lanes).
-template <int MODE, int PREC, int RPT>
-__global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
-{
-    constexpr int BS = 256;
-    constexpr int HS = RPT * BS;
-    constexpr int HBITS = (HS == 2048) ? 11 : (HS == 1024) ? 10 : (HS == 4096) ? 12 : 9;
-    static_assert((1 << HBITS) == HS, "hash size must be a power of two");
-    __shared__ uint32_t hkey[HS];
-    __shared__ double hre[HS], him[HS], hs2[HS];
-    __shared__ uint32_t hcnt[HS];
-    __shared__ uint32_t hlist[HS];
-    __shared__ uint32_t hn;
-    const int tid = threadIdx.x, lane = tid & 63;
-    for (int s = tid; s < HS; s += BS) {
-        hkey[s] = kEmpty;
-        hre[s] = 0.;
-        him[s] = 0.;
-        hs2[s] = 0.;
-        hcnt[s] = 0;
-    }
-    if (tid == 0) hn = 0;
-    const unsigned long row0 = (unsigned long)blockIdx.x * (RPT * BS) + (unsigned long)tid * RPT;
-    double ru[RPT], rv[RPT], rp[RPT];
-    bool rok[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const unsigned long i = row0 + k;
-        rok[k] = i < a.nrow;
-        ru[k] = rok[k] ? a.u[i] : 0.;
-        rv[k] = rok[k] ? a.v[i] : 0.;
-        const double wi = (rok[k] && MODE == 0) ? a.w[i] : 0.;
-        rp[k] = ru[k] * a.g.l0 + rv[k] * a.g.m0 + wi * a.g.n0;
-    }
-    const unsigned long ncell = (unsigned long)a.facet * a.facet;
-    const double inv_inc = 1.0 / a.inc;
-    double *acc_re = a.acc, *acc_im = a.acc + ncell, *acc_s2 = a.acc + 2 * ncell,
-           *acc_w = a.acc + 3 * ncell;
-    __syncthreads();
-    for (unsigned c = 0; c < a.nch; c++) {
-        const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
-        const double wn2 = wn * wn, spectra = a.spec[c];
-        uint32_t key[RPT], cnt[RPT];
-        double xr[RPT], xi[RPT], xs[RPT];
-        bool ok[RPT];
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            ok[k] = false;
-            key[k] = kEmpty;
-            cnt[k] = 1;
-            xr[k] = xi[k] = xs[k] = 0.;
-            if (!rok[k]) continue;
-            const size_t idx = (size_t)c * a.nrow + row0 + k;
-            const float s2 = a.s2f[idx];
-            if (signbit(s2)) continue;  // flagged
-            if (!cell_index_fast(ru[k], rv[k], inv_lambda, a.len, a.inc, inv_inc, a.facet,
-                                 &key[k]))
-                continue;
-            ok[k] = true;
-            xs[k] = (double)s2;
-            if (MODE == 0) {
-                // visGal = round model + (visData - visSkyMod), then shift by -k(ul+vm+wn)
-                // (source_extraction.c:51-65); the phase k p in revolutions is p / lambda.
-                const double shape = model_shape_rsq(a.g, ru[k], rv[k], wn2, spectra);
-                double sn, cs;
-                sincos_rev64(rp[k] * inv_lambda, sn, cs);
-                const double2 r = load_res(a.res, PREC, idx);
-                const double gre = shape * cs + r.x;
-                const double gim = shape * sn + r.y;
-                xr[k] = gre * cs - gim * (-sn);
-                xi[k] = gre * (-sn) + gim * cs;
-            } else if (MODE == 1) {
-                const double2 r = load_res(a.res, PREC, idx);
-                xr[k] = r.x;
-                xi[k] = r.y;
-            }
-        }
-        // 1. runs of equal cells within the thread
-        int last = -1;
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            if (!ok[k]) continue;
-            if (last >= 0 && key[k] == key[last]) {
-                xr[last] += xr[k];
-                xi[last] += xi[k];
-                xs[last] += xs[k];
-                cnt[last] += cnt[k];
-                ok[k] = false;
-            } else {
-                last = k;
-            }
-        }
-        // 2. equal cells across the wave, one row slot at a time
-#pragma unroll
-        for (int k = 0; k < RPT; k++) {
-            bool pending = ok[k];
-            uint64_t todo = __ballot(pending);
-            while (todo) {
-                const int leader = __ffsll((unsigned long long)todo) - 1;
-                const uint32_t lk = (uint32_t)__shfl((int)key[k], leader);
-                const bool mine = pending && key[k] == lk;
-                double sr = mine ? xr[k] : 0., si = mine ? xi[k] : 0., ss = mine ? xs[k] : 0.;
-                uint32_t sc = mine ? cnt[k] : 0u;
-                for (int off = 32; off >= 1; off >>= 1) {
-                    if (MODE != 2) {
-                        sr += __shfl_xor(sr, off);
-                        si += __shfl_xor(si, off);
-                    }
-                    ss += __shfl_xor(ss, off);
-                    sc += (uint32_t)__shfl_xor((int)sc, off);
-                }
-                if (lane == leader) {
-                    uint32_t h = (lk * 2654435761u) >> (32 - HBITS);
-                    for (int probe = 0; probe < HS; probe++) {
-                        const uint32_t prev = atomicCAS(&hkey[h], kEmpty, lk);
-                        if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
-                        if (prev == kEmpty || prev == lk) break;
-                        h = (h + 1) & (HS - 1);
-                    }
-                    if (MODE != 2) {
-                        atomicAdd(&hre[h], sr);
-                        atomicAdd(&him[h], si);
-                    }
-                    atomicAdd(&hs2[h], ss);
-                    atomicAdd(&hcnt[h], sc);
-                }
-                pending = pending && !mine;
-                todo = __ballot(pending);
-            }
-        }
-        __syncthreads();
-        // 3. flush the occupied slots
-        const uint32_t nocc = hn;
-        for (uint32_t j = tid; j < nocc; j += BS) {
-            const uint32_t s = hlist[j];
-            const uint32_t cell = hkey[s];
-            if (MODE != 2) {
-                unsafeAtomicAdd(&acc_re[cell], hre[s]);
-                unsafeAtomicAdd(&acc_im[cell], him[s]);
-            }
-            unsafeAtomicAdd(&acc_s2[cell], hs2[s]);
-            unsafeAtomicAdd(&acc_w[cell], (double)hcnt[s]);
-            hkey[s] = kEmpty;
-            hre[s] = 0.;
-            him[s] = 0.;
-            hs2[s] = 0.;
+            hre[s] = 0ull;
+            him[s] = 0ull;
+            hs2[s] = 0ull;
             hcnt[s] = 0;
         }
         __syncthreads();
@@ -653,11 +388,7 @@ __global__ __launch_bounds__(256) void grid_wave_kernel(GridLaunch a)
 template <int MODE, int PREC>
 static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
 {
-    if (tuning().grid_variant == 1) {
-        constexpr int RPT = 4;
-        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
-        grid_wave_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
-    } else if (tuning().grid_variant == 2) {
+    if (tuning().grid_variant == 2) {
         constexpr int RPT = 8;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
         grid_run_kernel<MODE, PREC, RPT, 2048><<<grid, 256, 0, s>>>(a);
@@ -672,12 +403,15 @@ static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
         constexpr int RPT = 4;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
         grid_run_kernel<MODE, PREC, RPT, 2048><<<grid, 256, 0, s>>>(a);
-    } else {
-        // 512 rows per work-group: a 1024-slot table (36 KB of LDS) leaves room for four
-        // work-groups per CU
+    } else if (tuning().grid_variant == 1) {
         constexpr int RPT = 2;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
-        grid_direct_kernel<MODE, PREC, RPT><<<grid, 256, 0, s>>>(a);
+        grid_run_kernel<MODE, PREC, RPT, 1024><<<grid, 256, 0, s>>>(a);
+    } else {
+        // runs of 2 rows: 512 rows per work-group and a 512-slot table (18 KB of LDS)
+        constexpr int RPT = 2;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        grid_run_kernel<MODE, PREC, RPT, 512><<<grid, 256, 0, s>>>(a);
     }
 }
 
@@ -757,13 +491,50 @@ __global__ __launch_bounds__(1024) void compact_scan_kernel(unsigned *bcount, un
     }
 }
 
+// Accumulators of cell p: FX (device gridding) the integer layout above, scaled back by
+// 2^-S; otherwise four double arrays (re, im, sigma^2, weight) given by a caller.
+template <bool FX>
+__device__ inline void acc_read(double *acc, unsigned long ncell, unsigned long p,
+                                double fx_unscale, double fs_unscale, double &re, double &im,
+                                double &sg)
+{
+    if (FX) {
+        const unsigned long long *q = (const unsigned long long *)acc;
+        auto get = [&](int hi, int lo, double unscale) {
+            const unsigned long long l = q[lo * ncell + p];
+            const long long h = (long long)q[hi * ncell + p] + (long long)(l >> 32);
+            return ((double)h * 4294967296.0 + (double)(l & 0xffffffffull)) * unscale;
+        };
+        re = get(kAccReHi, kAccReLo, fx_unscale);
+        im = get(kAccImHi, kAccImLo, fx_unscale);
+        sg = get(kAccS2Hi, kAccS2Lo, fs_unscale);
+    } else {
+        re = acc[p];
+        im = acc[ncell + p];
+        sg = acc[2 * ncell + p];
+    }
+}
+
+template <bool FX>
+__device__ inline void acc_clear(double *acc, unsigned long ncell, unsigned long p)
+{
+    if (FX) {
+        unsigned long long *q = (unsigned long long *)acc;
+        for (int k = 0; k < kAccW; k++) q[k * ncell + p] = 0ull;
+        acc[kAccW * ncell + p] = 0.;
+    } else {
+        for (int k = 0; k < 4; k++) acc[k * ncell + p] = 0.;
+    }
+}
+
+template <bool FX>
 __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
     double *acc, unsigned facet, unsigned long ncell, double len, const unsigned *boff,
     double *uu, double *vv, double *dre, double *dim, double *s2o, uint32_t *cid, double *ctr,
-    double *sum_w_dense)
+    double *sum_w_dense, double fx_unscale, double fs_unscale)
 {
     __shared__ unsigned tscan[kCompactBS];
-    double *acc_re = acc, *acc_im = acc + ncell, *acc_s2 = acc + 2 * ncell, *acc_w = acc + 3 * ncell;
+    const double *acc_w = acc + (FX ? kAccW : 3) * ncell;
     unsigned long base = (unsigned long)blockIdx.x * kCompactTile + threadIdx.x * kCompactPT;
     unsigned cnt = 0;
     for (int k = 0; k < kCompactPT; k++) {
@@ -798,7 +569,8 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
         double wgt = acc_w[p];
         if (sum_w_dense) sum_w_dense[p] = wgt;
         if (wgt != 0.) {
-            double re = acc_re[p], im = acc_im[p], sg = acc_s2[p];
+            double re, im, sg;
+            acc_read<FX>(acc, ncell, p, fx_unscale, fs_unscale, re, im, sg);
             dre[n] = re * wgt / sg;
             dim[n] = im * wgt / sg;
             s2o[n] = sg / (wgt * wgt);
@@ -807,26 +579,81 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
             vv[n] = (j + 0.5) * inc - len;
             if (cid) cid[n] = (j << 16) | i;
             n++;
-            acc_re[p] = 0.;
-            acc_im[p] = 0.;
-            acc_s2[p] = 0.;
-            acc_w[p] = 0.;
+            acc_clear<FX>(acc, ncell, p);
         }
     }
 }
 
+size_t grid_acc_words(unsigned facet) { return (size_t)kAccWords * facet * facet; }
+
+// Rows of a gridding work-group (256 threads x rows per thread) for the current variant.
+static int grid_rows_log2()
+{
+    switch (tuning().grid_variant) {
+    case 2: return 11;           // runs of 8 rows
+    case 3: case 4: return 10;   // runs of 4 rows
+    default: return 9;           // runs of 2 rows
+    }
+}
+
+AccScale fx_scales(double vis_bound, double s2_bound)
+{
+    // 2^S with bound * 2^S * rows <= 2^62: a channel's slot sums stay below 2^62
+    const int rows_log2 = grid_rows_log2();
+    auto scale = [&](double b) {
+        if (!(b > 0.) || !std::isfinite(b)) b = 1.;
+        const int S = std::max(-1000, std::min(1000, 62 - rows_log2 - (ilogb(b) + 1)));
+        return ldexp(1.0, S);
+    };
+    AccScale a;
+    a.fx_scale = scale(vis_bound);
+    a.fs_scale = scale(s2_bound);
+    return a;
+}
+
+// Bound max |re| + |im| over the residual (as the bit pattern of a non-negative double, so
+// that integer atomicMax orders it).
+template <int PREC>
+__global__ __launch_bounds__(256) void abs_bound_kernel(const void *res, unsigned long n,
+                                                        unsigned long long *out)
+{
+    double m = 0.;
+    for (unsigned long i = blockIdx.x * 256ul + threadIdx.x; i < n; i += (unsigned long)gridDim.x * 256ul) {
+        const double2 x = load_res(res, PREC, i);
+        m = fmax(m, fabs(x.x) + fabs(x.y));
+    }
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+void launch_abs_bound(hipStream_t s, int prec, unsigned long nvis, const void *res,
+                      unsigned long long *out)
+{
+    const unsigned grid = (unsigned)std::min<unsigned long>(4096ul, (nvis + 255) / 256);
+    if (prec == 0)
+        abs_bound_kernel<0><<<grid, 256, 0, s>>>(res, nvis, out);
+    else
+        abs_bound_kernel<1><<<grid, 256, 0, s>>>(res, nvis, out);
+}
+
 void launch_compact(hipStream_t s, double *acc, unsigned facet, double len, double *uu,
                     double *vv, double *dre, double *dim, double *s2, uint32_t *cid, double *ctr,
-                    unsigned *n_out, double *sum_w_dense, unsigned *scratch)
+                    unsigned *n_out, double *sum_w_dense, unsigned *scratch, const AccScale *fx)
 {
     unsigned long ncell = (unsigned long)facet * facet;
     unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
     // ctr needs a thread per grid index: nb * kCompactBS = ceil(facet^2 / 4096) * 256 >= facet
     // for every facet size
-    compact_count_kernel<<<nb, kCompactBS, 0, s>>>(acc + 3 * ncell, ncell, scratch);
+    compact_count_kernel<<<nb, kCompactBS, 0, s>>>(acc + (fx ? kAccW : 3) * ncell, ncell, scratch);
     compact_scan_kernel<<<1, 1024, 0, s>>>(scratch, nb, n_out);
-    compact_write_kernel<<<nb, kCompactBS, 0, s>>>(acc, facet, ncell, len, scratch, uu, vv, dre,
-                                                   dim, s2, cid, ctr, sum_w_dense);
+    if (fx)
+        compact_write_kernel<true><<<nb, kCompactBS, 0, s>>>(
+            acc, facet, ncell, len, scratch, uu, vv, dre, dim, s2, cid, ctr, sum_w_dense,
+            1.0 / fx->fx_scale, 1.0 / fx->fs_scale);
+    else
+        compact_write_kernel<false><<<nb, kCompactBS, 0, s>>>(
+            acc, facet, ncell, len, scratch, uu, vv, dre, dim, s2, cid, ctr, sum_w_dense, 1.0,
+            1.0);
 }
 
 // ------------------------------------------------------------------ likelihood (K3)

@@ -69,6 +69,12 @@ struct Obs {
     void *d_res_saved = nullptr;  // rl2_obs_checkpoint copy
     float *d_s2f = nullptr;
     double *d_wn = nullptr, *d_invlam = nullptr, *d_spec = nullptr;
+    // Bounds that fix the scales of the gridding's integer accumulators: res_bound >= max
+    // |re| + |im| of the residual (measured after every set-up change, raised by each
+    // subtraction's largest possible change), s2_max = largest sigma^2, spec_max = largest
+    // spectral factor (a model visibility is at most flux * spec_max).
+    double res_bound = 0., res_bound_saved = 0., s2_max = 0., spec_max = 0.;
+    bool measure_res_bound();
     std::vector<double> wn, spec, invlam;
     std::vector<FacetSlot> slots;
     FacetDev *d_facets = nullptr;

@@ -258,14 +258,16 @@ bool Obs::ensure_slots(int n)
     for (int i = old; i < n; i++) {
         FacetSlot &s = slots[i];
         // the cell arrays of the grid paths hold a zero pad cell after the last cell
-        if (!(dmalloc(&s.acc, 4 * ncell) && dmalloc(&s.uu, ncell) && dmalloc(&s.vv, ncell) &&
+        if (!(dmalloc(&s.acc, grid_acc_words(fmax)) && dmalloc(&s.uu, ncell) &&
+              dmalloc(&s.vv, ncell) &&
               dmalloc(&s.dre, ncell + 1) && dmalloc(&s.dim, ncell + 1) &&
               dmalloc(&s.s2, ncell) && dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
               dmalloc(&s.invs2_64, ncell + 1) && dmalloc(&s.cid, ncell + 1) &&
               dmalloc(&s.ctr, (size_t)fmax) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
-        if (!hip_ok(hipMemsetAsync(s.acc, 0, 4 * ncell * sizeof(double), g_stream), "memset"))
+        if (!hip_ok(hipMemsetAsync(s.acc, 0, grid_acc_words(fmax) * sizeof(double), g_stream),
+                    "memset"))
             return false;
     }
     FacetDev *nd = nullptr;
@@ -308,6 +310,24 @@ void Obs::facet_upload(int i)
     hipMemcpyAsync((char *)(d_facets + i) + off, (char *)&f + off, sizeof(FacetDev) - off,
                    hipMemcpyHostToDevice, g_stream);
     hipMemcpyAsync(d_facets + i, &f, offsetof(FacetDev, n), hipMemcpyHostToDevice, g_stream);
+}
+
+bool Obs::measure_res_bound()
+{
+    unsigned long long *d = nullptr, h = 0;
+    if (!dmalloc(&d, 1)) return false;
+    bool ok = hip_ok(hipMemsetAsync(d, 0, sizeof *d, g_stream), "memset") ;
+    if (ok) {
+        launch_abs_bound(g_stream, prec, nvis, d_res, d);
+        ok = hip_ok(hipGetLastError(), "abs_bound") &&
+             hip_ok(hipMemcpyAsync(&h, d, sizeof h, hipMemcpyDeviceToHost, g_stream), "copy") &&
+             hip_ok(hipStreamSynchronize(g_stream), "sync");
+    }
+    dfree(d);
+    double b;
+    memcpy(&b, &h, sizeof b);
+    res_bound = b;
+    return ok;
 }
 
 GalaxyGeom make_geom(double l0, double m0, double flux, double scale, double e1, double e2)
@@ -374,6 +394,11 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
     a.inc = 2 * o.len / facet;
     a.facet = facet;
     a.acc = s.acc;
+    // a visibility that is gridded is at most |round model| + |residual| per component
+    if (sub) o.res_bound += 3. * fabs(sub->flux) * o.spec_max;
+    const AccScale sc = fx_scales(fabs(flux) * o.spec_max + o.res_bound, o.s2_max);
+    a.fx_scale = sc.fx_scale;
+    a.fs_scale = sc.fs_scale;
     if (sub) {  // the previous galaxy's post-fit update in the same pass
         a.sub = 1;
         a.sfit = make_geom(sub->l0, sub->m0, sub->flux, sub->R_mu, sub->e1, sub->e2);
@@ -392,7 +417,7 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
     {
         KTimer t(RL2_K_COMPACT);
         launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
-                       s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan);
+                       s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan, &sc);
         launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
                           s.s2, s.pk, s.invs2, s.invs2_64);
     }
@@ -411,6 +436,7 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
 
 bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2)
 {
+    o.res_bound += 3. * fabs(flux) * o.spec_max;  // |fitted model - round model| per component
     GalaxyGeom fit = make_geom(l0, m0, flux, R_mu, e1, e2);
     GalaxyGeom rnd = make_geom(l0, m0, flux, R_mu, 0., 0.);
     {
@@ -911,7 +937,11 @@ int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const do
     o.wn.assign(wavenumbers, wavenumbers + nch);
     o.spec.assign(spec, spec + nch);
     o.invlam.resize(nch);
-    for (unsigned c = 0; c < nch; c++) o.invlam[c] = wavenumbers[c] / (2. * kPi);
+    for (unsigned c = 0; c < nch; c++) {
+        o.invlam[c] = wavenumbers[c] / (2. * kPi);
+        o.spec_max = fmax(o.spec_max, fabs(spec[c]));
+    }
+    for (size_t i = 0; i < o.nvis; i++) o.s2_max = fmax(o.s2_max, (double)fabsf(sigma2_vis[i]));
     hipStream_t s = g_stream;
     double *tu = nullptr, *tv = nullptr, *tw = nullptr;
     uint32_t *keys = nullptr, *idx = nullptr, *keys2 = nullptr, *idx2 = nullptr;
@@ -991,6 +1021,7 @@ int rl2_obs_set_vis(rl2_obs *obs, const double *visData, const double *visSkyMod
     }
     dfree(dD);
     dfree(dS);
+    ok = ok && o.measure_res_bound();
     return ok ? RL2_OK : RL2_ERR_HIP;
 }
 
@@ -1038,6 +1069,7 @@ int rl2_obs_add_sources(rl2_obs *obs, unsigned int ngal, const double *flux, con
     bool ok = hip_ok(hipGetLastError(), "add_sources") &&
               hip_ok(hipStreamSynchronize(g_stream), "sync");
     dfree(dg);
+    ok = ok && o.measure_res_bound();
     return ok ? RL2_OK : RL2_ERR_HIP;
 }
 
@@ -1046,7 +1078,8 @@ int rl2_obs_add_noise(rl2_obs *obs, double sigma2, uint64_t seed)
     if (!obs || !(sigma2 >= 0.)) return RL2_ERR_ARG;
     Obs &o = obs->o;
     launch_add_noise(g_stream, o.prec, o.nvis, sqrt(sigma2), seed, o.d_res);
-    return hip_ok(hipGetLastError(), "add_noise") && hip_ok(hipStreamSynchronize(g_stream), "sync")
+    return hip_ok(hipGetLastError(), "add_noise") &&
+                   hip_ok(hipStreamSynchronize(g_stream), "sync") && o.measure_res_bound()
                ? RL2_OK
                : RL2_ERR_HIP;
 }
@@ -1058,6 +1091,7 @@ int rl2_obs_checkpoint(rl2_obs *obs)
     const size_t bytes = o.nvis * (o.prec == RL2_PREC_FP64 ? 16 : 8);
     if (!o.d_res_saved && !hip_ok(hipMalloc(&o.d_res_saved, bytes), "hipMalloc checkpoint"))
         return RL2_ERR_HIP;
+    o.res_bound_saved = o.res_bound;
     return hip_ok(hipMemcpyAsync(o.d_res_saved, o.d_res, bytes, hipMemcpyDeviceToDevice, g_stream),
                   "checkpoint") && hip_ok(hipStreamSynchronize(g_stream), "sync")
                ? RL2_OK
@@ -1073,6 +1107,7 @@ int rl2_obs_restore(rl2_obs *obs)
         return RL2_ERR_STATE;
     }
     const size_t bytes = o.nvis * (o.prec == RL2_PREC_FP64 ? 16 : 8);
+    o.res_bound = o.res_bound_saved;
     return hip_ok(hipMemcpyAsync(o.d_res, o.d_res_saved, bytes, hipMemcpyDeviceToDevice, g_stream),
                   "restore") && hip_ok(hipStreamSynchronize(g_stream), "sync")
                ? RL2_OK
