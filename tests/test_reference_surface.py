@@ -335,8 +335,11 @@ def test_data_processing_writes_the_reference_lines(case):
     assert len(rows) == len(first) + len(second)
     for got, want in zip(rows, first + second):
         for k, (a, b) in enumerate(zip(got, want)):
-            if math.isnan(b):
-                assert math.isnan(a)
+            if k in (3, 6, 7) and (math.isnan(a) or math.isnan(b)):
+                # a square root of a variance that cancels to zero: its rounding residue may
+                # be +-1e-17 on either side, printed as 0.000000 or nan
+                other = b if math.isnan(a) else a
+                assert math.isnan(other) or abs(other) <= 1.5e-6, (k, got, want)
             else:
                 assert abs(a - b) <= 1.5e-6 + 1e-9 * abs(b), (k, got, want)
     assert np.all(vsk == 0)
