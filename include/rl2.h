@@ -192,9 +192,16 @@ int rl2_synchronize(void);
  * RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass, a multiple of 8 (each block
  *   then walks several chunks); 0 (default): one block per chunk.
  * RL2_TUNE_TILE_STREAMS: 2 splits the scalelength tasks of an evaluation in two groups
- *   whose Newton passes run on two streams; 1 (default) runs them all on one. */
+ *   whose Newton passes run on two streams; 1 (default) runs them all on one.
+ * RL2_TUNE_BATCH_EXTRACT: 1 (default) extracts a batch of 2..64 galaxies
+ *   (rl2_fit_options.batch) in one persistent pass over the residual, which also applies the
+ *   previous batch's subtractions; 0 extracts and subtracts one galaxy per pass.  Both give
+ *   identical results.
+ * RL2_TUNE_BATCH_GROUPS: persistent work-groups of a batched extraction pass (0: three per
+ *   compute unit). */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
-       RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5 };
+       RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5,
+       RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

@@ -77,7 +77,8 @@ struct AccScale {
     double fx_scale = 1., fs_scale = 1.;
 };
 // The scales for visibility components bounded by vis_bound and variances by s2_bound.
-AccScale fx_scales(double vis_bound, double s2_bound);
+// rows_log2 < 0: the rows of a work-group of the current gridding variant.
+AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2 = -1);
 // Words (8 bytes) of a facet's gridding accumulators.
 size_t grid_acc_words(unsigned facet);
 // *out = max(*out, max over the residual of |re| + |im|), as the bits of a double (*out
@@ -112,6 +113,48 @@ struct GridLaunch {
 };
 void launch_grid(hipStream_t s, const GridLaunch &a);
 
+// Batched extraction (data_processing-mpi.c:65-216 semantics: every galaxy of a batch is
+// extracted from the same residual).  One persistent launch streams the residual once for
+// the whole batch: work-groups take tiles of 1024 rows from a device work queue; for each
+// channel of a tile they first apply the pending post-fit updates of the previous batch, in
+// order (written back: the same arithmetic as that many subtraction passes), then grid the
+// tile into every target galaxy's facet in turn.
+constexpr int kMaxBatch = 64;
+struct BatchTarget {
+    GalaxyGeom g;               // round model and phase centre of the galaxy
+    double inc, inv_inc;        // cell width of its facet and its reciprocal
+    double fx_scale;            // AccScale of its accumulators
+    double *acc;                // grid_acc_words(facet) words
+    unsigned facet;
+    unsigned pad;
+};
+struct BatchSub {
+    GalaxyGeom fit, rnd;        // R <- (R - fit) + rnd (data_processing.c:109-135)
+};
+struct GridBatchLaunch {
+    int prec;
+    unsigned nch;
+    unsigned long nrow;
+    const double *u, *v, *w, *wn, *inv_lambda, *spec;
+    void *res;
+    const float *s2f;
+    double len, fs_scale;
+    unsigned ntarget, nsub;
+    const BatchTarget *targets; // device, ntarget entries
+    const BatchSub *subs;       // device, nsub entries (may be 0)
+    unsigned *queue;            // device: next tile, finished work-groups; zero before the
+                                // launch, and the launch leaves them zero again
+    unsigned ntiles;
+};
+constexpr unsigned kBatchTileRows = 1024;
+// groups: persistent work-groups to launch (at most the tile count).
+void launch_grid_batch(hipStream_t s, const GridBatchLaunch &a, unsigned groups);
+// R <- (R - fit_b) + rnd_b for b = 0 .. nsub-1, in order, per visibility.
+void launch_subtract_batch(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                           const double *u, const double *v, const double *w, const double *wn,
+                           const double *inv_lambda, const double *spec, const BatchSub *subs,
+                           unsigned nsub, void *res);
+
 // Launch-shape knobs (rl2_tune_set): read by the launchers on the host.
 struct Tuning {
     int grid_variant = 3;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
@@ -122,6 +165,10 @@ struct Tuning {
     unsigned tile_blocks = 0;  // RL2_TUNE_TILE_BLOCKS: cap on the blocks of a likelihood pass
                                // (multiple of 8; 0: one block per chunk)
     int tile_streams = 1;      // RL2_TUNE_TILE_STREAMS: task groups (streams) of the passes
+    int batch_extract = 1;     // RL2_TUNE_BATCH_EXTRACT: one residual pass per batch (1) or
+                               // one per galaxy (0)
+    unsigned batch_groups = 0; // RL2_TUNE_BATCH_GROUPS: persistent work-groups of a batched
+                               // extraction (0: three per CU)
 };
 Tuning &tuning();
 

@@ -586,6 +586,222 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
 
 size_t grid_acc_words(unsigned facet) { return (size_t)kAccWords * facet * facet; }
 
+// ------------------------------------------------------------------ batched extraction
+// One persistent pass over the residual for a whole batch (GridBatchLaunch).  A work-group
+// takes the next tile of kBatchTileRows sorted rows from the queue; each thread owns 4
+// consecutive rows.  Per channel: the thread's visibilities are loaded once, the pending
+// updates are applied in order and written back, then every target galaxy grids them with
+// the arithmetic of grid_run_kernel (runs of equal cells merged in registers, an LDS table
+// of integer sums, flushed to the target's accumulators before the next target).  No
+// work-group waits for another, so the launch may hold any number of them; the last one
+// to finish leaves the queue counters at zero for the next launch.
+template <int PREC>
+__global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
+{
+    constexpr int BS = 256, RPT = 4, HS = 1024, HBITS = 10;
+    static_assert(RPT * BS == kBatchTileRows, "tile rows");
+    __shared__ uint32_t hkey[HS];
+    __shared__ unsigned long long hre[HS], him[HS], hs2[HS];
+    __shared__ uint32_t hcnt[HS];
+    __shared__ uint32_t hlist[HS];
+    __shared__ uint32_t hn;
+    __shared__ unsigned s_tile;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < HS; s += BS) {
+        hkey[s] = kEmpty;
+        hre[s] = 0ull;
+        him[s] = 0ull;
+        hs2[s] = 0ull;
+        hcnt[s] = 0;
+    }
+    if (tid == 0) hn = 0;
+    while (true) {
+        if (tid == 0) s_tile = atomicAdd(&a.queue[0], 1u);
+        __syncthreads();
+        const unsigned tile = s_tile;
+        if (tile >= a.ntiles) break;
+        const unsigned long row0 = (unsigned long)tile * kBatchTileRows + (unsigned long)tid * RPT;
+        double ru[RPT], rv[RPT], rw[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const unsigned long i = row0 + k;
+            const bool in = i < a.nrow;
+            ru[k] = in ? a.u[i] : 0.;
+            rv[k] = in ? a.v[i] : 0.;
+            rw[k] = in ? a.w[i] : 0.;
+        }
+        for (unsigned c = 0; c < a.nch; c++) {
+            const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
+            const double wn2 = wn * wn, spectra = a.spec[c];
+            float vs2[RPT];
+            double2 vres[RPT];
+#pragma unroll
+            for (int k = 0; k < RPT; k++) {
+                const unsigned long i = row0 + k;
+                const size_t idx = (size_t)c * a.nrow + i;
+                vs2[k] = i < a.nrow ? a.s2f[idx] : -1.f;
+                vres[k] = i < a.nrow ? load_res(a.res, PREC, idx) : make_double2(0., 0.);
+            }
+            if (a.nsub) {
+                for (unsigned b = 0; b < a.nsub; b++) {
+                    const GalaxyGeom fit = a.subs[b].fit, rnd = a.subs[b].rnd;
+#pragma unroll
+                    for (int k = 0; k < RPT; k++) {
+                        const double p = ru[k] * fit.l0 + rv[k] * fit.m0 + rw[k] * fit.n0;
+                        vres[k] = subtract_vis(fit, rnd, p, ru[k], rv[k], wn2, spectra,
+                                               inv_lambda, vres[k]);
+                        // each update a pass of its own: a complex64 residual is stored
+                        // between them
+                        if (PREC != 0)
+                            vres[k] = make_double2((double)(float)vres[k].x,
+                                                   (double)(float)vres[k].y);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    const unsigned long i = row0 + k;
+                    if (i >= a.nrow) continue;
+                    store_res(a.res, PREC, (size_t)c * a.nrow + i, vres[k]);
+                }
+            }
+            for (unsigned t = 0; t < a.ntarget; t++) {
+                const BatchTarget T = a.targets[t];
+                uint32_t rkey = kEmpty, rn = 0;
+                double rre = 0., rim = 0., rs2 = 0.;
+                auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {
+                    uint32_t h = (key * 2654435761u) >> (32 - HBITS);
+                    while (true) {
+                        const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
+                        if (prev == kEmpty) hlist[atomicAdd(&hn, 1u)] = h;
+                        if (prev == kEmpty || prev == key) break;
+                        h = (h + 1) & (HS - 1);
+                    }
+                    atomicAdd(&hre[h], (unsigned long long)fx_round(xr, T.fx_scale));
+                    atomicAdd(&him[h], (unsigned long long)fx_round(xi, T.fx_scale));
+                    atomicAdd(&hs2[h], (unsigned long long)fx_round(xs, a.fs_scale));
+                    atomicAdd(&hcnt[h], n);
+                };
+#pragma unroll
+                for (int k = 0; k < RPT; k++) {
+                    const unsigned long i = row0 + k;
+                    if (i >= a.nrow) continue;
+                    const float s2 = vs2[k];
+                    if (signbit(s2)) continue;  // flagged
+                    const double u = ru[k], v = rv[k];
+                    uint32_t key;
+                    if (!cell_index_fast(u, v, inv_lambda, a.len, T.inc, T.inv_inc, T.facet, &key))
+                        continue;
+                    // visGal = round model + residual, shifted by -k(ul + vm + wn)
+                    // (source_extraction.c:51-65)
+                    const double p = u * T.g.l0 + v * T.g.m0 + rw[k] * T.g.n0;
+                    const double shape = model_shape_rsq(T.g, u, v, wn2, spectra);
+                    double sn, cs;
+                    sincos_rev64(p * inv_lambda, sn, cs);
+                    const double gre = shape * cs + vres[k].x;
+                    const double gim = shape * sn + vres[k].y;
+                    const double xr = gre * cs - gim * (-sn);
+                    const double xi = gre * (-sn) + gim * cs;
+                    if (key == rkey) {
+                        rre += xr;
+                        rim += xi;
+                        rs2 += (double)s2;
+                        rn++;
+                    } else {
+                        if (rn) emit(rkey, rre, rim, rs2, rn);
+                        rkey = key;
+                        rre = xr;
+                        rim = xi;
+                        rs2 = (double)s2;
+                        rn = 1;
+                    }
+                }
+                if (rn) emit(rkey, rre, rim, rs2, rn);
+                __syncthreads();
+                const unsigned long ncell = (unsigned long)T.facet * T.facet;
+                unsigned long long *acc = (unsigned long long *)T.acc;
+                double *acc_w = T.acc + kAccW * ncell;
+                const uint32_t nocc = hn;
+                for (uint32_t j = tid; j < nocc; j += BS) {
+                    const uint32_t s = hlist[j];
+                    const uint32_t key = hkey[s];
+                    fx_flush(acc + kAccReHi * ncell + key, acc + kAccReLo * ncell + key,
+                             (long long)hre[s]);
+                    fx_flush(acc + kAccImHi * ncell + key, acc + kAccImLo * ncell + key,
+                             (long long)him[s]);
+                    fx_flush(acc + kAccS2Hi * ncell + key, acc + kAccS2Lo * ncell + key,
+                             (long long)hs2[s]);
+                    unsafeAtomicAdd(&acc_w[key], (double)hcnt[s]);
+                    hkey[s] = kEmpty;
+                    hre[s] = 0ull;
+                    him[s] = 0ull;
+                    hs2[s] = 0ull;
+                    hcnt[s] = 0;
+                }
+                __syncthreads();
+                if (tid == 0) hn = 0;
+                __syncthreads();
+            }
+        }
+        __syncthreads();  // every thread has read s_tile before thread 0 takes the next tile
+    }
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(&a.queue[1], 1u) == gridDim.x - 1) {
+            atomicExch(&a.queue[0], 0u);
+            atomicExch(&a.queue[1], 0u);
+        }
+    }
+}
+
+void launch_grid_batch(hipStream_t s, const GridBatchLaunch &a, unsigned groups)
+{
+    groups = std::max(1u, std::min(groups, a.ntiles));
+    if (a.prec == 0)
+        grid_batch_kernel<0><<<groups, 256, 0, s>>>(a);
+    else
+        grid_batch_kernel<1><<<groups, 256, 0, s>>>(a);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void subtract_batch_kernel(
+    unsigned nch, unsigned long nrow, const double *__restrict__ u, const double *__restrict__ v,
+    const double *__restrict__ w, const double *__restrict__ wn,
+    const double *__restrict__ inv_lambda, const double *__restrict__ spec, const BatchSub *subs,
+    unsigned nsub, void *res)
+{
+    const unsigned long i = blockIdx.x * (unsigned long)blockDim.x + threadIdx.x;
+    if (i >= nrow) return;
+    const double ui = u[i], vi = v[i], wi = w[i];
+    for (unsigned c = 0; c < nch; c++) {
+        const double k = wn[c], k2 = k * k, spectra = spec[c];
+        const size_t idx = (size_t)c * nrow + i;
+        double2 r = load_res(res, PREC, idx);
+        for (unsigned b = 0; b < nsub; b++) {
+            const GalaxyGeom fit = subs[b].fit, rnd = subs[b].rnd;
+            const double p = ui * fit.l0 + vi * fit.m0 + wi * fit.n0;
+            r = subtract_vis(fit, rnd, p, ui, vi, k2, spectra, inv_lambda[c], r);
+            if (PREC != 0 && b + 1 < nsub)  // each update a pass of its own: stored as complex64
+                r = make_double2((double)(float)r.x, (double)(float)r.y);
+        }
+        store_res(res, PREC, idx, r);
+    }
+}
+
+void launch_subtract_batch(hipStream_t s, int prec, unsigned nch, unsigned long nrow,
+                           const double *u, const double *v, const double *w, const double *wn,
+                           const double *inv_lambda, const double *spec, const BatchSub *subs,
+                           unsigned nsub, void *res)
+{
+    if (nsub == 0) return;
+    dim3 grid((unsigned)((nrow + 255) / 256));
+    if (prec == 0)
+        subtract_batch_kernel<0><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, inv_lambda, spec,
+                                                      subs, nsub, res);
+    else
+        subtract_batch_kernel<1><<<grid, 256, 0, s>>>(nch, nrow, u, v, w, wn, inv_lambda, spec,
+                                                      subs, nsub, res);
+}
+
 // Rows of a gridding work-group (256 threads x rows per thread) for the current variant.
 static int grid_rows_log2()
 {
@@ -596,10 +812,10 @@ static int grid_rows_log2()
     }
 }
 
-AccScale fx_scales(double vis_bound, double s2_bound)
+AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2)
 {
     // 2^S with bound * 2^S * rows <= 2^62: a channel's slot sums stay below 2^62
-    const int rows_log2 = grid_rows_log2();
+    if (rows_log2 < 0) rows_log2 = grid_rows_log2();
     auto scale = [&](double b) {
         if (!(b > 0.) || !std::isfinite(b)) b = 1.;
         const int S = std::max(-1000, std::min(1000, 62 - rows_log2 - (ilogb(b) + 1)));

@@ -85,6 +85,10 @@ struct Obs {
     int loglik_prec = 0;     // precision of rl2_obs_loglik requests (0 FP64, 1 mixed)
     float *d_feat = nullptr; // learned-proposal inputs of a batch (training data)
     size_t feat_cap = 0;
+    // batched extraction: targets and pending updates of one launch, the work queue
+    BatchTarget *d_targets = nullptr;
+    BatchSub *d_subs = nullptr;
+    unsigned *d_queue = nullptr;
     ~Obs();
     bool ensure_slots(int n);
     void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device
@@ -116,6 +120,18 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
                  unsigned facet, bool want_sumw, double e1 = 0., double e2 = 0.,
                  const PendingSub *sub = nullptr);
 bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2);
+// One galaxy of a batched extraction into slot `slot`.
+struct ExtractReq {
+    int slot;
+    double l0, m0, flux, R_mu;
+    unsigned facet;
+};
+// Extract every request from one pass over the residual, after applying `subs` (in order)
+// in the same pass (data_processing-mpi.c:65-216 batch semantics).
+bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
+                       const std::vector<PendingSub> &subs);
+// Apply the updates in order in one pass over the residual.
+bool obs_subtract_batch(Obs &o, const std::vector<PendingSub> &subs);
 bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const double *l,
                   const double *m, const double *flux, const double *scale,
                   const rl2_fit_options &opt, rl2_gal_result *results,

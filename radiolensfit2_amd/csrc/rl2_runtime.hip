@@ -242,6 +242,9 @@ Obs::~Obs()
     }
     dfree(d_facets);
     dfree(d_feat);
+    dfree(d_targets);
+    dfree(d_subs);
+    dfree(d_queue);
     lik.release();
     dfree(lik.d_partials);
     lik.partials_cap = 0;
@@ -365,6 +368,31 @@ static bool ensure_stream2()
            hip_ok(hipEventCreateWithFlags(&g_join, hipEventDisableTiming), "hipEventCreate");
 }
 
+// Compaction of a slot's accumulators into its facet, the FP32 copy and the descriptor.
+static bool finish_extract(Obs &o, int slot, unsigned facet, bool want_sumw, const AccScale &sc)
+{
+    FacetSlot &s = o.slots[slot];
+    if (want_sumw && !s.sumw && !dmalloc(&s.sumw, (size_t)o.fmax * o.fmax)) return false;
+    {
+        KTimer t(RL2_K_COMPACT);
+        launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
+                       s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan, &sc);
+        launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
+                          s.s2, s.pk, s.invs2, s.invs2_64);
+    }
+    // the facet now has grid structure: cell ids and centres for the phase tables (packed
+    // in 16-bit row / column fields)
+    FacetDev &f = o.h_facets[slot];
+    const unsigned gf = facet < 65536u ? facet : 0u;
+    if (f.cid != s.cid || f.grid_facet != gf) {
+        f.cid = gf ? s.cid : nullptr;
+        f.ctr = gf ? s.ctr : nullptr;
+        f.grid_facet = gf;
+        o.facet_upload(slot);
+    }
+    return true;
+}
+
 bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_mu,
                  unsigned facet, bool want_sumw, double e1, double e2, const PendingSub *sub)
 {
@@ -413,25 +441,129 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
     stats_add_work(RL2_K_EXTRACT, 0., o.nvis * (bytes_per_vis(o, RL2_K_EXTRACT) +
                                                 (sub ? (o.prec == 0 ? 16. : 8.) : 0.)) +
                                           o.nrow * 24.);
-    if (want_sumw && !s.sumw && !dmalloc(&s.sumw, (size_t)o.fmax * o.fmax)) return false;
+    return finish_extract(o, slot, facet, want_sumw, sc) &&
+           hip_ok(hipGetLastError(), "extract launch");
+}
+
+static unsigned g_num_cus = 0;
+static unsigned num_cus()
+{
+    if (!g_num_cus) {
+        hipDeviceProp_t p;
+        g_num_cus = hipGetDeviceProperties(&p, g_device) == hipSuccess && p.multiProcessorCount > 0
+                        ? (unsigned)p.multiProcessorCount
+                        : 256u;
+    }
+    return g_num_cus;
+}
+
+bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
+                       const std::vector<PendingSub> &subs)
+{
+    const int nb = (int)reqs.size();
+    if (nb == 0 || nb > kMaxBatch || (int)subs.size() > kMaxBatch) {
+        set_error("batched extraction: 1 to " + std::to_string(kMaxBatch) +
+                  " galaxies and pending updates per pass");
+        return false;
+    }
+    int max_slot = 0;
+    for (const ExtractReq &r : reqs) {
+        if (r.facet == 0 || r.facet > o.fmax) {
+            set_error("facet size " + std::to_string(r.facet) +
+                      " outside the allocated facet (max " + std::to_string(o.fmax) + ")");
+            return false;
+        }
+        max_slot = std::max(max_slot, r.slot);
+    }
+    if (!o.ensure_slots(max_slot + 1)) return false;
+    if (!o.d_targets && !(dmalloc(&o.d_targets, kMaxBatch) && dmalloc(&o.d_subs, kMaxBatch) &&
+                          dmalloc(&o.d_queue, 2) &&
+                          hip_ok(hipMemsetAsync(o.d_queue, 0, 2 * sizeof(unsigned), g_stream),
+                                 "memset")))
+        return false;
+    for (const PendingSub &p : subs) o.res_bound += 3. * fabs(p.flux) * o.spec_max;
+    std::vector<BatchTarget> T(nb);
+    std::vector<AccScale> sc(nb);
+    for (int k = 0; k < nb; k++) {
+        const ExtractReq &r = reqs[k];
+        o.slots[r.slot].facet = r.facet;
+        T[k].g = make_geom(r.l0, r.m0, r.flux, r.R_mu, 0., 0.);
+        T[k].inc = 2 * o.len / r.facet;
+        T[k].inv_inc = 1.0 / T[k].inc;
+        sc[k] = fx_scales(fabs(r.flux) * o.spec_max + o.res_bound, o.s2_max, 10);
+        T[k].fx_scale = sc[k].fx_scale;
+        T[k].acc = o.slots[r.slot].acc;
+        T[k].facet = r.facet;
+        T[k].pad = 0;
+    }
+    std::vector<BatchSub> S(subs.size());
+    for (size_t b = 0; b < subs.size(); b++) {
+        const PendingSub &p = subs[b];
+        S[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
+        S[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
+    }
+    hipMemcpyAsync(o.d_targets, T.data(), nb * sizeof(BatchTarget), hipMemcpyHostToDevice,
+                   g_stream);
+    if (!S.empty())
+        hipMemcpyAsync(o.d_subs, S.data(), S.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
+                       g_stream);
+    GridBatchLaunch a = {};
+    a.prec = o.prec;
+    a.nch = o.nch;
+    a.nrow = o.nrow;
+    a.u = o.d_u;
+    a.v = o.d_v;
+    a.w = o.d_w;
+    a.wn = o.d_wn;
+    a.inv_lambda = o.d_invlam;
+    a.spec = o.d_spec;
+    a.res = o.d_res;
+    a.s2f = o.d_s2f;
+    a.len = o.len;
+    a.fs_scale = sc[0].fs_scale;
+    a.ntarget = (unsigned)nb;
+    a.nsub = (unsigned)S.size();
+    a.targets = o.d_targets;
+    a.subs = o.d_subs;
+    a.queue = o.d_queue;
+    a.ntiles = (unsigned)((o.nrow + kBatchTileRows - 1) / kBatchTileRows);
     {
-        KTimer t(RL2_K_COMPACT);
-        launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
-                       s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan, &sc);
-        launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
-                          s.s2, s.pk, s.invs2, s.invs2_64);
+        KTimer t(RL2_K_EXTRACT);
+        launch_grid_batch(g_stream, a,
+                          tuning().batch_groups ? tuning().batch_groups : 3 * num_cus());
     }
-    // the facet now has grid structure: cell ids and centres for the phase tables (packed
-    // in 16-bit row / column fields)
-    FacetDev &f = o.h_facets[slot];
-    const unsigned gf = facet < 65536u ? facet : 0u;
-    if (f.cid != s.cid || f.grid_facet != gf) {
-        f.cid = gf ? s.cid : nullptr;
-        f.ctr = gf ? s.ctr : nullptr;
-        f.grid_facet = gf;
-        o.facet_upload(slot);
+    const double res_b = o.prec == 0 ? 16. : 8.;
+    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * (res_b + 4. + (S.empty() ? 0. : res_b)) +
+                                          o.nrow * 24.);
+    for (int k = 0; k < nb; k++)
+        if (!finish_extract(o, reqs[k].slot, reqs[k].facet, false, sc[k])) return false;
+    return hip_ok(hipGetLastError(), "batched extraction launch");
+}
+
+bool obs_subtract_batch(Obs &o, const std::vector<PendingSub> &subs)
+{
+    if (subs.empty()) return true;
+    if ((int)subs.size() > kMaxBatch) {
+        set_error("batched subtraction: at most " + std::to_string(kMaxBatch) + " updates");
+        return false;
     }
-    return hip_ok(hipGetLastError(), "extract launch");
+    if (!o.d_subs && !dmalloc(&o.d_subs, kMaxBatch)) return false;
+    std::vector<BatchSub> S(subs.size());
+    for (size_t b = 0; b < subs.size(); b++) {
+        const PendingSub &p = subs[b];
+        o.res_bound += 3. * fabs(p.flux) * o.spec_max;
+        S[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
+        S[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
+    }
+    hipMemcpyAsync(o.d_subs, S.data(), S.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
+                   g_stream);
+    {
+        KTimer t(RL2_K_SUBTRACT);
+        launch_subtract_batch(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn,
+                              o.d_invlam, o.d_spec, o.d_subs, (unsigned)S.size(), o.d_res);
+    }
+    stats_add_work(RL2_K_SUBTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_SUBTRACT) + o.nrow * 24.);
+    return hip_ok(hipGetLastError(), "batched subtraction launch");
 }
 
 bool obs_subtract(Obs &o, double l0, double m0, double flux, double R_mu, double e1, double e2)
@@ -763,6 +895,10 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
     const bool fuse = tuning().fuse_subtract && tuning().grid_variant == 3;
     PendingSub pend;
     bool pending = false;
+    // batches extracted in one residual pass each, carrying the previous batch's
+    // subtractions (in order) in the same pass
+    const bool batched = B > 1 && B <= kMaxBatch && tuning().batch_extract;
+    std::vector<PendingSub> pend_list;
     o.lik_block = opt.lik_block > 0 ? opt.lik_block : 0;
     BatchEval eval = [&](const std::vector<LikItem> &items, std::vector<double> &Le,
                          std::vector<int> &err) {
@@ -771,7 +907,19 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
     for (size_t b0 = 0; b0 < todo.size(); b0 += B) {
         const size_t nb = std::min((size_t)B, todo.size() - b0);
         double t0 = now_s();
-        for (size_t k = 0; k < nb; k++) {
+        if (batched) {
+            std::vector<ExtractReq> reqs(nb);
+            for (size_t k = 0; k < nb; k++) {
+                const unsigned gal = todo[b0 + k];
+                prior_for(o, (int)k, flux[gal]);
+                const unsigned facet = facet_size(scale[gal], o.len);
+                reqs[k] = {(int)k, l[gal], m[gal], flux[gal], scale[gal], facet};
+                results[gal].facet = facet;
+            }
+            if (!obs_extract_batch(o, reqs, pend_list)) return false;
+            pend_list.clear();
+        }
+        for (size_t k = 0; k < nb && !batched; k++) {
             const unsigned gal = todo[b0 + k];
             prior_for(o, (int)k, flux[gal]);
             const unsigned facet = facet_size(scale[gal], o.len);
@@ -817,7 +965,9 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
                     r.e2 = 0.;
                 }
             } else {
-                if (fuse && more && (int)k == last_good) {
+                if (batched) {
+                    pend_list.push_back({l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2});
+                } else if (fuse && more && (int)k == last_good) {
                     pend = {l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2};
                     pending = true;
                 } else if (!obs_subtract(o, l[gal], m[gal], flux[gal], scale[gal], r.e1, r.e2)) {
@@ -832,7 +982,8 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
     }
     if (pending && !obs_subtract(o, pend.l0, pend.m0, pend.flux, pend.R_mu, pend.e1, pend.e2))
         return false;
-    return true;
+    if (!obs_subtract_batch(o, pend_list)) return false;
+    return hip_ok(hipStreamSynchronize(g_stream), "subtract sync");
 }
 
 bool process(Obs &o, unsigned ngal, const double *l, const double *m, const double *flux,
@@ -1338,6 +1489,14 @@ int rl2_tune_set(int knob, int value)
         if (value < 0 || value % 8 || value > (1 << 20)) break;
         tuning().tile_blocks = (unsigned)value;
         return RL2_OK;
+    case RL2_TUNE_BATCH_EXTRACT:
+        if (value < 0 || value > 1) break;
+        tuning().batch_extract = value;
+        return RL2_OK;
+    case RL2_TUNE_BATCH_GROUPS:
+        if (value < 0 || value > 65536) break;
+        tuning().batch_groups = (unsigned)value;
+        return RL2_OK;
     case RL2_TUNE_TILE_STREAMS:
         if (value != 1 && value != 2) break;
         tuning().tile_streams = value;
@@ -1360,6 +1519,8 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_FUSE_SUBTRACT: *value = tuning().fuse_subtract; return RL2_OK;
     case RL2_TUNE_TILE_BLOCKS: *value = (int)tuning().tile_blocks; return RL2_OK;
     case RL2_TUNE_TILE_STREAMS: *value = tuning().tile_streams; return RL2_OK;
+    case RL2_TUNE_BATCH_EXTRACT: *value = tuning().batch_extract; return RL2_OK;
+    case RL2_TUNE_BATCH_GROUPS: *value = (int)tuning().batch_groups; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

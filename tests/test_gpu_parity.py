@@ -261,3 +261,29 @@ def test_gridding_is_deterministic(dev, case):
     for k in ("e1", "e2", "var_e1", "var_e2", "oneDimvar", "maxL", "error", "pass_"):
         np.testing.assert_array_equal(r0[k], r1[k])
     np.testing.assert_array_equal(v0, v1)
+
+
+@pytest.mark.parametrize("prec,batch", [(0, 4), (0, 16), (1, 4)])
+def test_batched_extraction_matches_per_galaxy_passes(gpu, case, prec, batch):
+    """A batch extracted in one persistent pass over the residual, which also applies the
+    previous batch's subtractions (RL2_TUNE_BATCH_EXTRACT 1), gives the results and residual
+    of one extraction and one subtraction pass per galaxy (0), bit for bit, for complex128
+    and complex64 residuals."""
+    obs, cat, vs = case
+    H = N.hip()
+    out = {}
+    try:
+        for be in (1, 0):
+            N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_EXTRACT, be), "rl2_tune_set")
+            print(f"process prec {prec} batch {batch} batched extraction {be}", flush=True)
+            d = P.DeviceObservation(obs, precision=prec, vis_sky=vs)
+            res, st = d.process(cat, batch=batch)
+            out[be] = (P.results_array(res), d.residual())
+            d.close()
+    finally:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_EXTRACT, 1), "rl2_tune_set")
+    (ra, va), (rb, vb) = out[1], out[0]
+    for k in ("e1", "e2", "var_e1", "var_e2", "oneDimvar", "maxL", "error", "pass_",
+              "subtracted", "ncoords"):
+        np.testing.assert_array_equal(ra[k], rb[k])
+    np.testing.assert_array_equal(va, vb)
