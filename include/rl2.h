@@ -155,6 +155,20 @@ int rl2_process(rl2_obs *obs, unsigned int ngal, const double *l, const double *
                 const double *flux, const double *scale, const rl2_fit_options *opt,
                 rl2_gal_result *results, rl2_run_stats *stats);
 
+/* ---- allocation-free entry point ----
+ * No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops).  marginalise_posterior_r
+ * (marginalise_r.c:81-223, as the function of the same name below) on device arrays:
+ * xmarvals, ymarvals: num_marvals doubles each; out: 1 + num_marvals doubles receiving the
+ * log of the integral, then ymarvals shifted by their maximum (the in-place update of the
+ * reference).  Its QUADPACK workspace is a fixed 40 KB array per device held until the
+ * process ends, so num_marvals is limited to 3..32 (RL2_ERR_RANGE beyond).  Queued on
+ * `stream` (NULL: the library's stream) without waiting; calls that overlap on one device
+ * share that workspace and race.  Calls no allocator; rl2_init must have run (it looks the
+ * workspace up), else RL2_ERR_STATE. */
+int rl2_marginalise_posterior_r_noalloc(int num_marvals, const double *xmarvals,
+                                        const double *ymarvals, double *out, void *stream);
+
 /* ---- scalelength grid and the fixed constants (RadioLensfit2.cpp:370-392) ---- */
 int rl2_r_grid(double *ro /* >= 29 */);
 

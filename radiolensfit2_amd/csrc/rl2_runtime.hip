@@ -100,6 +100,30 @@ void stats_add_work(int kernel, double flops, double bytes)
     g_stats[kernel].bytes += bytes;
 }
 
+// Scratch of the allocation-free entry point rl2_marginalise_posterior_r_noalloc: the
+// QUADPACK interval lists of one scalelength integration (kQagDoubles doubles, then
+// kQagInts ints).  Each device holds its own copy until the process ends; host code uses
+// only the address looked up once per device at set-up (g_scratch), never the array's name.
+constexpr size_t kScratchBytes = kQagDoubles * sizeof(double) + kQagInts * sizeof(int);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch[kMaxDevices] = {};
+
+// With device `dev` current: its copy of the scratch array.
+static bool init_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device index beyond the scratch table");
+        return false;
+    }
+    if (g_scratch[dev]) return true;
+    void *p = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch"))
+        return false;
+    g_scratch[dev] = p;
+    return true;
+}
+
 static bool ensure_init()
 {
     if (g_stream) return true;
@@ -110,7 +134,7 @@ static bool ensure_init()
     }
     if (g_device < 0) g_device = 0;
     if (!hip_ok(hipSetDevice(g_device), "hipSetDevice")) return false;
-    return hip_ok(hipStreamCreate(&g_stream), "hipStreamCreate");
+    return init_scratch(g_device) && hip_ok(hipStreamCreate(&g_stream), "hipStreamCreate");
 }
 
 template <class T>
@@ -1057,6 +1081,29 @@ int rl2_r_grid(double *ro)
         nRo++;
     }
     return nRo;
+}
+
+int rl2_marginalise_posterior_r_noalloc(int num_marvals, const double *xmarvals,
+                                        const double *ymarvals, double *out, void *stream)
+{
+    if (!g_stream) {  // set-up (stream, scratch address) is rl2_init's, not this call's
+        set_error("rl2_marginalise_posterior_r_noalloc: call rl2_init first");
+        return RL2_ERR_STATE;
+    }
+    if (num_marvals < 3 || num_marvals > kMaxR) {
+        set_error("rl2_marginalise_posterior_r_noalloc: " + std::to_string(num_marvals) +
+                  " points; 3 to " + std::to_string(kMaxR) + " fit its fixed scratch");
+        return RL2_ERR_RANGE;
+    }
+    if (!xmarvals || !ymarvals || !out) {
+        set_error("rl2_marginalise_posterior_r_noalloc: null device pointer");
+        return RL2_ERR_ARG;
+    }
+    double *ws = (double *)g_scratch[g_device];
+    int *iws = (int *)(ws + kQagDoubles);
+    launch_marginalise_r(stream ? (hipStream_t)stream : g_stream, num_marvals, xmarvals,
+                         ymarvals, out, ws, iws);
+    return hip_ok(hipGetLastError(), "marginalise_posterior_r launch") ? RL2_OK : RL2_ERR_HIP;
 }
 
 int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const double *u,
