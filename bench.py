@@ -55,12 +55,12 @@ def subset(cat, lo, hi, stride=1):
                            ("flux", "scale", "e1", "e2", "l", "m", "snr")))
 
 
-def make_geometry_and_catalog(cfg, ngal, seed=1, shear=(0.03, 0.0)):
+def make_geometry_and_catalog(cfg, ngal, seed=1, shear=(0.03, 0.0), fmin=10.0):
     t0 = time.time()
     u, v, w, wn, spec, sigma2, flag, len_lambda = synth.make_geometry(cfg)
     obs = synth.Observation(cfg, u, v, w, wn, spec, sigma2, flag, len_lambda,
                             np.zeros(0, np.complex128))
-    cat = synth.make_catalog(ngal, seed=seed, g=shear)
+    cat = synth.make_catalog(ngal, seed=seed, fmin=fmin, g=shear)
     log(f"[bench] geometry + catalog: {time.time() - t0:.1f}s nrow={obs.nrow} nvis={obs.nvis}")
     return obs, cat
 
@@ -283,6 +283,168 @@ def run_compare(ngal=100, batch=16, log_prefix="[compare]"):
     return out
 
 
+def timed_process(dev, cat, opts, warm=None):
+    """The whole loop over `cat` from the checkpointed residual, timed (after an optional
+    warm-up on `warm` galaxies, restored)."""
+    if warm:
+        dev.restore()
+        dev.process(subset(cat, 0, warm), **dict(opts, verbose=0))
+    dev.restore()
+    N.check(N.hip().rl2_synchronize(), "sync")
+    t0 = time.perf_counter()
+    res, st = dev.process(cat, **opts)
+    N.check(N.hip().rl2_synchronize(), "sync")
+    return res, st, time.perf_counter() - t0
+
+
+def oracle_first_batch(obs, cat, resid, res, ks, log_prefix):
+    """Galaxies of the first batch (extracted from the initial residual) re-extracted and
+    fitted on the FP64 oracle from that residual; their first-pass GPU fits are compared."""
+    from oracle import oracle as O
+    zeros = np.zeros_like(resid)
+    out = []
+    for k in ks:
+        t0 = time.time()
+        facet = N.hip().facet_size(cat.scale[k], obs.len)
+        fv, fs, uu, vv, sw = O.extract(obs, cat.l[k], cat.m[k], cat.flux[k], cat.scale[k], facet,
+                                       resid, zeros)
+        ro, numr = O.r_grid()
+        ro40 = np.zeros(40)
+        ro40[:numr] = ro
+        par, keep = O.make_params(fv, fs, uu, vv, ro40, O.rprior_for(cat.flux[k], ro40, numr),
+                                  numr)
+        vals = [C.c_double() for _ in range(6)]
+        err = O.lib().orl2_source_fitting(C.byref(par), *[C.byref(v) for v in vals])
+        e1, e2 = vals[0].value, vals[1].value
+        de = max(abs(res[k].e1 - e1), abs(res[k].e2 - e2))
+        out.append({"galaxy": int(k), "flux": float(cat.flux[k]), "cells": len(fv),
+                    "abs_de": de, "error_equal": bool(err == res[k].error)})
+        log(f"{log_prefix} oracle galaxy {k}: flux {cat.flux[k]:.1f} cells {len(fv)} |de| "
+            f"{de:.2e} ({time.time() - t0:.1f}s)")
+    return out
+
+
+def run_c4(args):
+    """Batched variant: 20,000 galaxies on the 197-dish array with 64 channels of 15 MHz
+    (19,306 baselines x 480 times x 64 = 593M visibilities), fluxes 5-200 uJy, batches of 64
+    extracted by one persistent pass each (graph-replayed), launch knobs from the
+    autotuner."""
+    cfg = synth.ArrayConfig(n_dish=197, kind=1, r_max=150e3, n_chan=64, bw_hz=15e6)
+    ngal = args.c4_ngal
+    B = args.batch if args.batch > 1 else 64
+    obs, cat = make_geometry_and_catalog(cfg, ngal, seed=1, shear=(0.03, 0.0), fmin=5.0)
+    dev = make_device_observation(obs, cat, N.RL2_PREC_FP64)
+    opts = dict(batch=B, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0,
+                lik_precision=0, verbose=max(1, ngal // (B * 20)))
+    t0 = time.time()
+    tune = P.autotune(dev, subset(cat, 0, 2 * B), **dict(opts, verbose=0))
+    opts["lik_block"] = tune["chosen"]["lik_block"]
+    log(f"[c4] autotune: {tune['chosen']} ({len(tune['trials'])} trials, "
+        f"{time.time() - t0:.1f}s)")
+    resid0 = dev.residual() if args.cpu_sample > 0 else None
+    res, st, dt = timed_process(dev, cat, opts, warm=B)
+    log(f"[c4] batched {B}: {ngal} galaxies in {dt:.1f}s -> {ngal / dt:.2f} gal/s, bad "
+        f"{st.n_bad_first}/{st.n_bad_final}")
+    nsub = min(ngal, args.c4_plain)
+    sub = subset(cat, 0, nsub)
+    plain_opts = dict(opts, batch=1, verbose=0)
+    _, _, dtp = timed_process(dev, sub, plain_opts, warm=4)
+    _, _, dtb = timed_process(dev, sub, dict(opts, verbose=0), warm=B)
+    # kernel times from HIP events over the same batched subset, re-run
+    P.stats_enable(True)
+    P.stats_reset()
+    timed_process(dev, sub, dict(opts, verbose=0))
+    kst = P.stats()
+    P.stats_enable(False)
+    log(f"[c4] first {nsub} galaxies: plain {nsub / dtp:.2f} gal/s, batched {nsub / dtb:.2f}")
+    ext = kst["extract"]
+    lik = kst["likelihood"]
+    out = {"workload": f"197 dishes, {obs.nrow // cfg.n_times} baselines x {cfg.n_times} times x "
+                       f"{cfg.n_chan} ch ({obs.nvis / 1e6:.1f}M vis), {ngal} galaxies 5-200 uJy, "
+                       f"batches of {B}, whole loop with re-fit",
+           "galaxies_per_s": round(ngal / dt, 3), "seconds": round(dt, 2),
+           "bad_fits": [st.n_bad_first, st.n_bad_final],
+           "subset_first_galaxies": nsub,
+           "subset_plain_galaxies_per_s": round(nsub / dtp, 3),
+           "subset_batched_galaxies_per_s": round(nsub / dtb, 3),
+           "subset_speedup": round(dtp / dtb, 3),
+           "autotune": tune,
+           "extraction": {"kernel": "grid_batch_kernel + batched compaction (hipGraph)",
+                          "launches": ext["launches"], "ms": round(ext["ms"], 2),
+                          "achieved_GBs": round(ext["bytes"] / max(1e-9, ext["ms"] * 1e-3) / 1e9, 1),
+                          "frac_of_hbm_peak": round(ext["bytes"] / max(1e-9, ext["ms"] * 1e-3)
+                                                    / 1e9 / PEAK_HBM_GBS, 4)},
+           "likelihood": {"launches": lik["launches"], "ms": round(lik["ms"], 2),
+                          "tflops_eq": round(lik["flops"] / max(1e-9, lik["ms"] * 1e-3) / 1e12, 3)}}
+    if args.cpu_sample > 0:
+        ks = [k for k in range(min(B, ngal)) if res[k].pass_ == 0][:args.cpu_sample]
+        ks = ks[::max(1, len(ks) // 4)][:4]
+        out["oracle_first_batch"] = oracle_first_batch(obs, cat, resid0, res, ks, "[c4]")
+    dev.close()
+    return out
+
+
+def run_c3(args):
+    """Precision variant on the 64-dish observation with 5,000 galaxies: the FP32 likelihood
+    (FP64 phase reduction, FP64 sums, FP64 position and scalelength marginalisation) against
+    the FP64 path over the same whole loop, per galaxy, and the FP64 path against the oracle
+    on the first galaxies."""
+    ngal = args.c3_ngal
+    obs, cat = make_geometry_and_catalog(synth.ARRAY_64, ngal, seed=1, shear=(0.02, -0.01))
+    dev = make_device_observation(obs, cat, N.RL2_PREC_FP64)
+    base = dict(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0,
+                verbose=max(1, ngal // 20))
+    out = {"workload": f"64 dishes, {obs.nrow} rows x {obs.nch} ch ({obs.nvis / 1e6:.2f}M vis), "
+                       f"{ngal} galaxies, whole loop with re-fit", "paths": {}}
+    runs = {}
+    for name, prec in (("fp64", 0), ("fp32_likelihood", 1), ("fp32_sampling_guarded", 4)):
+        P.stats_enable(True)
+        P.stats_reset()
+        res, st, dt = timed_process(dev, cat, dict(base, lik_precision=prec), warm=4)
+        kst = P.stats()
+        P.stats_enable(False)
+        runs[name] = res
+        lik = kst["likelihood"]
+        tf = lik["flops"] / max(1e-9, lik["ms"] * 1e-3) / 1e12
+        out["paths"][name] = {"galaxies_per_s": round(ngal / dt, 2), "lik_precision": prec,
+                              "likelihood_tflops_eq": round(tf, 3),
+                              "likelihood_frac_of_peak": round(
+                                  tf / (PEAK_FP64_TFLOPS if prec == 0 else PEAK_FP32_TFLOPS), 4),
+                              "bad_fits": [st.n_bad_first, st.n_bad_final]}
+        log(f"[c3] {name}: {ngal / dt:.1f} gal/s")
+    ref = runs["fp64"]
+    for name in ("fp32_likelihood", "fp32_sampling_guarded"):
+        r = runs[name]
+        de = np.array([max(abs(r[k].e1 - ref[k].e1), abs(r[k].e2 - ref[k].e2)) for k in range(ngal)])
+        dl = np.array([abs(r[k].maxL - ref[k].maxL) / max(1.0, abs(ref[k].maxL))
+                       for k in range(ngal) if r[k].error == 0 and ref[k].error == 0])
+        same = np.array([r[k].error == ref[k].error for k in range(ngal)])
+        ok = (de <= 1e-4) & same
+        p = out["paths"][name]
+        p.update({"within_1e-4_of_fp64": round(float(ok.mean()), 4),
+                  "abs_de_quantiles": {q: float(np.quantile(de, q)) for q in (0.5, 0.9, 0.99, 1.0)},
+                  "rel_dlogL_quantiles": {q: float(np.quantile(dl, q)) for q in (0.5, 0.9, 0.99, 1.0)}
+                  if len(dl) else None,
+                  "error_flags_equal": round(float(same.mean()), 4),
+                  "throughput_ratio_vs_fp64": round(p["galaxies_per_s"] /
+                                                    out["paths"]["fp64"]["galaxies_per_s"], 3),
+                  "galaxies_per_s_within_gate": round(p["galaxies_per_s"] * float(ok.mean()), 2)})
+        log(f"[c3] {name}: within 1e-4 of FP64 {ok.mean():.3f}, median |de| {np.median(de):.2e}")
+    if args.cpu_sample > 0:
+        n = min(ngal, args.c3_oracle)
+        dev.restore()
+        resid0 = dev.residual()
+        tc, ores = cpu_baseline(obs, cat, resid0, n)
+        gres, _, _ = timed_process(dev, subset(cat, 0, n), dict(base, lik_precision=0, verbose=0))
+        ok, de, same = gate(gres, ores)
+        out["fp64_vs_oracle_first_galaxies"] = {
+            "galaxies": n, "within_1e-4": int(ok.sum()), "max_abs_de": float(de.max()),
+            "error_flags_equal": bool(same.all()), "oracle_galaxies_per_s": round(n / tc, 3)}
+        log(f"[c3] FP64 GPU vs oracle on the first {n} galaxies: within 1e-4 {ok.mean():.3f}")
+    dev.close()
+    return out
+
+
 def run_config1(ngal=100):
     """The reference's CPU-runnable case: 64 dishes, 480 times x 2016 baselines x 4 channels
     of 60 MHz, 100 galaxies (seed 1, shear (0.02, -0.01)), the whole loop on the GPU (FP64)
@@ -388,6 +550,16 @@ def main():
                          "batched / mixed / learned, each gated against the FP64 oracle) and "
                          "print its JSON line")
     ap.add_argument("--compare-batch", type=int, default=16)
+    ap.add_argument("--workload", choices=["default", "c3", "c4"], default="default",
+                    help="c3: precision variant on the 64-dish array (5,000 galaxies); c4: "
+                         "batched variant, 20,000 galaxies on 593M visibilities; each prints "
+                         "its own JSON line")
+    ap.add_argument("--c4-ngal", type=int, default=20000)
+    ap.add_argument("--c4-plain", type=int, default=1000,
+                    help="galaxies of the C4 catalog also timed on the plain path")
+    ap.add_argument("--c3-ngal", type=int, default=5000)
+    ap.add_argument("--c3-oracle", type=int, default=300,
+                    help="C3 galaxies run as one loop on the FP64 oracle")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (the reference kernel "
                          "for scaling FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -410,6 +582,12 @@ def main():
 
     if args.compare:
         print(json.dumps({"comparison": run_compare(batch=args.compare_batch)}), flush=True)
+        return
+    if args.workload == "c4":
+        print(json.dumps({"c4": run_c4(args)}), flush=True)
+        return
+    if args.workload == "c3":
+        print(json.dumps({"c3": run_c3(args)}), flush=True)
         return
 
     cfg = synth.ARRAY_197 if args.config == "197dish" else synth.ARRAY_64

@@ -74,7 +74,7 @@ typedef struct {
     int lik_block;         /* 0 (default): tiled likelihood, every (e, r) task split over
                               work-groups of 2048 facet cells; 64..1024: one work-group
                               of that many threads per (e, r) task */
-    int verbose;
+    int verbose;           /* > 0: a progress line on stderr every `verbose` batches */
     int lik_precision;     /* 0: FP64 likelihood throughout; 1: mixed-precision likelihood
                               throughout; 2: mixed for the simplex, FP64 for the sampling;
                               3: FP64 simplex, mixed sampling; 4: as 3, with samples whose
@@ -196,7 +196,7 @@ int rl2_synchronize(void);
  *   in integers, so the facet does not depend on the variant or on the order of the
  *   work-groups.
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
- *   in [2048, 65536], or 0 (default) to choose per launch.
+ *   in [256, 65536], or 0 (default) to choose per launch (2048 to 65536).
  * RL2_TUNE_PHASE_TABLES: 2 (default) evaluates the FP64 Newton phases on extracted facets
  *   as products of per-pass column and row tables, both staged in LDS when they fit (40
  *   bytes per grid index within 40 KB); 1 stages only the column table; 0 evaluates
@@ -212,12 +212,31 @@ int rl2_synchronize(void);
  *   previous batch's subtractions; 0 extracts and subtracts one galaxy per pass.  Both give
  *   identical results.
  * RL2_TUNE_BATCH_GROUPS: persistent work-groups of a batched extraction pass (0: three per
- *   compute unit). */
+ *   compute unit).
+ * RL2_TUNE_BATCH_GRAPH: 1 (default) replays a batch's extraction launches (the residual
+ *   pass, compactions and FP32 copies) as a hipGraph captured once per batch shape; 0 launches
+ *   them one by one.  Both give identical results. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
        RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5,
-       RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7 };
+       RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7, RL2_TUNE_BATCH_GRAPH = 8 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
+
+/* Autotuner: times the candidate values of each launch knob in turn (coordinate search) on
+ * the first pass of the loop over the given galaxies, from the observation's current
+ * residual, and sets the fastest; the residual is unchanged on return.  Knobs: the batched
+ * extraction's work-groups (batch > 1) or the gridding variant, the likelihood's facet cells
+ * per work-group (RL2_TUNE_TILE_CELLS), the split of the scalelength tasks over two streams
+ * and, as knob -1, the work-group threads per (e, r) task (0: the tiled engine), returned in
+ * *lik_block for rl2_fit_options.lik_block.  Every knob leaves results unchanged up to
+ * rounding.  trials (optional) receives up to max_trials timed trials. */
+typedef struct {
+    int knob, value;
+    double seconds;
+} rl2_tune_trial;
+int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double *m,
+                 const double *flux, const double *scale, const rl2_fit_options *opt,
+                 rl2_tune_trial *trials, int max_trials, int *ntrials, int *lik_block);
 
 /* ---- test hooks (host-side; no GPU needed) ----
  * The fit control flow driven by a caller-supplied log-likelihood; returns the number of

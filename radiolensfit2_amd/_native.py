@@ -34,6 +34,7 @@ RL2_TUNE_TILE_BLOCKS = 4
 RL2_TUNE_TILE_STREAMS = 5
 RL2_TUNE_BATCH_EXTRACT = 6
 RL2_TUNE_BATCH_GROUPS = 7
+RL2_TUNE_BATCH_GRAPH = 8
 (RL2_K_EXTRACT, RL2_K_COMPACT, RL2_K_LIKELIHOOD, RL2_K_RMARG, RL2_K_SUBTRACT,
  RL2_K_SOURCES) = range(6)
 KERNEL_NAMES = ["extract", "compact", "likelihood", "rmarg", "subtract", "sources"]
@@ -59,6 +60,10 @@ class FitOptions(C.Structure):
                 ("lik_precision", C.c_int), ("proposal", C.c_void_p),
                 ("features_out", C.POINTER(C.c_float)), ("labels_out", C.POINTER(C.c_double)),
                 ("proposal_mode", C.c_int)]
+
+
+class TuneTrial(C.Structure):
+    _fields_ = [("knob", C.c_int), ("value", C.c_int), ("seconds", C.c_double)]
 
 
 class KernelStat(C.Structure):
@@ -133,6 +138,9 @@ _HIP_SIGS = {
     "rl2_test_cell_index": (C.c_int, [C.c_int, c_double_p, c_double_p, C.c_double, C.c_double,
                                       C.c_double, C.c_uint, c_uint_p, c_uint_p]),
     "rl2_tune_get": (C.c_int, [C.c_int, c_int_p]),
+    "rl2_autotune": (C.c_int, [C.c_void_p, C.c_uint, c_double_p, c_double_p, c_double_p,
+                               c_double_p, C.POINTER(FitOptions), C.POINTER(TuneTrial), C.c_int,
+                               c_int_p, c_int_p]),
     "rl2_fit_with_callback": (C.c_int, [LOGLIK_CB, C.c_void_p, C.c_int, C.c_int,
                                         C.POINTER(GalResult)]),
     # learned proposal

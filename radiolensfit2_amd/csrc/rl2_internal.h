@@ -131,6 +131,30 @@ struct BatchTarget {
 struct BatchSub {
     GalaxyGeom fit, rnd;        // R <- (R - fit) + rnd (data_processing.c:109-135)
 };
+// Compaction of one target's accumulators into its facet slot.
+struct BatchCompact {
+    double *acc;
+    double fx_unscale, fs_unscale;
+    double *uu, *vv, *dre, *dim, *s2;
+    uint32_t *cid;
+    double *ctr;
+    float4 *pk;
+    float *invs2;
+    double *invs2_64;
+    unsigned *n_out;            // the slot descriptor's cell count
+    unsigned *scan;             // block counts, compact_scratch_words(facet) words
+    unsigned facet;
+    unsigned pad;
+};
+// Everything that changes from one batch to the next, in device memory, so that the same
+// launches (and a graph of them) serve every batch.
+struct BatchDesc {
+    unsigned ntarget, nsub;     // galaxies of the batch, pending updates applied first
+    unsigned pad[2];
+    BatchTarget targets[kMaxBatch];
+    BatchSub subs[kMaxBatch];
+    BatchCompact compact[kMaxBatch];
+};
 struct GridBatchLaunch {
     int prec;
     unsigned nch;
@@ -139,13 +163,15 @@ struct GridBatchLaunch {
     void *res;
     const float *s2f;
     double len, fs_scale;
-    unsigned ntarget, nsub;
-    const BatchTarget *targets; // device, ntarget entries
-    const BatchSub *subs;       // device, nsub entries (may be 0)
+    const BatchDesc *desc;      // device
     unsigned *queue;            // device: next tile, finished work-groups; zero before the
                                 // launch, and the launch leaves them zero again
     unsigned ntiles;
 };
+// The batch's compactions and FP32 copies: four launches over ncap targets of facets up to
+// fmax (targets beyond desc->ntarget and blocks beyond a facet exit at once).
+void launch_compact_batch(hipStream_t s, const BatchDesc *desc, unsigned ncap, unsigned fmax,
+                          double len);
 constexpr unsigned kBatchTileRows = 1024;
 // groups: persistent work-groups to launch (at most the tile count).
 void launch_grid_batch(hipStream_t s, const GridBatchLaunch &a, unsigned groups);
@@ -169,6 +195,8 @@ struct Tuning {
                                // one per galaxy (0)
     unsigned batch_groups = 0; // RL2_TUNE_BATCH_GROUPS: persistent work-groups of a batched
                                // extraction (0: three per CU)
+    int batch_graph = 1;       // RL2_TUNE_BATCH_GRAPH: replay a batch's extraction launches
+                               // as a hipGraph (1) or launch them one by one (0)
 };
 Tuning &tuning();
 
@@ -237,6 +265,7 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 // r-marginalisation follows as its own launch.
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
+constexpr unsigned kTileKnobMinCells = 256;   // smallest chunk RL2_TUNE_TILE_CELLS accepts
 constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
 constexpr int kInlItems = 8;
 constexpr unsigned kTileLdsBytes = 40960;  // column-table staging per work-group (<= 1706 columns)

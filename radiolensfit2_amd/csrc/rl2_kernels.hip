@@ -527,15 +527,16 @@ __device__ inline void acc_clear(double *acc, unsigned long ncell, unsigned long
     }
 }
 
+// One work-group's tile of the compaction (block `blk` of the nb blocks of one facet).
 template <bool FX>
-__global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
-    double *acc, unsigned facet, unsigned long ncell, double len, const unsigned *boff,
-    double *uu, double *vv, double *dre, double *dim, double *s2o, uint32_t *cid, double *ctr,
-    double *sum_w_dense, double fx_unscale, double fs_unscale)
+__device__ inline void compact_write_tile(
+    unsigned blk, unsigned nb, double *acc, unsigned facet, unsigned long ncell, double len,
+    const unsigned *boff, double *uu, double *vv, double *dre, double *dim, double *s2o,
+    uint32_t *cid, double *ctr, double *sum_w_dense, double fx_unscale, double fs_unscale,
+    unsigned *tscan)
 {
-    __shared__ unsigned tscan[kCompactBS];
     const double *acc_w = acc + (FX ? kAccW : 3) * ncell;
-    unsigned long base = (unsigned long)blockIdx.x * kCompactTile + threadIdx.x * kCompactPT;
+    unsigned long base = (unsigned long)blk * kCompactTile + threadIdx.x * kCompactPT;
     unsigned cnt = 0;
     for (int k = 0; k < kCompactPT; k++) {
         unsigned long p = base + k;
@@ -549,18 +550,18 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
         tscan[threadIdx.x] += t;
         __syncthreads();
     }
-    unsigned n = boff[blockIdx.x] + tscan[threadIdx.x] - cnt;
+    unsigned n = boff[blk] + tscan[threadIdx.x] - cnt;
     const double inc = 2 * len / facet;
-    if (cid && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (cid && blk == 0 && threadIdx.x == 0) {
         // zero pad cell after the last one (the arrays hold facet^2 + 1 entries): lanes past
         // the end of a chunk read it instead of masking their data
-        const unsigned nt = boff[gridDim.x];
+        const unsigned nt = boff[nb];
         dre[nt] = 0.;
         dim[nt] = 0.;
         cid[nt] = 0;
     }
     if (ctr) {  // centre of grid index k, the same expression as uu / vv below
-        const unsigned k = blockIdx.x * kCompactBS + threadIdx.x;
+        const unsigned k = blk * kCompactBS + threadIdx.x;
         if (k < facet) ctr[k] = (k + 0.5) * inc - len;
     }
     for (int k = 0; k < kCompactPT; k++) {
@@ -582,6 +583,121 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
             acc_clear<FX>(acc, ncell, p);
         }
     }
+}
+
+template <bool FX>
+__global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
+    double *acc, unsigned facet, unsigned long ncell, double len, const unsigned *boff,
+    double *uu, double *vv, double *dre, double *dim, double *s2o, uint32_t *cid, double *ctr,
+    double *sum_w_dense, double fx_unscale, double fs_unscale)
+{
+    __shared__ unsigned tscan[kCompactBS];
+    compact_write_tile<FX>(blockIdx.x, gridDim.x, acc, facet, ncell, len, boff, uu, vv, dre, dim,
+                           s2o, cid, ctr, sum_w_dense, fx_unscale, fs_unscale, tscan);
+}
+
+// ---- batched compaction: grid.y = target of a BatchDesc; targets beyond desc->ntarget and
+// blocks beyond a target's facet exit at once, so one launch shape serves every batch.
+__global__ __launch_bounds__(kCompactBS) void compact_count_batch_kernel(const BatchDesc *desc)
+{
+    const unsigned t = blockIdx.y;
+    if (t >= desc->ntarget) return;
+    const BatchCompact &C = desc->compact[t];
+    const unsigned long ncell = (unsigned long)C.facet * C.facet;
+    const unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    if (blockIdx.x >= nb) return;
+    __shared__ unsigned part[kCompactBS / 64];
+    const double *acc_w = C.acc + kAccW * ncell;
+    unsigned long base = (unsigned long)blockIdx.x * kCompactTile + threadIdx.x * kCompactPT;
+    unsigned cnt = 0;
+    for (int k = 0; k < kCompactPT; k++) {
+        unsigned long p = base + k;
+        if (p < ncell && acc_w[p] != 0.) cnt++;
+    }
+    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tt = 0;
+        for (int i = 0; i < kCompactBS / 64; i++) tt += part[i];
+        C.scan[blockIdx.x] = tt;
+    }
+}
+
+__global__ __launch_bounds__(1024) void compact_scan_batch_kernel(const BatchDesc *desc)
+{
+    const unsigned t = blockIdx.x;
+    if (t >= desc->ntarget) return;
+    const BatchCompact &C = desc->compact[t];
+    const unsigned long ncell = (unsigned long)C.facet * C.facet;
+    const unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    unsigned *bcount = C.scan;
+    __shared__ unsigned buf[1024];
+    __shared__ unsigned carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (unsigned base = 0; base < nb; base += 1024) {
+        unsigned i = base + threadIdx.x;
+        unsigned x = (i < nb) ? bcount[i] : 0;
+        buf[threadIdx.x] = x;
+        __syncthreads();
+        for (unsigned off = 1; off < 1024; off <<= 1) {
+            unsigned tt = (threadIdx.x >= off) ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += tt;
+            __syncthreads();
+        }
+        if (i < nb) bcount[i] = carry + buf[threadIdx.x] - x;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bcount[nb] = carry;
+        *C.n_out = carry;
+    }
+}
+
+__global__ __launch_bounds__(kCompactBS) void compact_write_batch_kernel(const BatchDesc *desc,
+                                                                          double len)
+{
+    const unsigned t = blockIdx.y;
+    if (t >= desc->ntarget) return;
+    const BatchCompact &C = desc->compact[t];
+    const unsigned long ncell = (unsigned long)C.facet * C.facet;
+    const unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    if (blockIdx.x >= nb) return;
+    __shared__ unsigned tscan[kCompactBS];
+    compact_write_tile<true>(blockIdx.x, nb, C.acc, C.facet, ncell, len, C.scan, C.uu, C.vv,
+                             C.dre, C.dim, C.s2, C.cid, C.ctr, nullptr, C.fx_unscale,
+                             C.fs_unscale, tscan);
+}
+
+__global__ void pack_facet_batch_kernel(const BatchDesc *desc)
+{
+    const unsigned t = blockIdx.y;
+    if (t >= desc->ntarget) return;
+    const BatchCompact &C = desc->compact[t];
+    const unsigned n = *C.n_out;
+    for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        C.pk[i] = make_float4((float)C.uu[i], (float)C.vv[i], (float)C.dre[i], (float)C.dim[i]);
+        const double w = 1.0 / C.s2[i];
+        C.invs2[i] = (float)w;
+        C.invs2_64[i] = w;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) C.invs2_64[n] = 0.;  // the zero pad cell
+}
+
+void launch_compact_batch(hipStream_t s, const BatchDesc *desc, unsigned ncap, unsigned fmax,
+                          double len)
+{
+    const unsigned long ncell = (unsigned long)fmax * fmax;
+    const unsigned nb = (unsigned)((ncell + kCompactTile - 1) / kCompactTile);
+    compact_count_batch_kernel<<<dim3(nb, ncap), kCompactBS, 0, s>>>(desc);
+    compact_scan_batch_kernel<<<ncap, 1024, 0, s>>>(desc);
+    compact_write_batch_kernel<<<dim3(nb, ncap), kCompactBS, 0, s>>>(desc, len);
+    const unsigned pb = std::min(256u, (unsigned)((ncell + 255) / 256) + 1);
+    pack_facet_batch_kernel<<<dim3(pb, ncap), 256, 0, s>>>(desc);
 }
 
 size_t grid_acc_words(unsigned facet) { return (size_t)kAccWords * facet * facet; }
@@ -615,6 +731,7 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
         hcnt[s] = 0;
     }
     if (tid == 0) hn = 0;
+    const unsigned ntarget = a.desc->ntarget, nsub = a.desc->nsub;
     while (true) {
         if (tid == 0) s_tile = atomicAdd(&a.queue[0], 1u);
         __syncthreads();
@@ -642,9 +759,9 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
                 vs2[k] = i < a.nrow ? a.s2f[idx] : -1.f;
                 vres[k] = i < a.nrow ? load_res(a.res, PREC, idx) : make_double2(0., 0.);
             }
-            if (a.nsub) {
-                for (unsigned b = 0; b < a.nsub; b++) {
-                    const GalaxyGeom fit = a.subs[b].fit, rnd = a.subs[b].rnd;
+            if (nsub) {
+                for (unsigned b = 0; b < nsub; b++) {
+                    const GalaxyGeom fit = a.desc->subs[b].fit, rnd = a.desc->subs[b].rnd;
 #pragma unroll
                     for (int k = 0; k < RPT; k++) {
                         const double p = ru[k] * fit.l0 + rv[k] * fit.m0 + rw[k] * fit.n0;
@@ -664,8 +781,8 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
                     store_res(a.res, PREC, (size_t)c * a.nrow + i, vres[k]);
                 }
             }
-            for (unsigned t = 0; t < a.ntarget; t++) {
-                const BatchTarget T = a.targets[t];
+            for (unsigned t = 0; t < ntarget; t++) {
+                const BatchTarget T = a.desc->targets[t];
                 uint32_t rkey = kEmpty, rn = 0;
                 double rre = 0., rim = 0., rs2 = 0.;
                 auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {

@@ -85,10 +85,12 @@ struct Obs {
     int loglik_prec = 0;     // precision of rl2_obs_loglik requests (0 FP64, 1 mixed)
     float *d_feat = nullptr; // learned-proposal inputs of a batch (training data)
     size_t feat_cap = 0;
-    // batched extraction: targets and pending updates of one launch, the work queue
-    BatchTarget *d_targets = nullptr;
-    BatchSub *d_subs = nullptr;
+    // batched extraction: the batch descriptor (host staging and device copy), the work
+    // queue, and the graphs of the batch's launches keyed by (targets, work-groups)
+    BatchDesc *h_desc = nullptr;
+    BatchDesc *d_desc = nullptr;
     unsigned *d_queue = nullptr;
+    std::vector<std::pair<unsigned long long, hipGraphExec_t>> batch_graphs;
     ~Obs();
     bool ensure_slots(int n);
     void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device

@@ -266,8 +266,11 @@ Obs::~Obs()
     }
     dfree(d_facets);
     dfree(d_feat);
-    dfree(d_targets);
-    dfree(d_subs);
+    for (auto &g : batch_graphs) hipGraphExecDestroy(g.second);
+    batch_graphs.clear();
+    if (h_desc) hipHostFree(h_desc);
+    h_desc = nullptr;
+    dfree(d_desc);
     dfree(d_queue);
     lik.release();
     dfree(lik.d_partials);
@@ -481,6 +484,40 @@ static unsigned num_cus()
     return g_num_cus;
 }
 
+static bool ensure_batch_buffers(Obs &o)
+{
+    if (o.d_desc) return true;
+    return hip_ok(hipHostMalloc((void **)&o.h_desc, sizeof(BatchDesc)), "hipHostMalloc") &&
+           dmalloc(&o.d_desc, 1) && dmalloc(&o.d_queue, 2) &&
+           hip_ok(hipMemsetAsync(o.d_queue, 0, 2 * sizeof(unsigned), g_stream), "memset");
+}
+
+// The launches of one batched extraction: the residual pass, then the compactions and FP32
+// copies of every target.  Their shapes depend only on the observation, the target capacity
+// and the work-group count, and everything else comes from the device descriptor.
+static void batch_launches(Obs &o, unsigned ncap, unsigned groups)
+{
+    GridBatchLaunch a = {};
+    a.prec = o.prec;
+    a.nch = o.nch;
+    a.nrow = o.nrow;
+    a.u = o.d_u;
+    a.v = o.d_v;
+    a.w = o.d_w;
+    a.wn = o.d_wn;
+    a.inv_lambda = o.d_invlam;
+    a.spec = o.d_spec;
+    a.res = o.d_res;
+    a.s2f = o.d_s2f;
+    a.len = o.len;
+    a.fs_scale = fx_scales(1., o.s2_max, 10).fs_scale;
+    a.desc = o.d_desc;
+    a.queue = o.d_queue;
+    a.ntiles = (unsigned)((o.nrow + kBatchTileRows - 1) / kBatchTileRows);
+    launch_grid_batch(g_stream, a, groups);
+    launch_compact_batch(g_stream, o.d_desc, ncap, o.fmax, o.len);
+}
+
 bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
                        const std::vector<PendingSub> &subs)
 {
@@ -499,68 +536,101 @@ bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
         }
         max_slot = std::max(max_slot, r.slot);
     }
-    if (!o.ensure_slots(max_slot + 1)) return false;
-    if (!o.d_targets && !(dmalloc(&o.d_targets, kMaxBatch) && dmalloc(&o.d_subs, kMaxBatch) &&
-                          dmalloc(&o.d_queue, 2) &&
-                          hip_ok(hipMemsetAsync(o.d_queue, 0, 2 * sizeof(unsigned), g_stream),
-                                 "memset")))
-        return false;
+    if (!o.ensure_slots(max_slot + 1) || !ensure_batch_buffers(o)) return false;
+    // the previous batch's descriptor may still be read by its launches
+    if (!hip_ok(hipStreamSynchronize(g_stream), "batch sync")) return false;
     for (const PendingSub &p : subs) o.res_bound += 3. * fabs(p.flux) * o.spec_max;
-    std::vector<BatchTarget> T(nb);
-    std::vector<AccScale> sc(nb);
+    BatchDesc &D = *o.h_desc;
+    D.ntarget = (unsigned)nb;
+    D.nsub = (unsigned)subs.size();
     for (int k = 0; k < nb; k++) {
         const ExtractReq &r = reqs[k];
-        o.slots[r.slot].facet = r.facet;
-        T[k].g = make_geom(r.l0, r.m0, r.flux, r.R_mu, 0., 0.);
-        T[k].inc = 2 * o.len / r.facet;
-        T[k].inv_inc = 1.0 / T[k].inc;
-        sc[k] = fx_scales(fabs(r.flux) * o.spec_max + o.res_bound, o.s2_max, 10);
-        T[k].fx_scale = sc[k].fx_scale;
-        T[k].acc = o.slots[r.slot].acc;
-        T[k].facet = r.facet;
-        T[k].pad = 0;
+        FacetSlot &sl = o.slots[r.slot];
+        sl.facet = r.facet;
+        BatchTarget &T = D.targets[k];
+        T.g = make_geom(r.l0, r.m0, r.flux, r.R_mu, 0., 0.);
+        T.inc = 2 * o.len / r.facet;
+        T.inv_inc = 1.0 / T.inc;
+        const AccScale sc = fx_scales(fabs(r.flux) * o.spec_max + o.res_bound, o.s2_max, 10);
+        T.fx_scale = sc.fx_scale;
+        T.acc = sl.acc;
+        T.facet = r.facet;
+        T.pad = 0;
+        BatchCompact &Cp = D.compact[k];
+        Cp.acc = sl.acc;
+        Cp.fx_unscale = 1.0 / sc.fx_scale;
+        Cp.fs_unscale = 1.0 / sc.fs_scale;
+        Cp.uu = sl.uu;
+        Cp.vv = sl.vv;
+        Cp.dre = sl.dre;
+        Cp.dim = sl.dim;
+        Cp.s2 = sl.s2;
+        Cp.cid = sl.cid;
+        Cp.ctr = sl.ctr;
+        Cp.pk = sl.pk;
+        Cp.invs2 = sl.invs2;
+        Cp.invs2_64 = sl.invs2_64;
+        Cp.n_out = &o.d_facets[r.slot].n;
+        Cp.scan = sl.scan;
+        Cp.facet = r.facet;
+        Cp.pad = 0;
     }
-    std::vector<BatchSub> S(subs.size());
     for (size_t b = 0; b < subs.size(); b++) {
         const PendingSub &p = subs[b];
-        S[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
-        S[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
+        D.subs[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
+        D.subs[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
     }
-    hipMemcpyAsync(o.d_targets, T.data(), nb * sizeof(BatchTarget), hipMemcpyHostToDevice,
-                   g_stream);
-    if (!S.empty())
-        hipMemcpyAsync(o.d_subs, S.data(), S.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
-                       g_stream);
-    GridBatchLaunch a = {};
-    a.prec = o.prec;
-    a.nch = o.nch;
-    a.nrow = o.nrow;
-    a.u = o.d_u;
-    a.v = o.d_v;
-    a.w = o.d_w;
-    a.wn = o.d_wn;
-    a.inv_lambda = o.d_invlam;
-    a.spec = o.d_spec;
-    a.res = o.d_res;
-    a.s2f = o.d_s2f;
-    a.len = o.len;
-    a.fs_scale = sc[0].fs_scale;
-    a.ntarget = (unsigned)nb;
-    a.nsub = (unsigned)S.size();
-    a.targets = o.d_targets;
-    a.subs = o.d_subs;
-    a.queue = o.d_queue;
-    a.ntiles = (unsigned)((o.nrow + kBatchTileRows - 1) / kBatchTileRows);
+    const size_t desc_bytes = offsetof(BatchDesc, compact) + nb * sizeof(BatchCompact);
+    if (!hip_ok(hipMemcpyAsync(o.d_desc, o.h_desc, desc_bytes, hipMemcpyHostToDevice, g_stream),
+                "batch descriptor"))
+        return false;
+    const unsigned groups = tuning().batch_groups ? tuning().batch_groups : 3 * num_cus();
     {
         KTimer t(RL2_K_EXTRACT);
-        launch_grid_batch(g_stream, a,
-                          tuning().batch_groups ? tuning().batch_groups : 3 * num_cus());
+        if (tuning().batch_graph) {
+            // one graph per (target capacity, work-groups), captured on first use and replayed
+            // for every later batch of that shape
+            const unsigned ncap = (unsigned)nb;
+            const unsigned long long key = ((unsigned long long)ncap << 32) | groups;
+            hipGraphExec_t exec = nullptr;
+            for (auto &g : o.batch_graphs)
+                if (g.first == key) exec = g.second;
+            if (!exec) {
+                hipGraph_t graph = nullptr;
+                if (!hip_ok(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal),
+                            "hipStreamBeginCapture"))
+                    return false;
+                batch_launches(o, ncap, groups);
+                if (!hip_ok(hipStreamEndCapture(g_stream, &graph), "hipStreamEndCapture"))
+                    return false;
+                const bool ok = hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
+                                       "hipGraphInstantiate");
+                hipGraphDestroy(graph);
+                if (!ok) return false;
+                o.batch_graphs.push_back({key, exec});
+            }
+            if (!hip_ok(hipGraphLaunch(exec, g_stream), "hipGraphLaunch")) return false;
+        } else {
+            batch_launches(o, (unsigned)nb, groups);
+        }
     }
     const double res_b = o.prec == 0 ? 16. : 8.;
-    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * (res_b + 4. + (S.empty() ? 0. : res_b)) +
+    stats_add_work(RL2_K_EXTRACT, 0., o.nvis * (res_b + 4. + (subs.empty() ? 0. : res_b)) +
                                           o.nrow * 24.);
-    for (int k = 0; k < nb; k++)
-        if (!finish_extract(o, reqs[k].slot, reqs[k].facet, false, sc[k])) return false;
+    // grid structure of each slot's facet for the phase tables (host-side descriptors)
+    for (int k = 0; k < nb; k++) {
+        const int slot = reqs[k].slot;
+        FacetSlot &sl = o.slots[slot];
+        FacetDev &f = o.h_facets[slot];
+        const unsigned facet = reqs[k].facet;
+        const unsigned gf = facet < 65536u ? facet : 0u;
+        if (f.cid != sl.cid || f.grid_facet != gf) {
+            f.cid = gf ? sl.cid : nullptr;
+            f.ctr = gf ? sl.ctr : nullptr;
+            f.grid_facet = gf;
+            o.facet_upload(slot);
+        }
+    }
     return hip_ok(hipGetLastError(), "batched extraction launch");
 }
 
@@ -571,20 +641,22 @@ bool obs_subtract_batch(Obs &o, const std::vector<PendingSub> &subs)
         set_error("batched subtraction: at most " + std::to_string(kMaxBatch) + " updates");
         return false;
     }
-    if (!o.d_subs && !dmalloc(&o.d_subs, kMaxBatch)) return false;
-    std::vector<BatchSub> S(subs.size());
+    if (!ensure_batch_buffers(o) || !hip_ok(hipStreamSynchronize(g_stream), "batch sync"))
+        return false;
+    BatchDesc &D = *o.h_desc;
     for (size_t b = 0; b < subs.size(); b++) {
         const PendingSub &p = subs[b];
         o.res_bound += 3. * fabs(p.flux) * o.spec_max;
-        S[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
-        S[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
+        D.subs[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
+        D.subs[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
     }
-    hipMemcpyAsync(o.d_subs, S.data(), S.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
+    hipMemcpyAsync(o.d_desc->subs, D.subs, subs.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
                    g_stream);
     {
         KTimer t(RL2_K_SUBTRACT);
         launch_subtract_batch(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn,
-                              o.d_invlam, o.d_spec, o.d_subs, (unsigned)S.size(), o.d_res);
+                              o.d_invlam, o.d_spec, o.d_desc->subs, (unsigned)subs.size(),
+                              o.d_res);
     }
     stats_add_work(RL2_K_SUBTRACT, 0., o.nvis * bytes_per_vis(o, RL2_K_SUBTRACT) + o.nrow * 24.);
     return hip_ok(hipGetLastError(), "batched subtraction launch");
@@ -1003,6 +1075,9 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
         }
         if (!hip_ok(hipStreamSynchronize(g_stream), "subtract sync")) return false;
         *t_sub += now_s() - t2;
+        if (opt.verbose && (b0 / B) % (size_t)opt.verbose == 0)
+            fprintf(stderr, "[rl2] pass %d: %zu of %zu galaxies fitted\n", pass, b0 + nb,
+                    todo.size());
     }
     if (pending && !obs_subtract(o, pend.l0, pend.m0, pend.flux, pend.R_mu, pend.e1, pend.e2))
         return false;
@@ -1471,6 +1546,88 @@ int rl2_process(rl2_obs *obs, unsigned int ngal, const double *l, const double *
     return process(obs->o, ngal, l, m, flux, scale, op, results, stats) ? RL2_OK : RL2_ERR_HIP;
 }
 
+// Coordinate search over the launch knobs that leave results unchanged up to rounding:
+// every candidate value of one knob is timed on the same galaxies from the same residual
+// (the first pass of the loop with the caller's options), the fastest is kept, then the
+// next knob.  The residual is put back afterwards.
+int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double *m,
+                 const double *flux, const double *scale, const rl2_fit_options *opt,
+                 rl2_tune_trial *trials, int max_trials, int *ntrials, int *lik_block)
+{
+    if (!obs || !l || !m || !flux || !scale || !opt || ngal == 0) return RL2_ERR_ARG;
+    Obs &o = obs->o;
+    const size_t bytes = o.nvis * (o.prec == RL2_PREC_FP64 ? 16 : 8);
+    void *saved = nullptr;
+    if (!hip_ok(hipMalloc(&saved, bytes), "hipMalloc autotune residual")) return RL2_ERR_HIP;
+    const double bound0 = o.res_bound;
+    bool ok = hip_ok(hipMemcpyAsync(saved, o.d_res, bytes, hipMemcpyDeviceToDevice, g_stream),
+                     "autotune save");
+    rl2_fit_options op = *opt;
+    op.do_refit = 0;
+    op.features_out = nullptr;
+    op.labels_out = nullptr;
+    const int B = op.batch > 0 ? op.batch : 1;
+    const bool batched = B > 1 && B <= kMaxBatch && tuning().batch_extract;
+    std::vector<rl2_gal_result> res(ngal);
+    std::vector<unsigned> todo(ngal);
+    for (unsigned g = 0; g < ngal; g++) todo[g] = g;
+    int nt = 0;
+    auto trial = [&](int knob, int value) -> double {
+        if (!ok) return 0.;
+        ok = hip_ok(hipMemcpyAsync(o.d_res, saved, bytes, hipMemcpyDeviceToDevice, g_stream),
+                    "autotune restore") &&
+             hip_ok(hipStreamSynchronize(g_stream), "sync");
+        o.res_bound = bound0;
+        double te = 0., tf = 0., ts = 0.;
+        const double t0 = now_s();
+        ok = ok && process_list(o, todo, 0, l, m, flux, scale, op, res.data(), nullptr, &te, &tf,
+                                &ts, nullptr) &&
+             hip_ok(hipStreamSynchronize(g_stream), "sync");
+        const double dt = now_s() - t0;
+        if (trials && nt < max_trials) trials[nt] = {knob, value, dt};
+        nt++;
+        return dt;
+    };
+    struct Knob {
+        int id;
+        std::vector<int> values;
+    };
+    std::vector<Knob> knobs;
+    if (batched)
+        knobs.push_back({RL2_TUNE_BATCH_GROUPS, {(int)num_cus(), 2 * (int)num_cus(),
+                                                 3 * (int)num_cus(), 4 * (int)num_cus()}});
+    else
+        knobs.push_back({RL2_TUNE_GRID_VARIANT, {3, 4, 2, 1, 0}});
+    knobs.push_back({RL2_TUNE_TILE_CELLS, {0, 512, 1024, 2048, 4096, 8192, 16384}});
+    knobs.push_back({RL2_TUNE_TILE_STREAMS, {1, 2}});
+    knobs.push_back({-1, {0, 256, 1024}});  // work-group threads per (e, r) task; 0: tiled
+    for (const Knob &k : knobs) {
+        int best = k.values[0];
+        double tbest = 1e300;
+        for (int v : k.values) {
+            if (k.id >= 0 && rl2_tune_set(k.id, v) != RL2_OK) continue;
+            if (k.id < 0) op.lik_block = v;
+            const double dt = trial(k.id, v);
+            if (ok && dt < tbest) {
+                tbest = dt;
+                best = v;
+            }
+        }
+        if (k.id >= 0)
+            rl2_tune_set(k.id, best);
+        else
+            op.lik_block = best;
+    }
+    ok = ok && hip_ok(hipMemcpyAsync(o.d_res, saved, bytes, hipMemcpyDeviceToDevice, g_stream),
+                      "autotune restore") &&
+         hip_ok(hipStreamSynchronize(g_stream), "sync");
+    o.res_bound = bound0;
+    hipFree(saved);
+    if (ntrials) *ntrials = nt;
+    if (lik_block) *lik_block = op.lik_block;
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
 int rl2_stats_enable(int on)
 {
     if (!ensure_init()) return RL2_ERR_NO_DEVICE;
@@ -1519,7 +1676,7 @@ int rl2_tune_set(int knob, int value)
         tuning().grid_variant = value;
         return RL2_OK;
     case RL2_TUNE_TILE_CELLS:
-        if (value != 0 && (value < (int)kTileMinCells || value > (int)kTileMaxCells ||
+        if (value != 0 && (value < (int)kTileKnobMinCells || value > (int)kTileMaxCells ||
                            (value & (value - 1))))
             break;
         tuning().tile_cells = (unsigned)value;
@@ -1543,6 +1700,10 @@ int rl2_tune_set(int knob, int value)
     case RL2_TUNE_BATCH_GROUPS:
         if (value < 0 || value > 65536) break;
         tuning().batch_groups = (unsigned)value;
+        return RL2_OK;
+    case RL2_TUNE_BATCH_GRAPH:
+        if (value < 0 || value > 1) break;
+        tuning().batch_graph = value;
         return RL2_OK;
     case RL2_TUNE_TILE_STREAMS:
         if (value != 1 && value != 2) break;
@@ -1568,6 +1729,7 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_TILE_STREAMS: *value = tuning().tile_streams; return RL2_OK;
     case RL2_TUNE_BATCH_EXTRACT: *value = tuning().batch_extract; return RL2_OK;
     case RL2_TUNE_BATCH_GROUPS: *value = (int)tuning().batch_groups; return RL2_OK;
+    case RL2_TUNE_BATCH_GRAPH: *value = tuning().batch_graph; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

@@ -32,14 +32,15 @@ def rprior_for(flux, ro):
 
 def fit_options(batch=1, do_refit=1, nm_speculate=1, line_chunk=4, lik_block=0,
                 lik_precision=0, proposal=None, features_out=None, labels_out=None,
-                proposal_mode=0):
+                proposal_mode=0, verbose=0):
     """rl2_fit_options; lik_block 0: tiled likelihood, else threads per (e, r) work-group;
     lik_precision 0: FP64, 1: mixed throughout, 2: mixed simplex + FP64 sampling, 3: FP64
     simplex + mixed sampling, 4: as 3 with FP64 verification; proposal: an rl2_proposal
     handle (proposal_mode 0: simplex start and speculation depth, 1: speculation depth
     only, results identical to the reference start); features_out / labels_out: float32
-    (ngal, 770) / float64 (ngal, 3) arrays."""
-    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, 0, lik_precision,
+    (ngal, 770) / float64 (ngal, 3) arrays; verbose: a progress line on stderr every
+    `verbose` batches."""
+    return N.FitOptions(batch, do_refit, nm_speculate, line_chunk, lik_block, verbose, lik_precision,
                         proposal, None if features_out is None else N.fptr(features_out),
                         None if labels_out is None else N.dptr(labels_out), proposal_mode)
 
@@ -168,6 +169,33 @@ class DeviceObservation:
                                     N.dptr(cat.scale), C.byref(o), res, C.byref(st)),
                 "rl2_process")
         return res, st
+
+
+KNOB_NAMES = {N.RL2_TUNE_GRID_VARIANT: "grid_variant", N.RL2_TUNE_TILE_CELLS: "tile_cells",
+              N.RL2_TUNE_TILE_STREAMS: "tile_streams", N.RL2_TUNE_BATCH_GROUPS: "batch_groups",
+              -1: "lik_block"}
+
+
+def autotune(dev, cat, **opts):
+    """rl2_autotune over the galaxies of `cat` (first pass, the given fit options): sets the
+    fastest launch knobs and returns them with every timed trial."""
+    n = len(cat)
+    trials = (N.TuneTrial * 64)()
+    nt = C.c_int(0)
+    lb = C.c_int(0)
+    o = fit_options(**opts)
+    N.check(N.hip().rl2_autotune(dev.h, n, N.dptr(cat.l), N.dptr(cat.m), N.dptr(cat.flux),
+                                 N.dptr(cat.scale), C.byref(o), trials, 64, C.byref(nt),
+                                 C.byref(lb)), "rl2_autotune")
+    chosen = {"lik_block": lb.value}
+    for k, name in KNOB_NAMES.items():
+        if k >= 0:
+            v = C.c_int()
+            N.check(N.hip().rl2_tune_get(k, C.byref(v)), "rl2_tune_get")
+            chosen[name] = v.value
+    return {"chosen": chosen,
+            "trials": [{"knob": KNOB_NAMES.get(t.knob, t.knob), "value": t.value,
+                        "seconds": round(t.seconds, 5)} for t in trials[:min(nt.value, 64)]]}
 
 
 def results_array(res):

@@ -66,3 +66,9 @@ def oracle_process(obs, cat, vis_sky, batch=1, refit=1):
                            O.cptr(vd), O.cptr(vs), O.ptr(cat.l), O.ptr(cat.m), O.ptr(cat.flux),
                            O.ptr(cat.scale), res, refit, C.byref(st))
     return res, st, vd - vs
+
+
+def subset(cat, lo, hi, stride=1):
+    """Galaxies lo, lo + stride, ... below hi of a catalog, as a Catalog."""
+    return synth.Catalog(*(np.ascontiguousarray(getattr(cat, f)[lo:hi:stride]) for f in
+                           ("flux", "scale", "e1", "e2", "l", "m", "snr")))

@@ -268,22 +268,59 @@ def test_batched_extraction_matches_per_galaxy_passes(gpu, case, prec, batch):
     """A batch extracted in one persistent pass over the residual, which also applies the
     previous batch's subtractions (RL2_TUNE_BATCH_EXTRACT 1), gives the results and residual
     of one extraction and one subtraction pass per galaxy (0), bit for bit, for complex128
-    and complex64 residuals."""
+    and complex64 residuals; so does replaying the batch's launches as a hipGraph
+    (RL2_TUNE_BATCH_GRAPH 1, one replay per batch, the fits' own launches in between)."""
     obs, cat, vs = case
     H = N.hip()
     out = {}
     try:
-        for be in (1, 0):
+        for be, graph in ((1, 1), (1, 0), (0, 0)):
             N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_EXTRACT, be), "rl2_tune_set")
-            print(f"process prec {prec} batch {batch} batched extraction {be}", flush=True)
+            N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_GRAPH, graph), "rl2_tune_set")
+            print(f"process prec {prec} batch {batch} batched extraction {be} graph {graph}",
+                  flush=True)
             d = P.DeviceObservation(obs, precision=prec, vis_sky=vs)
             res, st = d.process(cat, batch=batch)
-            out[be] = (P.results_array(res), d.residual())
+            out[(be, graph)] = (P.results_array(res), d.residual())
             d.close()
     finally:
         N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_EXTRACT, 1), "rl2_tune_set")
-    (ra, va), (rb, vb) = out[1], out[0]
-    for k in ("e1", "e2", "var_e1", "var_e2", "oneDimvar", "maxL", "error", "pass_",
-              "subtracted", "ncoords"):
-        np.testing.assert_array_equal(ra[k], rb[k])
-    np.testing.assert_array_equal(va, vb)
+        N.check(H.rl2_tune_set(N.RL2_TUNE_BATCH_GRAPH, 1), "rl2_tune_set")
+    ref_r, ref_v = out[(0, 0)]
+    for key in ((1, 1), (1, 0)):
+        r, v = out[key]
+        for k in ("e1", "e2", "var_e1", "var_e2", "oneDimvar", "maxL", "error", "pass_",
+                  "subtracted", "ncoords"):
+            np.testing.assert_array_equal(r[k], ref_r[k])
+        np.testing.assert_array_equal(v, ref_v)
+
+
+@pytest.mark.parametrize("batch", [1, 8])
+def test_autotune_keeps_residual_and_results(gpu, case, batch):
+    """rl2_autotune times every candidate of each knob, sets the fastest and leaves the
+    residual as it was; the loop with the tuned knobs gives the default knobs' results up to
+    rounding."""
+    obs, cat, vs = case
+    H = N.hip()
+    d = P.DeviceObservation(obs, vis_sky=vs)
+    try:
+        r0, st0 = d.process(cat, batch=batch)
+        ref = P.results_array(r0)
+        d.close()
+        d = P.DeviceObservation(obs, vis_sky=vs)
+        before = d.residual()
+        out = P.autotune(d, T.subset(cat, 0, 8), batch=batch)
+        print(f"autotune batch {batch}: {out['chosen']}, {len(out['trials'])} trials", flush=True)
+        assert len(out["trials"]) >= 12
+        assert all(t["seconds"] > 0 for t in out["trials"])
+        np.testing.assert_array_equal(d.residual(), before)
+        r1, st1 = d.process(cat, batch=batch, lik_block=out["chosen"]["lik_block"])
+        got = P.results_array(r1)
+        np.testing.assert_array_equal(got["error"], ref["error"])
+        assert np.abs(got["e1"] - ref["e1"]).max() <= 1e-6
+        assert np.abs(got["e2"] - ref["e2"]).max() <= 1e-6
+    finally:
+        d.close()
+        for k, v in ((N.RL2_TUNE_GRID_VARIANT, 3), (N.RL2_TUNE_TILE_CELLS, 0),
+                     (N.RL2_TUNE_TILE_STREAMS, 1), (N.RL2_TUNE_BATCH_GROUPS, 0)):
+            N.check(H.rl2_tune_set(k, v), "rl2_tune_set")

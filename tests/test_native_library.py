@@ -218,11 +218,14 @@ def test_tune_knobs_validate_and_round_trip():
     H = N.hip()
     v = C.c_int()
     cases = [(N.RL2_TUNE_GRID_VARIANT, [0, 1, 2, 3, 4], [-1, 5]),
-             (N.RL2_TUNE_TILE_CELLS, [0, 2048, 4096, 65536], [1024, 3000, 131072, -2048]),
+             (N.RL2_TUNE_TILE_CELLS, [0, 256, 2048, 4096, 65536], [128, 3000, 131072, -2048]),
              (N.RL2_TUNE_PHASE_TABLES, [0, 1, 2], [3, -1]),
              (N.RL2_TUNE_FUSE_SUBTRACT, [0, 1], [2, -1]),
              (N.RL2_TUNE_TILE_BLOCKS, [8, 1024, 0], [4, -8, 1001]),
-             (N.RL2_TUNE_TILE_STREAMS, [2, 1], [0, 3])]
+             (N.RL2_TUNE_TILE_STREAMS, [2, 1], [0, 3]),
+             (N.RL2_TUNE_BATCH_EXTRACT, [0, 1], [2, -1]),
+             (N.RL2_TUNE_BATCH_GROUPS, [512, 0], [-1, 70000]),
+             (N.RL2_TUNE_BATCH_GRAPH, [0, 1], [2, -1])]
     try:
         for knob, good, bad in cases:
             for x in good:
@@ -239,6 +242,9 @@ def test_tune_knobs_validate_and_round_trip():
         H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
         H.rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, 0)
         H.rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, 1)
+        H.rl2_tune_set(N.RL2_TUNE_BATCH_EXTRACT, 1)
+        H.rl2_tune_set(N.RL2_TUNE_BATCH_GROUPS, 0)
+        H.rl2_tune_set(N.RL2_TUNE_BATCH_GRAPH, 1)
 
 
 def _philox_py(ctr, key, rounds=10):
