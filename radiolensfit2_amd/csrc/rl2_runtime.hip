@@ -1598,7 +1598,7 @@ int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double 
                                                  3 * (int)num_cus(), 4 * (int)num_cus()}});
     else
         knobs.push_back({RL2_TUNE_GRID_VARIANT, {3, 4, 2, 1, 0}});
-    knobs.push_back({RL2_TUNE_TILE_CELLS, {0, 512, 1024, 2048, 4096, 8192, 16384}});
+    knobs.push_back({RL2_TUNE_TILE_CELLS, {0, 2048, 4096, 8192, 16384}});
     knobs.push_back({RL2_TUNE_TILE_STREAMS, {1, 2}});
     knobs.push_back({-1, {0, 256, 1024}});  // work-group threads per (e, r) task; 0: tiled
     for (const Knob &k : knobs) {
