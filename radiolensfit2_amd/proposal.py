@@ -195,6 +195,9 @@ def evaluate(cfg, net, ngal, seed=9000, batch=64, log=print):
         ok = (de <= 1e-4) & (a["error"] == b["error"])
         summary[k]["within_1e-4_of_plain"] = float(ok.mean())
         summary[k]["median_abs_de_vs_plain"] = float(np.median(de))
+        summary[k]["within_1e-2_of_plain"] = float(((de <= 1e-2) & (a["error"] == b["error"])).mean())
+        # galaxies/s counting only galaxies inside the |de| <= 1e-4 gate
+        summary[k]["galaxies_per_s_within_gate"] = summary[k]["galaxies_per_s"] * float(ok.mean())
     # galaxies whose fit kept the reference's start (mode 1 never uses the proposal start)
     summary["learned_start"]["fallback_fraction"] = \
         1.0 - summary["learned_start"]["proposal_start_fraction"]
