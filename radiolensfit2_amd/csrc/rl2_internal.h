@@ -42,6 +42,10 @@ struct LikItem {
     double e1, e2;
     unsigned chunk_first;      // tiled engine: chunks of the items before this one
     unsigned tab_first;        // tiled engine: first phase-table entry of this item
+    unsigned chunk_cells;      // tiled engine: facet cells per work-group of this item (a
+                               // function of its facet alone, so that its sums are added in
+                               // the same order whatever else the launch carries)
+    unsigned pad2;
 };
 
 struct RGrid {
@@ -276,7 +280,7 @@ struct TileLaunch {
     TileTask *tasks;           // nitems * kMaxR
     double *partials;          // 8 doubles per work-group
     unsigned nwg;              // work-groups of a pass (nr_g * chunks summed over items)
-    unsigned chunk_cells;      // facet cells per work-group
+
     // The launch covers the scalelength tasks r_lo .. r_lo + nr_g - 1 of every item: the
     // tasks can be split in two groups whose passes run on two streams, so that one group's
     // step kernel and launch gaps overlap the other group's sums.

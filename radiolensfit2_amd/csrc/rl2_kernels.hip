@@ -2426,11 +2426,11 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
     const TileTask *T = t.tasks + (size_t)item * kMaxR + r;
     if (PASS == 1 && !T->active) return;  // finished task: the whole work-group moves on
     const LikItem it = items[item];
-    const unsigned beg = chunk * t.chunk_cells;
+    const unsigned beg = chunk * it.chunk_cells;
     double s[7];
     if constexpr (GRID) {
         auto grid_sums = [&](const FacetDev &F) {
-            const unsigned end = min(F.n, beg + t.chunk_cells);
+            const unsigned end = min(F.n, beg + it.chunk_cells);
             extern __shared__ double2 tab_lds[];
             if (PASS == 1) {
                 // t.tab_lds_facet * (16 + 8) bytes, + 16 with the rows: X, ctr, Y
@@ -2456,7 +2456,7 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
         grid_sums(a.facets[it.slot]);
     } else {
         const FacetDev &F = a.facets[it.slot];
-        const unsigned end = min(F.n, beg + t.chunk_cells);
+        const unsigned end = min(F.n, beg + it.chunk_cells);
         if (PASS == 1 && it.prec == 0 && t.tab && F.cid) {
             extern __shared__ double2 tab_lds[];
             const unsigned gf = F.grid_facet;

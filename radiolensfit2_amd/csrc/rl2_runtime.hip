@@ -715,24 +715,28 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         }
     };
     if (tiled) {
-        // chunks per (item, r) from the host copy of each slot's cell count; the chunk
-        // doubles from kTileMinCells while the launch keeps >= kTileTargetGroups groups
+        // chunks per (item, r) from the host copy of each slot's cell count.  An item's chunk
+        // depends on its own facet only (the largest power of two from kTileMinCells that
+        // still gives the item's tasks kTileTargetGroups work-groups), so its chunk sums, and
+        // with them its likelihood, do not depend on the other items of the launch.
         const int nr = o.rg.numr - 1;
-        auto groups = [&](unsigned chunk) {
-            unsigned long long g = 0;
-            for (int i = 0; i < n; i++)
-                g += std::max(1u, (o.slots[items[i].slot].n + chunk - 1) / chunk);
-            return g * (unsigned)nr;
+        auto item_chunk = [&](unsigned cells) {
+            unsigned chunk = tuning().tile_cells;
+            if (chunk == 0) {
+                chunk = kTileMinCells;
+                while (chunk < kTileMaxCells &&
+                       (unsigned long long)nr * ((cells + 2 * chunk - 1) / (2 * chunk)) >=
+                           kTileTargetGroups)
+                    chunk *= 2;
+            }
+            return chunk;
         };
-        unsigned chunk = tuning().tile_cells;
-        if (chunk == 0) {
-            chunk = kTileMinCells;
-            while (chunk < kTileMaxCells && groups(2 * chunk) >= kTileTargetGroups) chunk *= 2;
-        }
         unsigned chunks = 0;
         for (int i = 0; i < n; i++) {
             const unsigned cells = o.slots[items[i].slot].n;
+            const unsigned chunk = item_chunk(cells);
             b.h_items[i].chunk_first = chunks;
+            b.h_items[i].chunk_cells = chunk;
             chunks += std::max(1u, (cells + chunk - 1) / chunk);
         }
         const unsigned total = chunks * (unsigned)nr;
@@ -770,7 +774,6 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             t.nr_g = ng == 1 ? nr : (g == 0 ? (nr + 1) / 2 : nr / 2);
             t.nwg = chunks * (unsigned)t.nr_g;
             t.partials = b.d_partials + (g == 0 ? 0 : (size_t)tg[0].nwg * 8);
-            t.chunk_cells = chunk;
             t.tab = ntab ? b.d_tab : nullptr;
             t.tab_lds_facet = ntab ? tab_facet : 0;
             t.tab_y_lds = ntab && tuning().phase_tables == 2 && tab_facet * 40u <= kTileLdsBytes;

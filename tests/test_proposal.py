@@ -144,13 +144,11 @@ def test_gpu_training_reduces_loss(net):
 
 @pytest.mark.gpu
 def test_gpu_trained_proposal_gains_over_reference_start(gpu):
-    """The committed network (trained on the GPU on ~19k simulated 64-dish galaxies) on
-    fresh galaxies: starting the simplex at its peak cuts the batched launches a fit waits
-    for, and the proposal is taken for most galaxies (the rest fall back to the reference
-    start); in speculation-only mode the results are the reference start's, up to the
-    run-to-run variation of the extraction's atomic sums (~1e-15), which the fit of a faint
-    galaxy near one of the estimator's thresholds can amplify and the shared residual then
-    carries to later galaxies of the batch."""
+    """The committed network (trained on the GPU on 161k simulated 64-dish galaxies labelled
+    by exact-likelihood fits) on fresh galaxies: starting the simplex at its peak cuts the
+    batched launches a fit waits for, and the proposal is taken for most galaxies (the rest
+    fall back to the reference start); in speculation-only mode the results are the
+    reference start's exactly."""
     import os
     from radiolensfit2_amd import proposal as PR
     from radiolensfit2_amd import synth
@@ -165,5 +163,11 @@ def test_gpu_trained_proposal_gains_over_reference_start(gpu):
     assert start["rounds_per_galaxy"] < 0.95 * plain["rounds_per_galaxy"]
     assert start["proposal_start_fraction"] > 0.5
     assert spec["rounds_per_galaxy"] < plain["rounds_per_galaxy"]
-    assert spec["median_abs_de_vs_plain"] <= 1e-10
-    assert spec["within_1e-4_of_plain"] >= 0.9
+    # speculation only: every likelihood value is a function of its point and facet alone, so
+    # the fits, and through them the shared residual, are the plain path's bit for bit
+    assert spec["median_abs_de_vs_plain"] == 0.0
+    assert spec["within_1e-4_of_plain"] == 1.0
+    # the learned start moves the simplex end point within the simplex tolerance, which the
+    # sampling grid anchored there turns into a different posterior mean: statistically the
+    # same measurement, not the same numbers
+    assert start["within_1e-2_of_plain"] >= 0.5
