@@ -538,6 +538,8 @@ def main():
     ap.add_argument("--phase-tables", type=int, default=2, choices=[0, 1, 2],
                     help="FP64 Newton phases from per-pass column x row tables with the rows "
                          "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")
+    ap.add_argument("--item-groups", type=int, default=0,
+                    help="work-groups a likelihood item's chunking aims at (0: 512)")
     ap.add_argument("--tile-blocks", type=int, default=0,
                     help="cap on the blocks of a likelihood pass (multiple of 8; 0: one per chunk)")
     ap.add_argument("--tile-streams", type=int, default=1, choices=[1, 2],
@@ -611,6 +613,7 @@ def main():
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, args.fuse_subtract), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, args.tile_blocks), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, args.tile_streams), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_ITEM_GROUPS, args.item_groups), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)

@@ -215,10 +215,14 @@ int rl2_synchronize(void);
  *   compute unit).
  * RL2_TUNE_BATCH_GRAPH: 1 (default) replays a batch's extraction launches (the residual
  *   pass, compactions and FP32 copies) as a hipGraph captured once per batch shape; 0 launches
- *   them one by one.  Both give identical results. */
+ *   them one by one.  Both give identical results.
+ * RL2_TUNE_ITEM_GROUPS: work-groups the per-facet chunk choice aims at for one likelihood
+ *   item (RL2_TUNE_TILE_CELLS 0): the chunk is the largest power of two from 2048 cells that
+ *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 512. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
        RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5,
-       RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7, RL2_TUNE_BATCH_GRAPH = 8 };
+       RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7, RL2_TUNE_BATCH_GRAPH = 8,
+       RL2_TUNE_ITEM_GROUPS = 9 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

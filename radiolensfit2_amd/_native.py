@@ -35,6 +35,7 @@ RL2_TUNE_TILE_STREAMS = 5
 RL2_TUNE_BATCH_EXTRACT = 6
 RL2_TUNE_BATCH_GROUPS = 7
 RL2_TUNE_BATCH_GRAPH = 8
+RL2_TUNE_ITEM_GROUPS = 9
 (RL2_K_EXTRACT, RL2_K_COMPACT, RL2_K_LIKELIHOOD, RL2_K_RMARG, RL2_K_SUBTRACT,
  RL2_K_SOURCES) = range(6)
 KERNEL_NAMES = ["extract", "compact", "likelihood", "rmarg", "subtract", "sources"]

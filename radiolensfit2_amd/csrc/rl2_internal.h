@@ -199,6 +199,8 @@ struct Tuning {
                                // one per galaxy (0)
     unsigned batch_groups = 0; // RL2_TUNE_BATCH_GROUPS: persistent work-groups of a batched
                                // extraction (0: three per CU)
+    unsigned item_groups = 0;  // RL2_TUNE_ITEM_GROUPS: work-groups an item's chunking aims at
+                               // (0: kTileItemGroups)
     int batch_graph = 1;       // RL2_TUNE_BATCH_GRAPH: replay a batch's extraction launches
                                // as a hipGraph (1) or launch them one by one (0)
 };
@@ -270,7 +272,7 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned kTileKnobMinCells = 256;   // smallest chunk RL2_TUNE_TILE_CELLS accepts
-constexpr unsigned long long kTileTargetGroups = 2048;  // 8 work-groups per CU
+constexpr unsigned long long kTileItemGroups = 512;  // per item: 4 items give 8 per CU
 constexpr int kInlItems = 8;
 constexpr unsigned kTileLdsBytes = 40960;  // column-table staging per work-group (<= 1706 columns)
 

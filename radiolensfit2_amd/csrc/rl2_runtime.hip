@@ -717,8 +717,10 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     if (tiled) {
         // chunks per (item, r) from the host copy of each slot's cell count.  An item's chunk
         // depends on its own facet only (the largest power of two from kTileMinCells that
-        // still gives the item's tasks kTileTargetGroups work-groups), so its chunk sums, and
-        // with them its likelihood, do not depend on the other items of the launch.
+        // still gives the item's tasks kTileItemGroups work-groups), so its chunk sums, and
+        // with them its likelihood, do not depend on the other items of the launch.  A
+        // launch typically carries 3 to 8 items (the simplex's candidates, a line search's
+        // steps), which then fill the device with a few thousand work-groups.
         const int nr = o.rg.numr - 1;
         auto item_chunk = [&](unsigned cells) {
             unsigned chunk = tuning().tile_cells;
@@ -726,7 +728,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 chunk = kTileMinCells;
                 while (chunk < kTileMaxCells &&
                        (unsigned long long)nr * ((cells + 2 * chunk - 1) / (2 * chunk)) >=
-                           kTileTargetGroups)
+                           (tuning().item_groups ? tuning().item_groups : kTileItemGroups))
                     chunk *= 2;
             }
             return chunk;
@@ -1601,6 +1603,7 @@ int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double 
                                                  3 * (int)num_cus(), 4 * (int)num_cus()}});
     else
         knobs.push_back({RL2_TUNE_GRID_VARIANT, {3, 4, 2, 1, 0}});
+    knobs.push_back({RL2_TUNE_ITEM_GROUPS, {0, 256, 1024}});
     knobs.push_back({RL2_TUNE_TILE_CELLS, {0, 2048, 4096, 8192, 16384}});
     knobs.push_back({RL2_TUNE_TILE_STREAMS, {1, 2}});
     knobs.push_back({-1, {0, 256, 1024}});  // work-group threads per (e, r) task; 0: tiled
@@ -1708,6 +1711,10 @@ int rl2_tune_set(int knob, int value)
         if (value < 0 || value > 1) break;
         tuning().batch_graph = value;
         return RL2_OK;
+    case RL2_TUNE_ITEM_GROUPS:
+        if (value < 0 || value > 65536) break;
+        tuning().item_groups = (unsigned)value;
+        return RL2_OK;
     case RL2_TUNE_TILE_STREAMS:
         if (value != 1 && value != 2) break;
         tuning().tile_streams = value;
@@ -1733,6 +1740,7 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_BATCH_EXTRACT: *value = tuning().batch_extract; return RL2_OK;
     case RL2_TUNE_BATCH_GROUPS: *value = (int)tuning().batch_groups; return RL2_OK;
     case RL2_TUNE_BATCH_GRAPH: *value = tuning().batch_graph; return RL2_OK;
+    case RL2_TUNE_ITEM_GROUPS: *value = (int)tuning().item_groups; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

@@ -173,6 +173,7 @@ class DeviceObservation:
 
 KNOB_NAMES = {N.RL2_TUNE_GRID_VARIANT: "grid_variant", N.RL2_TUNE_TILE_CELLS: "tile_cells",
               N.RL2_TUNE_TILE_STREAMS: "tile_streams", N.RL2_TUNE_BATCH_GROUPS: "batch_groups",
+              N.RL2_TUNE_ITEM_GROUPS: "item_groups",
               -1: "lik_block"}
 
 

@@ -322,5 +322,6 @@ def test_autotune_keeps_residual_and_results(gpu, case, batch):
     finally:
         d.close()
         for k, v in ((N.RL2_TUNE_GRID_VARIANT, 3), (N.RL2_TUNE_TILE_CELLS, 0),
-                     (N.RL2_TUNE_TILE_STREAMS, 1), (N.RL2_TUNE_BATCH_GROUPS, 0)):
+                     (N.RL2_TUNE_TILE_STREAMS, 1), (N.RL2_TUNE_BATCH_GROUPS, 0),
+                     (N.RL2_TUNE_ITEM_GROUPS, 0)):
             N.check(H.rl2_tune_set(k, v), "rl2_tune_set")
