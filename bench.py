@@ -531,10 +531,11 @@ def main():
                     help="0: tiled likelihood; 64..1024: one work-group per (e, r) task")
     ap.add_argument("--tile-cells", type=int, default=0,
                     help="tiled likelihood cells per work-group (0: chosen per launch)")
-    ap.add_argument("--grid-variant", type=int, default=3, choices=[0, 1, 2, 3, 4],
-                    help="extraction gridding: 0 per-visibility LDS hash, 1 wave-aggregated, "
-                         "2 / 3 / 4 per-thread runs of 8 / 4 / 4 rows, tables of 2048 / "
-                         "1024 / 2048 slots")
+    ap.add_argument("--grid-variant", type=int, default=3, choices=[0, 1, 2, 3, 4, 5, 6],
+                    help="extraction gridding: runs of 2 / 2 / 8 / 4 / 4 rows per thread with "
+                         "tables of 512 / 1024 / 2048 / 1024 / 2048 slots flushed per channel "
+                         "(0-4), or runs of 2 / 4 rows with 1024 / 2048 slots kept across "
+                         "channels (5, 6)")
     ap.add_argument("--phase-tables", type=int, default=2, choices=[0, 1, 2],
                     help="FP64 Newton phases from per-pass column x row tables with the rows "
                          "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")

@@ -192,9 +192,11 @@ int rl2_synchronize(void);
 /* ---- launch-shape knobs (set by an autotuner; process-wide) ----
  * RL2_TUNE_GRID_VARIANT: extraction gridding, runs of consecutive rows merged per thread
  *   before one LDS table update: 0 / 1 / 2 / 3 / 4 runs of 2 / 2 / 8 / 4 / 4 rows with
- *   tables of 512 / 1024 / 2048 / 1024 / 2048 slots; default 3.  Every variant accumulates
- *   in integers, so the facet does not depend on the variant or on the order of the
- *   work-groups.
+ *   tables of 512 / 1024 / 2048 / 1024 / 2048 slots, flushed to the facet after every
+ *   channel; 5 / 6 runs of 2 / 4 rows with 1024 / 2048 slots that keep their cells across
+ *   channels and are flushed only when the next channel might not fit; default 3.  Every
+ *   variant accumulates in integers, so the facet does not depend on the order of the
+ *   work-groups (variants with the same rows per work-group give the same bits).
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two
  *   in [256, 65536], or 0 (default) to choose per launch (2048 to 65536).
  * RL2_TUNE_PHASE_TABLES: 2 (default) evaluates the FP64 Newton phases on extracted facets

@@ -292,7 +292,7 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
         for (size_t i = 0; i < nvis; i++)
             vb = fmax(vb, fmax(fabs(vis[i].real), fabs(vis[i].imag)));
     for (size_t i = 0; i < nvis; i++) sb = fmax(sb, (double)fabsf(s2[i]));
-    const AccScale sc = fx_scales(vb, sb);
+    const AccScale sc = fx_scales(vb, sb, -1, nch);
     launch_pack_sigma(stream(), nch, ncoords, dperm.p, ds2.p, flag ? dfl.p : nullptr, ds2f.p);
     GridLaunch a = {};
     a.mode = mode;

@@ -81,8 +81,10 @@ struct AccScale {
     double fx_scale = 1., fs_scale = 1.;
 };
 // The scales for visibility components bounded by vis_bound and variances by s2_bound.
-// rows_log2 < 0: the rows of a work-group of the current gridding variant.
-AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2 = -1);
+// rows_log2 < 0: the terms a slot of the current gridding variant takes between flushes
+// (the rows of a work-group, times the nch channels for the variants that keep cells across
+// channels).
+AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2 = -1, unsigned nch = 1);
 // Words (8 bytes) of a facet's gridding accumulators.
 size_t grid_acc_words(unsigned facet);
 // *out = max(*out, max over the residual of |re| + |im|), as the bits of a double (*out

@@ -231,7 +231,12 @@ __device__ inline void fx_flush(unsigned long long *hi, unsigned long long *lo, 
 // thread's visibilities and write it back, then grid the updated value: the subtraction and
 // the next extraction share one pass over the residual, and every visibility gets the same
 // arithmetic, in the same order, as subtract_kernel followed by this kernel.
-template <int MODE, int PREC, int RPT, int HS, bool SUB = false>
+// LAZY: the table keeps its cells from one channel to the next and is flushed only when
+// another channel's runs might not fit (more than HS - RPT*256 slots in use) and after the
+// last channel.  Neighbouring channels put a dense-core visibility in the same cell, so the
+// global atomics of a cell are made once for several channels; the integer sums make the
+// facet the same bit for bit whenever the flushes happen.
+template <int MODE, int PREC, int RPT, int HS, bool SUB = false, bool LAZY = false>
 __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
 {
     constexpr int BS = 256;
@@ -361,6 +366,7 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
         if (rn) emit(rkey, rre, rim, rs2, rn);
         __syncthreads();
         const uint32_t nocc = hn;
+        if (LAZY && c + 1 < a.nch && nocc <= (uint32_t)(HS - RPT * BS)) continue;
         for (uint32_t j = tid; j < nocc; j += BS) {
             const uint32_t s = hlist[j];
             const uint32_t key = hkey[s];
@@ -388,7 +394,23 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
 template <int MODE, int PREC>
 static void launch_grid_variant(hipStream_t s, const GridLaunch &a)
 {
-    if (tuning().grid_variant == 2) {
+    if (tuning().grid_variant == 5) {
+        // runs of 2 rows, 1024 slots, flushed when another channel might not fit
+        constexpr int RPT = 2;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        if (MODE == 0 && a.sub)
+            grid_run_kernel<MODE, PREC, RPT, 1024, true, true><<<grid, 256, 0, s>>>(a);
+        else
+            grid_run_kernel<MODE, PREC, RPT, 1024, false, true><<<grid, 256, 0, s>>>(a);
+    } else if (tuning().grid_variant == 6) {
+        // runs of 4 rows, 2048 slots, flushed when another channel might not fit
+        constexpr int RPT = 4;
+        dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
+        if (MODE == 0 && a.sub)
+            grid_run_kernel<MODE, PREC, RPT, 2048, true, true><<<grid, 256, 0, s>>>(a);
+        else
+            grid_run_kernel<MODE, PREC, RPT, 2048, false, true><<<grid, 256, 0, s>>>(a);
+    } else if (tuning().grid_variant == 2) {
         constexpr int RPT = 8;
         dim3 grid((unsigned)((a.nrow + RPT * 256 - 1) / (RPT * 256)));
         grid_run_kernel<MODE, PREC, RPT, 2048><<<grid, 256, 0, s>>>(a);
@@ -920,19 +942,26 @@ void launch_subtract_batch(hipStream_t s, int prec, unsigned nch, unsigned long 
 }
 
 // Rows of a gridding work-group (256 threads x rows per thread) for the current variant.
-static int grid_rows_log2()
+// log2 of the visibilities one LDS slot can receive between flushes: the rows of a
+// work-group, times the channels for the variants that keep cells across channels.
+static int grid_rows_log2(unsigned nch)
 {
+    int lazy = 0;
+    if (tuning().grid_variant >= 5)
+        while ((1u << lazy) < nch) lazy++;
     switch (tuning().grid_variant) {
-    case 2: return 11;           // runs of 8 rows
-    case 3: case 4: return 10;   // runs of 4 rows
-    default: return 9;           // runs of 2 rows
+    case 2: return 11;                 // runs of 8 rows
+    case 3: case 4: return 10;         // runs of 4 rows
+    case 6: return 10 + lazy;
+    case 5: return 9 + lazy;
+    default: return 9;                 // runs of 2 rows
     }
 }
 
-AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2)
+AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2, unsigned nch)
 {
-    // 2^S with bound * 2^S * rows <= 2^62: a channel's slot sums stay below 2^62
-    if (rows_log2 < 0) rows_log2 = grid_rows_log2();
+    // 2^S with bound * 2^S * (terms a slot takes between flushes) <= 2^62
+    if (rows_log2 < 0) rows_log2 = grid_rows_log2(nch);
     auto scale = [&](double b) {
         if (!(b > 0.) || !std::isfinite(b)) b = 1.;
         const int S = std::max(-1000, std::min(1000, 62 - rows_log2 - (ilogb(b) + 1)));

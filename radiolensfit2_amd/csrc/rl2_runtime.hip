@@ -451,7 +451,7 @@ bool obs_extract(Obs &o, int slot, double l0, double m0, double flux, double R_m
     a.acc = s.acc;
     // a visibility that is gridded is at most |round model| + |residual| per component
     if (sub) o.res_bound += 3. * fabs(sub->flux) * o.spec_max;
-    const AccScale sc = fx_scales(fabs(flux) * o.spec_max + o.res_bound, o.s2_max);
+    const AccScale sc = fx_scales(fabs(flux) * o.spec_max + o.res_bound, o.s2_max, -1, o.nch);
     a.fx_scale = sc.fx_scale;
     a.fs_scale = sc.fs_scale;
     if (sub) {  // the previous galaxy's post-fit update in the same pass
@@ -993,7 +993,8 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
     // The last subtraction of a batch can ride on the first extraction of the next one
     // (one pass over the residual instead of two): it is held here until then, and made on
     // its own at the end of the list.
-    const bool fuse = tuning().fuse_subtract && tuning().grid_variant == 3;
+    const bool fuse = tuning().fuse_subtract &&
+                      (tuning().grid_variant == 3 || tuning().grid_variant >= 5);
     PendingSub pend;
     bool pending = false;
     // batches extracted in one residual pass each, carrying the previous batch's
@@ -1678,7 +1679,7 @@ int rl2_tune_set(int knob, int value)
 {
     switch (knob) {
     case RL2_TUNE_GRID_VARIANT:
-        if (value < 0 || value > 4) break;
+        if (value < 0 || value > 6) break;
         tuning().grid_variant = value;
         return RL2_OK;
     case RL2_TUNE_TILE_CELLS:

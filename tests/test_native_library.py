@@ -217,7 +217,7 @@ def test_tune_knobs_validate_and_round_trip():
     host-only, so it runs without a device."""
     H = N.hip()
     v = C.c_int()
-    cases = [(N.RL2_TUNE_GRID_VARIANT, [0, 1, 2, 3, 4], [-1, 5]),
+    cases = [(N.RL2_TUNE_GRID_VARIANT, [0, 1, 2, 3, 4, 5, 6], [-1, 7]),
              (N.RL2_TUNE_TILE_CELLS, [0, 256, 2048, 4096, 65536], [128, 3000, 131072, -2048]),
              (N.RL2_TUNE_PHASE_TABLES, [0, 1, 2], [3, -1]),
              (N.RL2_TUNE_FUSE_SUBTRACT, [0, 1], [2, -1]),
