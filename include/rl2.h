@@ -180,8 +180,10 @@ typedef struct {
     double flops;         /* algorithmic FLOP-equivalents (likelihood) */
     double bytes;         /* algorithmic HBM bytes (streaming kernels) */
 } rl2_kernel_stat;
+/* RL2_K_LIKELIHOOD: the likelihood's chunk-sum kernels (lik_tile_kernel); RL2_K_STEP: the
+ * per-task step kernels that follow each of them (lik_step_kernel). */
 enum { RL2_K_EXTRACT = 0, RL2_K_COMPACT = 1, RL2_K_LIKELIHOOD = 2, RL2_K_RMARG = 3,
-       RL2_K_SUBTRACT = 4, RL2_K_SOURCES = 5, RL2_K_COUNT = 6 };
+       RL2_K_SUBTRACT = 4, RL2_K_SOURCES = 5, RL2_K_STEP = 6, RL2_K_COUNT = 7 };
 int rl2_stats_enable(int on);
 int rl2_stats_reset(void);
 int rl2_stats_get(int kernel, rl2_kernel_stat *out);

@@ -306,6 +306,9 @@ struct TileLaunch {
     LikItem inl[kInlItems];
 };
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
+// The two launches of a pass separately: the chunk sums, then the per-task step.
+void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass);
+void launch_tile_step(hipStream_t s, const TileLaunch &t, int pass);
 
 // Residual update for one fitted galaxy: R -= model(e) - model(0).
 void launch_subtract(hipStream_t s, int prec, unsigned nch, unsigned long nrow, const double *u,

@@ -2603,11 +2603,10 @@ __device__ void tile_status(const LikLaunch &a)
     }
 }
 
-void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
+void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass)
 {
-    const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
+    const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
     if (pass == 0) {
-        const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
         const unsigned lds = t.tab_lds_facet * 8;
         if (t.grid == 2)
             lik_tile_kernel<0, 2><<<g, kTileBS, lds, s>>>(t);
@@ -2615,18 +2614,30 @@ void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
             lik_tile_kernel<0, 1><<<g, kTileBS, lds, s>>>(t);
         else
             lik_tile_kernel<0, 0><<<g, kTileBS, lds, s>>>(t);
-        lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
     } else {
         const unsigned lds = t.tab ? t.tab_lds_facet * (t.tab_y_lds ? 40 : 24) : 0;
-        const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
         if (t.grid == 2)
             lik_tile_kernel<1, 2><<<g, kTileBS, lds, s>>>(t);
         else if (t.grid == 1)
             lik_tile_kernel<1, 1><<<g, kTileBS, lds, s>>>(t);
         else
             lik_tile_kernel<1, 0><<<g, kTileBS, lds, s>>>(t);
-        lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
     }
+}
+
+void launch_tile_step(hipStream_t s, const TileLaunch &t, int pass)
+{
+    const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
+    if (pass == 0)
+        lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
+    else
+        lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
+}
+
+void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
+{
+    launch_tile_sums(s, t, pass);
+    launch_tile_step(s, t, pass);
 }
 
 void launch_rmarg(hipStream_t s, const LikLaunch &a)

@@ -795,9 +795,19 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 hipStreamWaitEvent(g_stream2, g_fork, 0);
             }
             for (int p = first; p < first + count; p++) {
-                KTimer k(RL2_K_LIKELIHOOD);
-                launch_tile_pass(g_stream, tg[0], p == 0 ? 0 : 1);
-                if (ng == 2) launch_tile_pass(g_stream2, tg[1], p == 0 ? 0 : 1);
+                const int pass = p == 0 ? 0 : 1;
+                if (ng == 2) {
+                    KTimer k(RL2_K_LIKELIHOOD);
+                    launch_tile_pass(g_stream, tg[0], pass);
+                    launch_tile_pass(g_stream2, tg[1], pass);
+                    continue;
+                }
+                {
+                    KTimer k(RL2_K_LIKELIHOOD);  // the chunk sums: the kernel the FLOPs count
+                    launch_tile_sums(g_stream, tg[0], pass);
+                }
+                KTimer k(RL2_K_STEP);
+                launch_tile_step(g_stream, tg[0], pass);
             }
             if (ng == 2) {
                 hipEventRecord(g_join, g_stream2);

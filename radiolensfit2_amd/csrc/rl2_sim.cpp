@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <vector>
 
 namespace {
@@ -17,10 +18,9 @@ constexpr double kSidereal = 2.0 * kPi / 86164.0905;  // rad per second of time
 
 // default_params.h:38-62
 constexpr double kRmin = 0.3, kRmax = 3.5, kAdd = -0.93, kEsp = 0.33, kRstd = 0.3136;
-constexpr double kMexp = -1.34, kNormS = 3.0825;
+constexpr double kMexp = -1.34;
 constexpr double kEmax = 0.804, kE0 = 0.0732, kDisp = 0.2298, kNormE = 2.595;
-constexpr int kNumOrient = 5, kNPoints = 1000, kJmax = 30;
-constexpr double kEps = 1.0e-5;
+constexpr int kNumOrient = 5, kNPoints = 1000;
 
 struct MT19937 {
     static constexpr int N = 624, M = 397;
@@ -95,97 +95,68 @@ double e_pdf(double e)
 {
     return kNormE * e * (1. - exp((e - kEmax) / kDisp)) / ((1. + e) * sqrt(e * e + kE0 * kE0));
 }
-// distributions.c:49-56
-double flux_cdf(double alpha, double flux)
-{
-    const double a = alpha + 1.;
-    return kNormS * pow(flux, a) / a;
-}
 double scale_mean(double flux) { return kAdd + kEsp * log(flux); }
-// distributions.c:94-101
-double r_cdf(double mean, double r)
+
+// ---- sampling of the catalog priors (distributions.c:35-129 define the densities) ----
+// Flux: density proportional to S^kMexp on [fmin, fmax]; the inverse of its cumulative
+// distribution is closed-form, S = (fmin^a + u (fmax^a - fmin^a))^(1/a) with a = kMexp + 1.
+double draw_flux(MT19937 &gen, double fmin, double fmax)
 {
-    double x = (log(r) - mean) / (sqrt(2.) * kRstd);
-    return (1. + erf(x)) * 0.5;
-}
-// distributions.c:106-129 extended trapezoidal rule
-double cdf(double (*pdf)(double), double b)
-{
-    double x, tnm, del, sum;
-    double s, olds = 0.;
-    int it, j, k;
-    s = 0.5 * b * (*pdf)(b);
-    for (k = 2; k <= kJmax; k++) {
-        for (it = 1, j = 1; j < k - 1; j++) it <<= 1;
-        tnm = (double)it;
-        del = b / tnm;
-        x = 0.5 * del;
-        for (sum = 0., j = 1; j < it; j++, x += del) sum += (*pdf)(x);
-        s = 0.5 * (s + del * sum);
-        if (k > 5)
-            if (fabs(s - olds) < kEps * fabs(olds) || (s == 0.0 && olds == 0.0)) return s;
-        olds = s;
-    }
-    return 0.0;
+    const double a = kMexp + 1.;
+    const double lo = pow(fmin, a), hi = pow(fmax, a);
+    return pow(lo + gen.uniform() * (hi - lo), 1. / a);
 }
 
-// generate_random_values.c:40-72
-void inverse_cdf_draw(MT19937 &gen, int nr, double *data, double min_value, double max_value,
-                 double (*cdfunc)(double, double), double param)
+// Scalelength: log-normal of mean log `mean` and width kRstd, truncated to [kRmin, kRmax].
+// The target probability is placed between the two truncation points, and log r is found by
+// bisection on the normal cumulative distribution (60 halvings: below one ulp of log r).
+double draw_scale(MT19937 &gen, double mean)
 {
-    const int N = kNPoints;
-    std::vector<double> F(N + 1);
-    double h = (max_value - min_value) / N;
-    double CFmin = cdfunc(param, min_value);
-    double CFrange = cdfunc(param, max_value) - CFmin;
-    F[0] = 0.;
-    for (int i = 1; i <= N; i++) F[i] = cdfunc(param, min_value + i * h) - CFmin;
-    for (int i = 0; i < nr; i++) {
-        double u = gen.uniform() * CFrange;
-        int k = 0;
-        while ((k <= N) && (u > F[k])) k++;
-        if (k > N) k = N;
-        if (k < 1) k = 1;
-        data[i] = min_value + h * (k - 1) + h * (u - F[k - 1]) / (F[k] - F[k - 1]);
+    auto Phi = [&](double logr) { return 0.5 * (1. + erf((logr - mean) / (sqrt(2.) * kRstd))); };
+    double lo = log(kRmin), hi = log(kRmax);
+    const double target = Phi(lo) + gen.uniform() * (Phi(hi) - Phi(lo));
+    for (int it = 0; it < 60; it++) {
+        const double mid = 0.5 * (lo + hi);
+        (Phi(mid) < target ? lo : hi) = mid;
     }
+    return exp(0.5 * (lo + hi));
 }
 
-// generate_random_values.c:74-121
-void random_ellipticity(MT19937 &gen, int ne, int NP, double *e1, double *e2)
+// Ellipticity modulus: the cumulative distribution of e_pdf on kNPoints intervals of
+// [0, kEmax], each interval's integral by 8-point Gauss-Legendre quadrature (the density is
+// smooth there), normalised to end at 1; a draw is the linear interpolation of the inverse
+// on the interval that binary search finds.
+const std::vector<double> &e_cdf_table()
 {
-    const int N = kNPoints;
-    const double h = kEmax / N;
-    // The table depends on constants only.  The reference's trapezoid refinement adds
-    // it-1 midpoints per level (distributions.c:117), so it converges slowly and often
-    // runs all JMAX levels: build it once per process, in parallel as the reference does
-    // (generate_random_values.c:85).
     static const std::vector<double> F = [] {
+        static const double xg[4] = {0.1834346424956498, 0.5255324099163290, 0.7966664774136267,
+                                     0.9602898564975363};
+        static const double wg[4] = {0.3626837833783620, 0.3137066458778873, 0.2223810344533745,
+                                     0.1012285362903763};
         std::vector<double> t(kNPoints + 1, 0.);
-        const double hh = kEmax / kNPoints;
-#pragma omp parallel for schedule(dynamic, 4)
-        for (int i = 1; i <= kNPoints; i++) t[i] = cdf(e_pdf, i * hh);
+        const double h = kEmax / kNPoints;
+        for (int i = 0; i < kNPoints; i++) {
+            const double c = (i + 0.5) * h, r = 0.5 * h;
+            double q = 0.;
+            for (int j = 0; j < 4; j++) q += wg[j] * (e_pdf(c - r * xg[j]) + e_pdf(c + r * xg[j]));
+            t[i + 1] = t[i] + r * q;
+        }
+        const double total = t[kNPoints];
+        for (double &v : t) v /= total;
         return t;
     }();
-    double inc = (double)kPi / NP;
-    int ind = 0;
-    for (int i = 0; i < ne; i++) {
-        double u = gen.uniform();
-        int k = 0;
-        while (k <= N && u > F[k]) k++;
-        if (k > N) k = N;  // the tabulated CDF ends slightly below 1
-        if (k < 1) k = 1;
-        double module = h * (k - 1) + h * (u - F[k - 1]) / (F[k] - F[k - 1]);
-        double phi_0 = 2 * kPi * gen.uniform_pos();
-        for (k = 0; k < NP; k++) {
-            double phi = phi_0 + k * inc;
-            e1[ind] = module * cos(phi);
-            e2[ind] = module * sin(phi);
-            ind++;
-            e1[ind] = -e1[ind - 1];
-            e2[ind] = -e2[ind - 1];
-            ind++;
-        }
-    }
+    return F;
+}
+
+double draw_modulus(MT19937 &gen)
+{
+    const std::vector<double> &F = e_cdf_table();
+    const double u = gen.uniform();
+    const size_t k = std::min<size_t>(
+        kNPoints, std::max<size_t>(1, std::upper_bound(F.begin(), F.end(), u) - F.begin()));
+    const double h = kEmax / kNPoints;
+    const double span = F[k] - F[k - 1];
+    return h * (k - 1) + (span > 0. ? h * (u - F[k - 1]) / span : 0.);
 }
 
 // Exponential-disk visibility of one galaxy in one channel (galaxy_visibilities.c:154-190).
@@ -297,21 +268,31 @@ unsigned int rl2sim_catalog(unsigned int nge, unsigned long seed, double fov_arc
     const int NP = kNumOrient;
     unsigned int diffgal = nge / (2 * NP);
     unsigned int my_gal = diffgal * 2 * NP;
+    // one flux, scalelength and ellipticity modulus per group of 2 NP galaxies: NP
+    // orientations, each with its opposite, so that the group's intrinsic shapes average to
+    // zero (the catalog layout of generate_catalog.c:97-117); groups in decreasing flux
     std::vector<double> f2(diffgal);
-    inverse_cdf_draw(gen, (int)diffgal, f2.data(), fmin, fmax, flux_cdf, kMexp);
-    std::sort(f2.begin(), f2.end());
-    unsigned int gal = 0;
-    for (unsigned int i = 0; i < diffgal; i++)
-        for (int k = 0; k < 2 * NP; k++) gflux[gal++] = f2[diffgal - i - 1];
-    double scalelength = 0.;
-    for (unsigned int g = 0; g < my_gal; g++) {
-        if (g % (2 * NP) == 0) {
-            double mu = scale_mean(gflux[g]);
-            inverse_cdf_draw(gen, 1, &scalelength, kRmin, kRmax, r_cdf, mu);
+    for (unsigned int i = 0; i < diffgal; i++) f2[i] = draw_flux(gen, fmin, fmax);
+    std::sort(f2.begin(), f2.end(), std::greater<double>());
+    for (unsigned int grp = 0; grp < diffgal; grp++) {
+        const double r = draw_scale(gen, scale_mean(f2[grp]));
+        for (int k = 0; k < 2 * NP; k++) {
+            gflux[grp * 2 * NP + k] = f2[grp];
+            gscale[grp * 2 * NP + k] = r;
         }
-        gscale[g] = scalelength;
     }
-    random_ellipticity(gen, (int)diffgal, NP, ge1, ge2);
+    for (unsigned int grp = 0; grp < diffgal; grp++) {
+        const double modulus = draw_modulus(gen);
+        const double phi0 = 2 * kPi * gen.uniform_pos();
+        for (int k = 0; k < NP; k++) {
+            const double phi = phi0 + k * (kPi / NP);
+            const unsigned int g = grp * 2 * NP + 2 * k;
+            ge1[g] = modulus * cos(phi);
+            ge2[g] = modulus * sin(phi);
+            ge1[g + 1] = -ge1[g];
+            ge2[g + 1] = -ge2[g];
+        }
+    }
     for (unsigned int g = 0; g < my_gal; g++) {
         double radius = sqrt(gen.uniform()) * 0.5 * fov_eff;
         double orient = gen.uniform() * 2 * kPi;

@@ -289,3 +289,38 @@ def test_noise_generator_philox_known_answers():
                              out.ctypes.data_as(N.c_uint_p))
     for i in range(64):
         assert [int(x) for x in out[i]] == _philox_py([int(x) for x in ctr[i]], (12345, 0))
+
+
+def test_catalog_draws_follow_the_priors():
+    """Kolmogorov-Smirnov distance of the catalog's draws (one per group of ten) from the
+    priors they sample (distributions.c:35-101): the flux power law S^-1.34 on [10, 200] uJy,
+    the truncated log-normal scalelength around scale_mean(flux), the ellipticity modulus
+    density e_pdf on [0, 0.804]; each below the 0.1% critical value."""
+    from scipy import integrate, special
+    cat = synth.make_catalog(30000, seed=21, fmin=10.0, fmax=200.0)
+    flux, scale = cat.flux[::10], cat.scale[::10]
+    mod = np.hypot(cat.e1[::10], cat.e2[::10])
+    n = len(flux)
+    crit = 1.95 / math.sqrt(n)
+
+    def ks(x, cdf):
+        x = np.sort(x)
+        f = cdf(x)
+        i = np.arange(1, len(x) + 1)
+        return max((i / len(x) - f).max(), (f - (i - 1) / len(x)).max())
+
+    a = -0.34
+    assert ks(flux, lambda s: (s ** a - 10 ** a) / (200 ** a - 10 ** a)) < crit
+    # scalelength: transform each draw to its own truncated-normal probability (the mean
+    # depends on the galaxy's flux), which must be uniform
+    mu = -0.93 + 0.33 * np.log(flux)
+    phi = lambda x: 0.5 * (1 + special.erf((x - mu) / (math.sqrt(2) * 0.3136)))  # noqa: E731
+    u = (phi(np.log(scale)) - phi(np.log(0.3))) / (phi(np.log(3.5)) - phi(np.log(0.3)))
+    assert ks(u, lambda x: x) < crit + 1e-6 / 0.05           # %f rounding of the catalog
+    emax, e0, disp = 0.804, 0.0732, 0.2298
+    pdf = lambda e: e * (1 - np.exp((e - emax) / disp)) / ((1 + e) * np.sqrt(e * e + e0 * e0))  # noqa: E731
+    grid = np.linspace(0, emax, 4001)
+    F = np.concatenate([[0], np.cumsum([integrate.quad(pdf, grid[k], grid[k + 1])[0]
+                                        for k in range(len(grid) - 1)])])
+    F /= F[-1]
+    assert ks(mod, lambda x: np.interp(x, grid, F)) < crit
