@@ -9,9 +9,9 @@ path is RadioLensfit2's: extract -> fit (simplex + sampling) -> subtract, galaxi
 decreasing flux order, failed fits re-fitted after the others.
 
 A step is one batch of --batch galaxies through that path.  The timed galaxies are every
---stride-th galaxy of the flux-ordered catalog (default every 10th: 100 galaxies spanning
-the whole flux and size range), processed in flux order against the residual of the full
-catalog.  The residual is checkpointed after set-up; --warmup steps run and the checkpoint
+--stride-th galaxy of the flux-ordered catalog (default stride: catalog size // steps, so
+any step count spans the whole flux and size range: every 10th at 100 steps, every 50th at
+20), processed in flux order against the residual of the full catalog.  The residual is checkpointed after set-up; --warmup steps run and the checkpoint
 is restored, so every configuration times exactly --steps steps over the same galaxies
 (their bad-list re-fit included), inputs resident in HBM.  The same galaxies are then run
 again with per-launch HIP events on the library stream for the kernel rooflines.  One JSON
