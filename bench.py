@@ -483,16 +483,20 @@ def run_config1(ngal=100):
             "bad_fits": {"first_pass": st.n_bad_first, "after_refit": st.n_bad_final}}
 
 
-def roofline(kstats, lik_peak):
-    """Rooflines of the two kernels that bound a galaxy: the likelihood's chunk-sum kernel
-    (lik_tile_kernel, FP64 VALU) and the extraction's gridding kernel (grid_run_kernel, HBM).
+def roofline(kstats, lik_peak, fused=True):
+    """Rooflines of the two kernels that bound a galaxy: the likelihood's chunk-sum kernels
+    (lik_tile_kernel for pass 0, and lik_fused_kernel for the Newton passes when the steps
+    are folded into them, else lik_tile_kernel; FP64 VALU) and the extraction's gridding
+    kernel (grid_run_kernel, HBM).
     achieved = the device-counted algorithmic FLOP-equivalents (bytes) of the timed launches
     / their summed durations from HIP events around each launch on the library stream."""
     lik, ext, step = kstats["likelihood"], kstats["extract"], kstats["step"]
     tr = measured_traffic()
     lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
     ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
-    r_lik = {"kernel": "lik_tile_kernel (chunk sums of the zero-offset and Newton passes)",
+    r_lik = {"kernel": ("lik_tile_kernel + lik_fused_kernel (chunk sums of the zero-offset "
+                        "and Newton passes, each Newton pass taking the previous step)" if fused
+                        else "lik_tile_kernel (chunk sums of the zero-offset and Newton passes)"),
              "bound": "valu", "achieved": round(lik_tf, 4),
              "peak": lik_peak, "unit": "TFLOP/s", "frac": round(lik_tf / lik_peak, 5),
              "traffic": tr.get("likelihood_tile_launch_bytes"), "launches": lik["launches"],
@@ -547,6 +551,12 @@ def main():
                          "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")
     ap.add_argument("--item-groups", type=int, default=0,
                     help="work-groups a likelihood item's chunking aims at (0: 512)")
+    ap.add_argument("--fuse-step", type=int, default=1, choices=[0, 1],
+                    help="Newton steps folded into the next pass (1) or a step kernel per "
+                         "pass (0)")
+    ap.add_argument("--tile-passes", type=int, default=0,
+                    help="Newton passes queued before an evaluation's status check "
+                         "(0: the previous evaluation's largest step count + 1)")
     ap.add_argument("--tile-blocks", type=int, default=0,
                     help="cap on the blocks of a likelihood pass (multiple of 8; 0: one per chunk)")
     ap.add_argument("--tile-streams", type=int, default=1, choices=[1, 2],
@@ -621,6 +631,8 @@ def main():
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_BLOCKS, args.tile_blocks), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_STREAMS, args.tile_streams), "rl2_tune_set")
     N.check(N.hip().rl2_tune_set(N.RL2_TUNE_ITEM_GROUPS, args.item_groups), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_FUSE_STEP, args.fuse_step), "rl2_tune_set")
+    N.check(N.hip().rl2_tune_set(N.RL2_TUNE_TILE_PASSES, args.tile_passes), "rl2_tune_set")
     opts = dict(batch=B, do_refit=1, nm_speculate=0 if args.no_speculate else 1,
                 line_chunk=args.line_chunk, lik_block=args.lik_block,
                 lik_precision=args.lik_precision)
@@ -641,7 +653,7 @@ def main():
         log(f"[bench]   {k:10s} {s['ms']:10.2f} ms  {s['launches']:7d} launches")
     mixed = args.lik_precision != 0 or res_prec != N.RL2_PREC_FP64
     peak = PEAK_FP64_TFLOPS if args.lik_precision == 0 else PEAK_FP32_TFLOPS
-    dominant, r_lik, r_ext = roofline(kstats, peak)
+    dominant, r_lik, r_ext = roofline(kstats, peak, fused=bool(args.fuse_step))
 
     accept = None
     if args.compare_fp64:

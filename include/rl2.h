@@ -222,11 +222,18 @@ int rl2_synchronize(void);
  *   them one by one.  Both give identical results.
  * RL2_TUNE_ITEM_GROUPS: work-groups the per-facet chunk choice aims at for one likelihood
  *   item (RL2_TUNE_TILE_CELLS 0): the chunk is the largest power of two from 2048 cells that
- *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 512. */
+ *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 512.
+ * RL2_TUNE_FUSE_STEP: 1 (default) folds each Newton step of the likelihood into the next
+ *   pass's launch (its work-groups take the step and make their own phase tables), so a pass
+ *   is one launch; 0 runs a step kernel after every pass.  Both give identical results.
+ * RL2_TUNE_TILE_PASSES: Newton passes queued before the first status check of an evaluation
+ *   (1..50); 0 (default): the previous evaluation's largest step count + 1.  Passes beyond a
+ *   task's last step leave it at once, and missing ones are queued after the check, so the
+ *   results do not depend on it. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
        RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5,
        RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7, RL2_TUNE_BATCH_GRAPH = 8,
-       RL2_TUNE_ITEM_GROUPS = 9 };
+       RL2_TUNE_ITEM_GROUPS = 9, RL2_TUNE_FUSE_STEP = 10, RL2_TUNE_TILE_PASSES = 11 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

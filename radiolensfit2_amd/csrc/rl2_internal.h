@@ -205,6 +205,10 @@ struct Tuning {
                                // (0: kTileItemGroups)
     int batch_graph = 1;       // RL2_TUNE_BATCH_GRAPH: replay a batch's extraction launches
                                // as a hipGraph (1) or launch them one by one (0)
+    int fuse_step = 1;         // RL2_TUNE_FUSE_STEP: Newton steps folded into the next pass
+                               // (1) or a step kernel after every pass (0)
+    int tile_passes = 0;       // RL2_TUNE_TILE_PASSES: Newton passes queued before the first
+                               // status check (0: the previous evaluation's largest count + 1)
 };
 Tuning &tuning();
 
@@ -304,8 +308,17 @@ struct TileLaunch {
     unsigned n_inl;
     LikItem *items_w;
     LikItem inl[kInlItems];
+    // Two generations of task state and chunk sums (fused passes, and the step kernel after
+    // them): the state is read from `tasks` and written to `tasks_out` (null: `tasks`), the
+    // previous pass's sums read from `part_in` while this pass's go to `partials`.
+    TileTask *tasks_out = nullptr;
+    const double *part_in = nullptr;
+    int mode = 0;              // fused passes: 0 first step (from the pass-0 sums), 1 Newton
+                               // step, 2 resume from the state in `tasks`
 };
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
+// One Newton pass with the previous pass's step folded in (grid items, row tables in LDS).
+void launch_tile_fused(hipStream_t s, const TileLaunch &t);
 // The two launches of a pass separately: the chunk sums, then the per-task step.
 void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass);
 void launch_tile_step(hipStream_t s, const TileLaunch &t, int pass);

@@ -2149,11 +2149,14 @@ struct TabCells {
     double dr[kTabU], di[kTabU];
 };
 
-template <bool YL>
-__device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
-                                       const double2 *X, const double2 *Yg, unsigned gf,
-                                       double2 *sX, double2 *sY, double *sC, unsigned beg,
-                                       unsigned end, double (&s)[7])
+// `stage` fills sX, sC and (YL) sY and ends with a barrier; it runs after the first cells'
+// loads are issued, so their round trip overlaps it.  It returns false (the same in every
+// thread) when the work-group has nothing to sum; the function then returns false too.
+template <bool YL, class Stage>
+__device__ inline bool tile_sums64_tab(const FacetDev &F, const LikItem &it, double ro,
+                                       const Stage &stage, const double2 *Yg, double2 *sX,
+                                       double2 *sY, double *sC, unsigned beg, unsigned end,
+                                       double (&s)[7])
 {
     const double e1 = it.e1, e2 = it.e2;
     const double detA = 1. - e1 * e1 - e2 * e2;
@@ -2226,13 +2229,7 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     // the first cells are requested before the tables are staged in LDS, so the two
     // memory round trips at the start of the work-group overlap
     fetch(A, beg + threadIdx.x);
-    const double *ctr = F.ctr;
-    for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
-        sX[k] = gload(X, k);
-        sC[k] = gload(ctr, k);
-        if (YL) sY[k] = gload(Yg, k);
-    }
-    __syncthreads();
+    if (!stage()) return false;
 #pragma unroll
     for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.c[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
     for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
@@ -2250,6 +2247,7 @@ __device__ inline void tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     s[3] *= kTwoPi2;
     s[4] *= kTwoPi2;
     s[5] *= kTwoPi2;
+    return true;
 }
 
 // The same sums from the FP32 facet copy: model and terms in FP32, the per-thread FP32
@@ -2365,16 +2363,93 @@ __device__ int newton_update(TileTask &T, const double (&v)[7], double *ho_out,
 
 // newton_update on the task's state in memory, writing L_r when the task finishes (the
 // step kernel of the two-kernel passes).
+// The state is read from T and written to To (the same task in another buffer when the
+// passes keep two generations of state; T itself otherwise).
 template <int PASS>
-__device__ int tile_step(const LikLaunch &a, TileTask *T, int item, int r,
-                         const double (&v)[7])
+__device__ int tile_step(const LikLaunch &a, const TileTask *T, TileTask *To, int item, int r,
+                         const double (&v)[7], double *xo, double *yo)
 {
     TileTask S = *T;
     double ho, det;
     const int done = newton_update<PASS>(S, v, &ho, &det);
-    *T = S;
+    *To = S;
     if (done) finish_lr(a, item, r, ho, det, S.iter);
+    *xo = S.xo;
+    *yo = S.yo;
     return done;
+}
+
+// A task's chunk sums of one pass added in chunk order by one wave: lanes over chunks, then
+// across lanes into lane 63.  P0 is the task's first chunk; chunk c follows at c * nr * 8.
+__device__ inline void task_sums(const double *P0, unsigned nch, int nr, int lane,
+                                 double (&v)[7])
+{
+#pragma unroll
+    for (int j = 0; j < 7; j++) v[j] = 0.;
+    for (unsigned c = lane; c < nch; c += 64)
+#pragma unroll
+        for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * nr * 8 + j];
+#pragma unroll
+    for (int j = 0; j < 7; j++) v[j] = wave_sum_last(v[j]);  // totals in lane 63
+}
+
+// Newton phase tables e^{2 pi i o ctr[k]} (o = xo for the columns, yo for the rows) by runs
+// of kTabRun grid indices.  The cell centres are evenly spaced, ctr[k] = (k + 0.5) inc - len
+// (evaluate_uv_grid.c:127-128), so ctr[run * kTabRun + j] - ctr[run * kTabRun] equals
+// ctr[j] - ctr[0] to within a few ulp of len: entry k is the product of the run's first
+// entry E[k / kTabRun] (sincos_rev64) and W[k % kTabRun] = e^{2 pi i o (ctr[j] - ctr[0])}, one
+// sin/cos per run and per run offset instead of one per entry.  The step kernel and the
+// fused passes build their tables this same way, so both give the same entries.
+constexpr unsigned kTabRun = 16;
+template <unsigned NRUN>
+struct TabRot {
+    double2 ex[NRUN], ey[NRUN], wx[kTabRun], wy[kTabRun];
+};
+__device__ inline double2 cis_rev(double t)
+{
+    double sn, cs;
+    sincos_rev64(t, sn, cs);
+    return make_double2(cs, sn);
+}
+__device__ inline double2 tab_entry(const double2 &e, const double2 &w)
+{
+    return make_double2(fma(e.x, w.x, -(e.y * w.y)), fma(e.y, w.x, e.x * w.y));
+}
+// The runs' first entries (columns: every run of the gf columns; rows: runs yb0..yb1 and,
+// with y0, run 0) and both W tables, spread over the block's threads; the caller
+// synchronises before reading R.
+template <unsigned NRUN>
+__device__ inline void tab_rot_stage(TabRot<NRUN> &R, const double *ctr, unsigned gf, double xo,
+                                     double yo, unsigned yb0, unsigned yb1, bool y0)
+{
+    const unsigned nb = (gf + kTabRun - 1) / kTabRun;
+    const unsigned ny = yb1 - yb0 + 1 + (y0 ? 1u : 0u);
+    const unsigned nw = min(gf, kTabRun);
+    const double c0 = ctr[0];
+    for (unsigned q = threadIdx.x; q < nb + ny + 2 * nw; q += blockDim.x) {
+        if (q < nb) {
+            R.ex[q] = cis_rev(xo * ctr[q * kTabRun]);
+        } else if (q < nb + ny) {
+            const unsigned b = q - nb <= yb1 - yb0 ? yb0 + (q - nb) : 0u;
+            R.ey[b] = cis_rev(yo * ctr[b * kTabRun]);
+        } else {
+            const unsigned w = q - nb - ny, j = w % nw;
+            if (w < nw)
+                R.wx[j] = cis_rev(xo * (ctr[j] - c0));
+            else
+                R.wy[j] = cis_rev(yo * (ctr[j] - c0));
+        }
+    }
+}
+template <unsigned NRUN>
+__device__ inline double2 tab_x(const TabRot<NRUN> &R, unsigned k)
+{
+    return tab_entry(R.ex[k / kTabRun], R.wx[k % kTabRun]);
+}
+template <unsigned NRUN>
+__device__ inline double2 tab_y(const TabRot<NRUN> &R, unsigned k)
+{
+    return tab_entry(R.ey[k / kTabRun], R.wy[k % kTabRun]);
 }
 
 // The chunk sums of a tile work-group into P[0..N-1]: each wave's total from lane 63, then
@@ -2469,11 +2544,21 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
                 double2 *sX = tab_lds;
                 double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
                 double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
-                if (t.tab_y_lds)
-                    tile_sums64_tab<true>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+                const bool yl = t.tab_y_lds;
+                auto copy = [&]() {  // the step kernel's tables and the cell centres
+                    const double *ctr = F.ctr;
+                    for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+                        sX[k] = gload(X, k);
+                        sC[k] = gload(ctr, k);
+                        if (yl) sY[k] = gload(Y, k);
+                    }
+                    __syncthreads();
+                    return true;
+                };
+                if (yl)
+                    tile_sums64_tab<true>(F, it, a.rg.ro[r], copy, Y, sX, sY, sC, beg, end, s);
                 else
-                    tile_sums64_tab<false>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end,
-                                           s);
+                    tile_sums64_tab<false>(F, it, a.rg.ro[r], copy, Y, sX, sY, sC, beg, end, s);
             } else {
                 double *sC = reinterpret_cast<double *>(tab_lds);  // tab_lds_facet * 8 bytes
                 for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS)
@@ -2494,10 +2579,21 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
             double2 *sX = tab_lds;
             double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
             double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
-            if (t.tab_y_lds)
-                tile_sums64_tab<true>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+            const bool yl = t.tab_y_lds;
+            auto copy = [&]() {
+                const double *ctr = F.ctr;
+                for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+                    sX[k] = gload(X, k);
+                    sC[k] = gload(ctr, k);
+                    if (yl) sY[k] = gload(Y, k);
+                }
+                __syncthreads();
+                return true;
+            };
+            if (yl)
+                tile_sums64_tab<true>(F, it, a.rg.ro[r], copy, Y, sX, sY, sC, beg, end, s);
             else
-                tile_sums64_tab<false>(F, it, a.rg.ro[r], X, Y, gf, sX, sY, sC, beg, end, s);
+                tile_sums64_tab<false>(F, it, a.rg.ro[r], copy, Y, sX, sY, sC, beg, end, s);
         } else if (PASS == 0 && it.prec == 0 && F.cid && F.grid_facet <= t.tab_lds_facet) {
             extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 8 bytes: ctr
             double *sC = reinterpret_cast<double *>(tab_lds);
@@ -2531,6 +2627,7 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     const unsigned k = blockIdx.x;
     const int item = (int)(k / nr), r = (int)(k % nr) + t.r_lo;
     TileTask *T = t.tasks + (size_t)item * kMaxR + r;
+    TileTask *To = (t.tasks_out ? t.tasks_out : t.tasks) + (size_t)item * kMaxR + r;
     // the item and its facet fields are requested by every wave at once, so their latency
     // overlaps the reduction instead of following the barrier
     const LikItem it = a.items[item];
@@ -2544,18 +2641,12 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
         if (PASS == 0 || T->active) {  // read by the whole wave before lane 63 updates it
             unsigned first, nch;
             tile_range(t, a.items, item, &first, &nch);
-            const double *P0 = t.partials + (size_t)(first + (unsigned)(r - t.r_lo)) * 8;
-            double v[7] = {0., 0., 0., 0., 0., 0., 0.};
-            for (unsigned c = lane; c < nch; c += 64)
-#pragma unroll
-                for (int j = 0; j < 7; j++) v[j] += P0[(size_t)c * nr * 8 + j];
-#pragma unroll
-            for (int j = 0; j < 7; j++) v[j] = wave_sum_last(v[j]);  // totals in lane 63
-            if (lane == 63) {
-                go = !tile_step<PASS>(a, T, item, r, v);
-                s_off[0] = T->xo;
-                s_off[1] = T->yo;
-            }
+            double v[7];
+            task_sums(t.partials + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch, nr, lane,
+                      v);
+            if (lane == 63) go = !tile_step<PASS>(a, T, To, item, r, v, &s_off[0], &s_off[1]);
+        } else if (lane == 63 && To != T) {
+            *To = *T;  // a finished task keeps its state in the other generation too
         }
         if (lane == 63) s_go = go;
     }
@@ -2563,13 +2654,13 @@ __global__ __launch_bounds__(kStepBS) void lik_step_kernel(TileLaunch t)
     if (!s_go || !t.tab) return;
     if (it.prec != 0 || !fcid) return;
     double2 *X = t.tab + it.tab_first + (size_t)(r - 1) * 2 * gf;
-    const double xo = s_off[0], yo = s_off[1];
-    for (unsigned q = threadIdx.x; q < 2 * gf; q += blockDim.x) {
-        const bool col = q < gf;
-        double sn, cs;
-        sincos_rev64((col ? xo : yo) * fctr[col ? q : q - gf], sn, cs);
-        X[q] = make_double2(cs, sn);
-    }
+    // every column and every row (gf <= 1706: the host's LDS limit for the column tables)
+    __shared__ TabRot<128> R;
+    const unsigned nb = (gf + kTabRun - 1) / kTabRun;
+    tab_rot_stage(R, fctr, gf, s_off[0], s_off[1], 0u, nb - 1, false);
+    __syncthreads();
+    for (unsigned q = threadIdx.x; q < 2 * gf; q += blockDim.x)
+        X[q] = q < gf ? tab_x(R, q) : tab_y(R, q - gf);
 }
 
 
@@ -2601,6 +2692,137 @@ __device__ void tile_status(const LikLaunch &a)
         a.status[1] = m;
         *a.flops = f;
     }
+}
+
+// ------------------------------------------------------------------ Newton passes, fused
+// One Newton pass with the previous pass's step folded in, so that a pass is one launch
+// instead of two.  Each work-group first adds its task's chunk sums of the previous pass
+// (wave 0, in the step kernel's order) and takes the Newton step itself, so every
+// work-group of a task holds the same state bit for bit.  The task's chunk-0 work-group
+// stores that state to the other generation of the task buffer (the rest of the pass still
+// reads this one) and writes L_r when the task ends.  The work-group then evaluates its own
+// phase tables in LDS with the step kernel's arithmetic: the column table over the facet's
+// width, the row table over the rows its chunk spans (cells are in row-major order) and row
+// 0, which the zero pad cell reads.  t.mode: 0 the first step, from the pass-0 sums; 1 a
+// Newton step from the previous pass's sums; 2 resume from the state the step kernel left.
+template <int KIND>
+__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(TileLaunch t)
+{
+    __shared__ double red[(kTileBS / 64) * 8];
+    __shared__ int s_go;
+    __shared__ double s_off[2];
+    // one block per logical work-group, in the XCD-aware order of lik_tile_kernel
+    const unsigned per = (t.nwg + kXcds - 1) / kXcds;
+    const unsigned wg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (wg >= t.nwg) return;
+    const LikLaunch &a = t.lk;
+    const LikItem *items = KIND == 2 ? t.inl : a.items;
+    const int item = tile_item(items, a.nitems, wg, t.nr_g);
+    unsigned first, nch;
+    tile_range(t, items, item, &first, &nch);
+    const unsigned local = wg - first;
+    const unsigned chunk = local / (unsigned)t.nr_g;
+    const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
+    const size_t ti = (size_t)item * kMaxR + r;
+    const LikItem it = items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned beg = chunk * it.chunk_cells;
+    const unsigned end = min(F.n, beg + it.chunk_cells);
+    const unsigned gf = F.grid_facet;  // <= 1024: the host checks the LDS tables fit
+    extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 40 bytes: X, ctr, Y
+    double2 *sX = tab_lds;
+    double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
+    double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
+    const uint32_t *cid = F.cid;
+    const double *ctr = F.ctr;
+    // Runs after the first cells' loads are issued (inside tile_sums64_tab): the cell
+    // centres this thread's table entries need are requested too, then wave 0 adds the
+    // previous pass's chunk sums and takes the step while those loads are in flight.
+    __shared__ TabRot<1024 / kTabRun> R;  // gf <= 1024
+    auto make = [&]() -> bool {
+        constexpr int NC = 1024 / kTileBS;
+        double c[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            const unsigned k = threadIdx.x + j * kTileBS;
+            c[j] = k < gf ? gload(ctr, k) : 0.;
+        }
+        const unsigned row_lo = end > beg ? gload(cid, beg) >> 16 : 0u;
+        const unsigned row_hi = end > beg ? gload(cid, end - 1) >> 16 : 0u;
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            const TileTask *T = t.tasks + ti;
+            int go = 0;
+            double xo = 0., yo = 0.;
+            if (t.mode == 2) {
+                if (lane == 63) {
+                    go = T->active;
+                    xo = T->xo;
+                    yo = T->yo;
+                }
+            } else if (t.mode == 0 || T->active) {
+                double v[7];
+                task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch,
+                          t.nr_g, lane, v);
+                if (lane == 63) {
+                    TileTask S = t.mode == 0 ? TileTask{} : *T;
+                    double ho, det;
+                    const int done = t.mode == 0 ? newton_update<0>(S, v, &ho, &det)
+                                                 : newton_update<1>(S, v, &ho, &det);
+                    if (chunk == 0) {
+                        t.tasks_out[ti] = S;
+                        if (done) finish_lr(a, item, r, ho, det, S.iter);
+                    }
+                    go = !done;
+                    xo = S.xo;
+                    yo = S.yo;
+                }
+            } else if (lane == 63 && chunk == 0) {
+                t.tasks_out[ti] = *T;  // a finished task keeps its state in both generations
+            }
+            if (lane == 63) {
+                s_go = go;
+                s_off[0] = xo;
+                s_off[1] = yo;
+            }
+        }
+        __syncthreads();
+        if (!s_go) return false;
+        // the column runs, the runs of the rows the chunk spans (cells are in row-major
+        // order) and run 0 (row 0: the zero pad cell's), then the entries
+        const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
+        tab_rot_stage(R, ctr, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            const unsigned k = threadIdx.x + j * kTileBS;
+            if (k < gf) {
+                sC[k] = c[j];
+                sX[k] = tab_x(R, k);
+            }
+        }
+        const unsigned span = row_hi - row_lo + 1;
+        for (unsigned q = threadIdx.x; q < span + (row_lo > 0 ? 1u : 0u); q += kTileBS) {
+            const unsigned k = q < span ? row_lo + q : 0u;
+            sY[k] = tab_y(R, k);
+        }
+        __syncthreads();
+        return true;
+    };
+    double s[7];
+    if (tile_sums64_tab<true>(F, it, a.rg.ro[r], make, (const double2 *)nullptr, sX, sY, sC,
+                              beg, end, s))
+        tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
+}
+
+void launch_tile_fused(hipStream_t s, const TileLaunch &t)
+{
+    const unsigned g = tile_grid(t.nwg, 0);
+    const unsigned lds = t.tab_lds_facet * 40;
+    if (t.grid == 2)
+        lik_fused_kernel<2><<<g, kTileBS, lds, s>>>(t);
+    else
+        lik_fused_kernel<1><<<g, kTileBS, lds, s>>>(t);
 }
 
 void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass)

@@ -162,8 +162,9 @@ bool LikBuffers::ensure(int n)
     bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
               dmalloc(&d_nit, (size_t)c * kMaxR) &&
               dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts) &&
-              dmalloc(&d_tasks, (size_t)c * kMaxR);
-    ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * sizeof(TileTask), g_stream),
+              dmalloc(&d_tasks, (size_t)c * kMaxR * 2);  // two generations of task state
+    ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * 2 * sizeof(TileTask),
+                                     g_stream),
                       "memset");
     // the results block lives in pinned host memory that the kernels write directly: the
     // host reads it after the evaluation's one synchronisation, with no copy launch
@@ -742,7 +743,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             chunks += std::max(1u, (cells + chunk - 1) / chunk);
         }
         const unsigned total = chunks * (unsigned)nr;
-        if (!b.ensure_partials(total)) return false;
+        if (!b.ensure_partials(2 * (size_t)total)) return false;  // two generations
         // phase tables for FP64 items on grid facets: 2 * grid_facet entries per task
         size_t ntab = 0;
         unsigned tab_facet = 0;
@@ -814,10 +815,82 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 hipStreamWaitEvent(g_stream, g_join, 0);
             }
         };
+        // Fused passes (RL2_TUNE_FUSE_STEP): every pass after pass 0 takes the previous
+        // pass's Newton step in its own work-groups, and one step kernel after the last
+        // queued pass finishes the tasks that converged with it.  Task state and chunk sums
+        // alternate between two generations (index si / pi: the latest).  They need every
+        // item on a grid facet with its row table in LDS, and one stream.
+        const bool fused = tuning().fuse_step && ng == 1 && tg[0].grid && tg[0].tab_y_lds;
+        if (fused) {
+            TileTask *st[2] = {b.d_tasks, b.d_tasks + (size_t)b.cap * kMaxR};
+            double *pt[2] = {b.d_partials, b.d_partials + (size_t)total * 8};
+            int si = 0, pi = 0, mode = 0, passes = 0;
+            int queued = tuning().tile_passes ? tuning().tile_passes : b.tile_passes;
+            {
+                TileLaunch p0 = tg[0];
+                p0.partials = pt[0];
+                KTimer k(RL2_K_LIKELIHOOD);
+                launch_tile_sums(g_stream, p0, 0);
+            }
+            while (true) {
+                for (int q = 0; q < queued; q++) {
+                    TileLaunch f = tg[0];
+                    f.mode = mode;
+                    f.tasks = st[si];
+                    f.tasks_out = st[si ^ 1];
+                    f.part_in = pt[pi];
+                    f.partials = pt[pi ^ 1];
+                    {
+                        KTimer k(RL2_K_LIKELIHOOD);
+                        launch_tile_fused(g_stream, f);
+                    }
+                    if (mode != 2) si ^= 1;
+                    pi ^= 1;
+                    mode = 1;
+                }
+                passes += queued;
+                {
+                    TileLaunch sk = tg[0];
+                    sk.tasks = st[si];
+                    sk.tasks_out = st[si ^ 1];
+                    sk.partials = pt[pi];
+                    sk.tab = nullptr;  // no tables: the next pass, if any, makes its own
+                    KTimer k(RL2_K_STEP);
+                    launch_tile_step(g_stream, sk, 1);
+                    si ^= 1;
+                }
+                {
+                    KTimer k(RL2_K_RMARG);
+                    LikLaunch ra = a;  // plus one work-group writing the status block
+                    ra.tasks = st[si];
+                    ra.status = b.d_status;
+                    ra.flops = b.d_flops;
+                    launch_rmarg(g_stream, ra);
+                }
+                copy_out();
+                if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+                if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+                if (b.h_status[0] == 0) break;
+                if (passes > 50) {  // the step count is capped at 50 (likelihood.c:284)
+                    set_error("likelihood: Newton passes did not terminate");
+                    return false;
+                }
+                queued = 2;
+                mode = 2;  // the step kernel took the last step: the next pass resumes from it
+            }
+            b.tile_passes = std::min(12, std::max(2, (int)b.h_status[1] + 1));
+            if (g_stats_on) stats_add_work(RL2_K_LIKELIHOOD, *b.h_flops, 0.);
+            memcpy(Le, b.h_Le, n * sizeof(double));
+            memcpy(err, b.h_err, n * sizeof(int));
+            if (Lr) memcpy(Lr, b.h_Lr, (size_t)n * kMaxR * sizeof(double));
+            if (nit) memcpy(nit, b.h_nit, (size_t)n * kMaxR * sizeof(int));
+            return true;
+        }
         // Newton passes are queued ahead of the status check (the previous evaluation's
         // largest step count + 1); a pass leaves finished tasks at once.  The
         // r-marginalisation is queued with them and repeated if tasks were still iterating.
-        int queued = b.tile_passes, passes = 0, first = 0;
+        int queued = tuning().tile_passes ? tuning().tile_passes : b.tile_passes;
+        int passes = 0, first = 0;
         while (true) {
             passes_of(first, first == 0 ? queued + 1 : queued);
             first = 1;
@@ -1726,6 +1799,14 @@ int rl2_tune_set(int knob, int value)
         if (value < 0 || value > 65536) break;
         tuning().item_groups = (unsigned)value;
         return RL2_OK;
+    case RL2_TUNE_FUSE_STEP:
+        if (value < 0 || value > 1) break;
+        tuning().fuse_step = value;
+        return RL2_OK;
+    case RL2_TUNE_TILE_PASSES:
+        if (value < 0 || value > 50) break;
+        tuning().tile_passes = value;
+        return RL2_OK;
     case RL2_TUNE_TILE_STREAMS:
         if (value != 1 && value != 2) break;
         tuning().tile_streams = value;
@@ -1752,6 +1833,8 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_BATCH_GROUPS: *value = (int)tuning().batch_groups; return RL2_OK;
     case RL2_TUNE_BATCH_GRAPH: *value = tuning().batch_graph; return RL2_OK;
     case RL2_TUNE_ITEM_GROUPS: *value = (int)tuning().item_groups; return RL2_OK;
+    case RL2_TUNE_FUSE_STEP: *value = tuning().fuse_step; return RL2_OK;
+    case RL2_TUNE_TILE_PASSES: *value = tuning().tile_passes; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

@@ -142,6 +142,24 @@ def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
     np.testing.assert_array_equal(nit1, nit)
     np.testing.assert_array_equal(Lr1[:, 1:24], Lr[:, 1:24])
     np.testing.assert_array_equal(Le1, Le)
+    # Newton steps in a step kernel after every pass (RL2_TUNE_FUSE_STEP 0) instead of in the
+    # next pass's work-groups, and the fused passes queued one at a time (every evaluation
+    # then resumes from the step kernel's state after each status check): the same sums,
+    # steps and tables, so the same values bit for bit
+    for fuse, passes in ((0, 0), (1, 1), (0, 1)):
+        N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, fuse), "rl2_tune_set")
+        N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_PASSES, passes), "rl2_tune_set")
+        try:
+            Le3, err3, Lr3, nit3 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+            fit_3 = dev.fit(slot=3)
+        finally:
+            N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, 1), "rl2_tune_set")
+            N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_PASSES, 0), "rl2_tune_set")
+        np.testing.assert_array_equal(err3, err)
+        np.testing.assert_array_equal(nit3, nit)
+        np.testing.assert_array_equal(Lr3[:, 1:24], Lr[:, 1:24])
+        np.testing.assert_array_equal(Le3, Le)
+        assert (fit_3.error, fit_3.e1, fit_3.e2) == (fit_t.error, fit_t.e1, fit_t.e2), (fuse, passes)
 
 
 @pytest.mark.parametrize("g", [0, 3, 7])
