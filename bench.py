@@ -550,7 +550,7 @@ def main():
                     help="FP64 Newton phases from per-pass column x row tables with the rows "
                          "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")
     ap.add_argument("--item-groups", type=int, default=0,
-                    help="work-groups a likelihood item's chunking aims at (0: 512)")
+                    help="work-groups a likelihood item's chunking aims at (0: 256)")
     ap.add_argument("--fuse-step", type=int, default=1, choices=[0, 1],
                     help="Newton steps folded into the next pass (1) or a step kernel per "
                          "pass (0)")

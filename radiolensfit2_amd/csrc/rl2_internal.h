@@ -28,6 +28,11 @@ struct FacetDev {
     // (ctr[column], ctr[row]), bit-identical to uu[k], vv[k].
     const uint32_t *cid;
     const double *ctr;
+    // Packed copies for the fused Newton passes (grid facets up to kRecFacet wide, else
+    // unset): the data {D_re, D_im} in one 16-byte load, and each cell's LDS record offsets
+    // (row * kRecBytes) << 16 | column * kRecBytes; both with the zero pad cell.
+    const double2 *dc;
+    const uint32_t *loff;
     unsigned int grid_facet;
     unsigned int pad0;
     unsigned int n;
@@ -144,6 +149,8 @@ struct BatchCompact {
     double *uu, *vv, *dre, *dim, *s2;
     uint32_t *cid;
     double *ctr;
+    double2 *dc;
+    uint32_t *loff;
     float4 *pk;
     float *invs2;
     double *invs2_64;
@@ -227,7 +234,8 @@ size_t compact_scratch_words(unsigned facet);
 // FP32 packing of a compacted facet (count read from the device descriptor).
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
                        const double *vv, const double *dre, const double *dim, const double *s2,
-                       float4 *pk, float *invs2, double *invs2_64);
+                       float4 *pk, float *invs2, double *invs2_64, const uint32_t *cid,
+                       unsigned facet, double2 *dc, uint32_t *loff);
 
 struct TileTask {
     double xo, yo;             // position offset of the current Newton iterate
@@ -278,8 +286,11 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned kTileKnobMinCells = 256;   // smallest chunk RL2_TUNE_TILE_CELLS accepts
-constexpr unsigned long long kTileItemGroups = 512;  // per item: 4 items give 8 per CU
+constexpr unsigned long long kTileItemGroups = 256;  // per item (fastest measured with the fused passes)
 constexpr int kInlItems = 8;
+// LDS record of one grid index in the fused Newton passes: {X (16 bytes), Y (16), ctr (8)}.
+constexpr unsigned kRecBytes = 40;
+constexpr unsigned kRecFacet = 1024;  // widest facet with records (40 KB of LDS)
 constexpr unsigned kTileLdsBytes = 40960;  // column-table staging per work-group (<= 1706 columns)
 
 struct TileLaunch {

@@ -618,6 +618,20 @@ __global__ __launch_bounds__(kCompactBS) void compact_write_kernel(
                            s2o, cid, ctr, sum_w_dense, fx_unscale, fs_unscale, tscan);
 }
 
+// The packed data and record offsets of the fused Newton passes (cell i and the zero pad
+// cell n); offsets only for grid facets with records (facet <= kRecFacet).
+__device__ inline void pack_cell_records(unsigned i, unsigned n, const double *dre,
+                                         const double *dim, const uint32_t *cid, unsigned facet,
+                                         double2 *dc, uint32_t *loff)
+{
+    const bool in = i < n;
+    dc[i] = in ? make_double2(dre[i], dim[i]) : make_double2(0., 0.);
+    if (cid && facet <= kRecFacet) {
+        const uint32_t c = in ? cid[i] : 0u;
+        loff[i] = ((c >> 16) * kRecBytes) << 16 | (c & 0xffffu) * kRecBytes;
+    }
+}
+
 // ---- batched compaction: grid.y = target of a BatchDesc; targets beyond desc->ntarget and
 // blocks beyond a target's facet exit at once, so one launch shape serves every batch.
 __global__ __launch_bounds__(kCompactBS) void compact_count_batch_kernel(const BatchDesc *desc)
@@ -706,8 +720,12 @@ __global__ void pack_facet_batch_kernel(const BatchDesc *desc)
         const double w = 1.0 / C.s2[i];
         C.invs2[i] = (float)w;
         C.invs2_64[i] = w;
+        pack_cell_records(i, n, C.dre, C.dim, C.cid, C.facet, C.dc, C.loff);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) C.invs2_64[n] = 0.;  // the zero pad cell
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the zero pad cell
+        C.invs2_64[n] = 0.;
+        pack_cell_records(n, n, C.dre, C.dim, C.cid, C.facet, C.dc, C.loff);
+    }
 }
 
 void launch_compact_batch(hipStream_t s, const BatchDesc *desc, unsigned ncap, unsigned fmax,
@@ -1369,7 +1387,9 @@ void launch_likelihood(hipStream_t s, const LikLaunch &a)
 
 __global__ void pack_facet_kernel(const FacetDev *dfacet, const double *uu, const double *vv,
                                   const double *dre, const double *dim, const double *s2,
-                                  float4 *pk, float *invs2, double *invs2_64)
+                                  float4 *pk, float *invs2, double *invs2_64,
+                                  const uint32_t *cid, unsigned facet, double2 *dc,
+                                  uint32_t *loff)
 {
     const unsigned n = dfacet->n;
     for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1377,16 +1397,22 @@ __global__ void pack_facet_kernel(const FacetDev *dfacet, const double *uu, cons
         const double w = 1.0 / s2[i];
         invs2[i] = (float)w;
         invs2_64[i] = w;
+        pack_cell_records(i, n, dre, dim, cid, facet, dc, loff);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) invs2_64[n] = 0.;  // the zero pad cell
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // the zero pad cell
+        invs2_64[n] = 0.;
+        pack_cell_records(n, n, dre, dim, cid, facet, dc, loff);
+    }
 }
 
 void launch_pack_facet(hipStream_t s, const FacetDev *dfacet, unsigned cap, const double *uu,
                        const double *vv, const double *dre, const double *dim, const double *s2,
-                       float4 *pk, float *invs2, double *invs2_64)
+                       float4 *pk, float *invs2, double *invs2_64, const uint32_t *cid,
+                       unsigned facet, double2 *dc, uint32_t *loff)
 {
     unsigned blocks = std::min(2048u, (cap + 255) / 256 + 1);
-    pack_facet_kernel<<<blocks, 256, 0, s>>>(dfacet, uu, vv, dre, dim, s2, pk, invs2, invs2_64);
+    pack_facet_kernel<<<blocks, 256, 0, s>>>(dfacet, uu, vv, dre, dim, s2, pk, invs2, invs2_64,
+                                             cid, facet, dc, loff);
 }
 
 // ------------------------------------------------------------------ r-marginalisation (K4)
@@ -2250,6 +2276,103 @@ __device__ inline bool tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
     return true;
 }
 
+// The Newton-pass sums of tile_sums64_tab from LDS records: grid index k's record at byte
+// k * kRecBytes holds {X[k], Y[k], ctr[k]} as five doubles (8-byte aligned: read in pairs),
+// and a cell's packed loff gives both its records' offsets, so per cell the vector memory
+// carries loff and the packed {D_re, D_im}, and the LDS addresses need no arithmetic
+// beyond unpacking.  The arithmetic and its order are tile_sums64_tab's.
+struct RecEntry {
+    double xr, xi, yr, yi, c;
+};
+static_assert(sizeof(RecEntry) == kRecBytes, "record layout");
+struct RecCells {
+    uint32_t o[kTabU];
+    double dr[kTabU], di[kTabU];
+};
+template <class Stage>
+__device__ inline bool tile_sums64_rec(const FacetDev &F, const LikItem &it, double ro,
+                                       const Stage &stage, const char *rec, unsigned beg,
+                                       unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    const uint32_t *loff = F.loff;
+    const double2 *dc = F.dc;
+    const unsigned pad = F.n;
+    auto fetch = [&](RecCells &b, unsigned base) {
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const unsigned i = base + q * kTileBS;
+            const unsigned k = i < end ? i : pad;
+            b.o[q] = gload(loff, k);
+            const double2 d = gload(dc, k);
+            b.dr[q] = d.x;
+            b.di[q] = d.y;
+        }
+    };
+    auto eval = [&](const RecCells &b) {
+        double uu[kTabU], vv[kTabU];
+        double2 x[kTabU], y[kTabU];
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const RecEntry &col = *reinterpret_cast<const RecEntry *>(rec + (b.o[q] & 0xffffu));
+            const RecEntry &row = *reinterpret_cast<const RecEntry *>(rec + (b.o[q] >> 16));
+            uu[q] = col.c;
+            vv[q] = row.c;
+            x[q] = make_double2(col.xr, col.xi);
+            y[q] = make_double2(row.yr, row.yi);
+        }
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const double k1 = fma(e2, vv[q], a1 * uu[q]);
+            const double k2 = fma(a2, vv[q], e2 * uu[q]);
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = b.dr[q] * M;
+            const double hi = -b.di[q] * M;
+            const double cs = fma(x[q].x, y[q].x, -(x[q].y * y[q].y));
+            const double sn = fma(x[q].y, y[q].x, x[q].x * y[q].y);
+            const double re = fma(hr, cs, -hi * sn);
+            const double im = fma(hr, sn, hi * cs);
+            s[0] += re;
+            s[1] = fma(-im, uu[q], s[1]);
+            s[2] = fma(-im, vv[q], s[2]);
+            const double rU = re * uu[q];
+            s[3] = fma(-rU, uu[q], s[3]);
+            s[4] = fma(-re * vv[q], vv[q], s[4]);
+            s[5] = fma(-rU, vv[q], s[5]);
+        }
+    };
+    constexpr unsigned S = kTabU * kTileBS;
+    RecCells A, B;
+    fetch(A, beg + threadIdx.x);
+    if (!stage()) return false;
+#pragma unroll
+    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.o[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
+    for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
+        fetch(B, base + S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(A);
+        if (base + S >= end) break;
+        fetch(A, base + 2 * S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(B);
+    }
+    s[1] *= kTwoPi;
+    s[2] *= kTwoPi;
+    constexpr double kTwoPi2 = kTwoPi * kTwoPi;
+    s[3] *= kTwoPi2;
+    s[4] *= kTwoPi2;
+    s[5] *= kTwoPi2;
+    return true;
+}
+
 // The same sums from the FP32 facet copy: model and terms in FP32, the per-thread FP32
 // partials added into FP64 every kFlush cells; the Newton phase formed and reduced in
 // FP64 revolutions.
@@ -2729,10 +2852,8 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     const unsigned beg = chunk * it.chunk_cells;
     const unsigned end = min(F.n, beg + it.chunk_cells);
     const unsigned gf = F.grid_facet;  // <= 1024: the host checks the LDS tables fit
-    extern __shared__ double2 tab_lds[];  // t.tab_lds_facet * 40 bytes: X, ctr, Y
-    double2 *sX = tab_lds;
-    double *sC = reinterpret_cast<double *>(tab_lds + t.tab_lds_facet);
-    double2 *sY = reinterpret_cast<double2 *>(sC + t.tab_lds_facet);
+    extern __shared__ double2 tab_lds[];  // t.tab_lds_facet records of kRecBytes
+    RecEntry *rec = reinterpret_cast<RecEntry *>(tab_lds);
     const uint32_t *cid = F.cid;
     const double *ctr = F.ctr;
     // Runs after the first cells' loads are issued (inside tile_sums64_tab): the cell
@@ -2797,28 +2918,32 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         for (int j = 0; j < NC; j++) {
             const unsigned k = threadIdx.x + j * kTileBS;
             if (k < gf) {
-                sC[k] = c[j];
-                sX[k] = tab_x(R, k);
+                const double2 x = tab_x(R, k);
+                rec[k].xr = x.x;
+                rec[k].xi = x.y;
+                rec[k].c = c[j];
             }
         }
         const unsigned span = row_hi - row_lo + 1;
         for (unsigned q = threadIdx.x; q < span + (row_lo > 0 ? 1u : 0u); q += kTileBS) {
             const unsigned k = q < span ? row_lo + q : 0u;
-            sY[k] = tab_y(R, k);
+            const double2 y = tab_y(R, k);
+            rec[k].yr = y.x;
+            rec[k].yi = y.y;
         }
         __syncthreads();
         return true;
     };
     double s[7];
-    if (tile_sums64_tab<true>(F, it, a.rg.ro[r], make, (const double2 *)nullptr, sX, sY, sC,
-                              beg, end, s))
+    if (tile_sums64_rec(F, it, a.rg.ro[r], make, reinterpret_cast<const char *>(rec), beg, end,
+                        s))
         tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
 }
 
 void launch_tile_fused(hipStream_t s, const TileLaunch &t)
 {
     const unsigned g = tile_grid(t.nwg, 0);
-    const unsigned lds = t.tab_lds_facet * 40;
+    const unsigned lds = t.tab_lds_facet * kRecBytes;
     if (t.grid == 2)
         lik_fused_kernel<2><<<g, kTileBS, lds, s>>>(t);
     else

@@ -23,6 +23,8 @@ struct FacetSlot {
     double *invs2_64 = nullptr;
     unsigned *scan = nullptr;
     uint32_t *cid = nullptr; // packed (row << 16 | column) per compacted cell
+    double2 *dc = nullptr;   // packed {D_re, D_im} (fused Newton passes)
+    uint32_t *loff = nullptr;// LDS record offsets per cell (fused Newton passes)
     double *ctr = nullptr;   // cell-centre coordinate per grid index (fmax)
     unsigned facet = 0;
     unsigned n = 0;          // host copy of the compacted size (valid after a sync)

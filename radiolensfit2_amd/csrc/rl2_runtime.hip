@@ -264,6 +264,8 @@ Obs::~Obs()
         dfree(s.scan);
         dfree(s.cid);
         dfree(s.ctr);
+        dfree(s.dc);
+        dfree(s.loff);
     }
     dfree(d_facets);
     dfree(d_feat);
@@ -294,6 +296,7 @@ bool Obs::ensure_slots(int n)
               dmalloc(&s.dre, ncell + 1) && dmalloc(&s.dim, ncell + 1) &&
               dmalloc(&s.s2, ncell) && dmalloc(&s.pk, ncell) && dmalloc(&s.invs2, ncell) &&
               dmalloc(&s.invs2_64, ncell + 1) && dmalloc(&s.cid, ncell + 1) &&
+              dmalloc(&s.dc, ncell + 1) && dmalloc(&s.loff, ncell + 1) &&
               dmalloc(&s.ctr, (size_t)fmax) &&
               dmalloc(&s.scan, compact_scratch_words(fmax) + 1)))
             return false;
@@ -322,6 +325,8 @@ bool Obs::ensure_slots(int n)
         f.pk = slots[i].pk;
         f.invs2 = slots[i].invs2;
         f.invs2_64 = slots[i].invs2_64;
+        f.dc = slots[i].dc;
+        f.loff = slots[i].loff;
         f.n = 0;
         // full descriptor once (n = 0 until a compaction or load writes it)
         if (!hip_ok(hipMemcpyAsync(d_facets + i, &f, sizeof(FacetDev), hipMemcpyHostToDevice,
@@ -406,7 +411,7 @@ static bool finish_extract(Obs &o, int slot, unsigned facet, bool want_sumw, con
         launch_compact(g_stream, s.acc, facet, o.len, s.uu, s.vv, s.dre, s.dim, s.s2, s.cid,
                        s.ctr, &o.d_facets[slot].n, want_sumw ? s.sumw : nullptr, s.scan, &sc);
         launch_pack_facet(g_stream, &o.d_facets[slot], facet * facet, s.uu, s.vv, s.dre, s.dim,
-                          s.s2, s.pk, s.invs2, s.invs2_64);
+                          s.s2, s.pk, s.invs2, s.invs2_64, s.cid, facet, s.dc, s.loff);
     }
     // the facet now has grid structure: cell ids and centres for the phase tables (packed
     // in 16-bit row / column fields)
@@ -568,6 +573,8 @@ bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
         Cp.s2 = sl.s2;
         Cp.cid = sl.cid;
         Cp.ctr = sl.ctr;
+        Cp.dc = sl.dc;
+        Cp.loff = sl.loff;
         Cp.pk = sl.pk;
         Cp.invs2 = sl.invs2;
         Cp.invs2_64 = sl.invs2_64;
@@ -1532,7 +1539,7 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
     hipMemcpyAsync(s.vv, vv, n * 8, hipMemcpyHostToDevice, g_stream);
     hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
     launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
-                      s.invs2, s.invs2_64);
+                      s.invs2, s.invs2_64, nullptr, 0, s.dc, s.loff);
     FacetDev &f = o.h_facets[slot];  // caller-given cells: no grid structure
     if (f.cid || f.grid_facet) {
         f.cid = nullptr;
