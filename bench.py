@@ -551,9 +551,9 @@ def main():
                          "in LDS (2) or in global memory (1), or a sin/cos per cell (0)")
     ap.add_argument("--item-groups", type=int, default=0,
                     help="work-groups a likelihood item's chunking aims at (0: 256)")
-    ap.add_argument("--fuse-step", type=int, default=1, choices=[0, 1],
-                    help="Newton steps folded into the next pass (1) or a step kernel per "
-                         "pass (0)")
+    ap.add_argument("--fuse-step", type=int, default=1, choices=[0, 1, 2],
+                    help="Newton steps folded into the next pass (1), every Newton pass of an "
+                         "evaluation in one launch (2), or a step kernel per pass (0)")
     ap.add_argument("--tile-passes", type=int, default=0,
                     help="Newton passes queued before an evaluation's status check "
                          "(0: the previous evaluation's largest step count + 1)")

@@ -225,7 +225,9 @@ int rl2_synchronize(void);
  *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 256.
  * RL2_TUNE_FUSE_STEP: 1 (default) folds each Newton step of the likelihood into the next
  *   pass's launch (its work-groups take the step and make their own phase tables), so a pass
- *   is one launch; 0 runs a step kernel after every pass.  Both give identical results.
+ *   is one launch; 2 runs every Newton pass of an evaluation in one cooperative launch (the
+ *   blocks wait on per-task counters between passes); 0 runs a step kernel after every pass.
+ *   All three give identical results.
  * RL2_TUNE_TILE_PASSES: Newton passes queued before the first status check of an evaluation
  *   (1..50); 0 (default): the previous evaluation's largest step count + 1.  Passes beyond a
  *   task's last step leave it at once, and missing ones are queued after the check, so the

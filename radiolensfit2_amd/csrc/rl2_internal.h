@@ -330,6 +330,21 @@ struct TileLaunch {
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
 // One Newton pass with the previous pass's step folded in (grid items, row tables in LDS).
 void launch_tile_fused(hipStream_t s, const TileLaunch &t);
+// Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8), and each XCD
+// has its own L2.
+constexpr unsigned kXcds = 8;
+// All passes of one evaluation in one cooperative launch (lik_persist_kernel).
+constexpr int kPersistTasks = 8;    // tasks one block holds
+constexpr int kPersistPasses = 51;  // pass 0 and at most 50 Newton steps (likelihood.c:284)
+struct PersistLaunch {
+    TileLaunch t;              // items, facets, outputs; r_lo = 1, nr_g = all scalelengths
+    double *part;              // [kPersistPasses][t.nwg][8] chunk sums
+    unsigned *cnt;             // per task (item * kMaxR + r): chunks done, zero at launch
+    unsigned nbx;              // blocks per XCD
+    unsigned spin_limit;       // polls of one wait before the block gives up
+};
+unsigned persist_capacity(unsigned lds, int kind);
+bool launch_persist(hipStream_t s, const PersistLaunch &p, unsigned blocks);
 // The two launches of a pass separately: the chunk sums, then the per-task step.
 void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass);
 void launch_tile_step(hipStream_t s, const TileLaunch &t, int pass);

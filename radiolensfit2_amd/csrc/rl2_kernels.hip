@@ -5,6 +5,7 @@
 // complex64 (RL2_PREC_MIXED).  sigma^2 is one float per visibility with the sign bit
 // marking a flagged visibility (sigma^2 > 0 after defaulting, RadioLensfit2.cpp:266-269).
 #include <hip/hip_runtime.h>
+#include <map>
 
 #include <algorithm>
 #include <cmath>
@@ -2000,7 +2001,6 @@ __device__ inline void tile_range(const TileLaunch &t, const LikItem *items, int
 // Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD
 // has its own L2.  The blocks of XCD x walk the logical work-groups x * per ... (x + 1) *
 // per - 1 (per = ceil(nwg / 8)), so the tasks of a chunk run on one XCD and share its L2.
-constexpr unsigned kXcds = 8;
 // Blocks of a pass: one per logical work-group, or at most `cap` (a multiple of 8).
 inline unsigned tile_grid(unsigned nwg, unsigned cap)
 {
@@ -2938,6 +2938,263 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     if (tile_sums64_rec(F, it, a.rg.ro[r], make, reinterpret_cast<const char *>(rec), beg, end,
                         s))
         tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
+}
+
+// ------------------------------------------------------------------ one launch per evaluation
+// tile_reduce, with the chunk's sums published to the other work-groups of the launch: the
+// stores are made at agent scope, wave 0 (which issued them) waits until they have
+// completed, and only then counts the chunk as done on the task's counter.  A reader that
+// sees the count loads the sums at agent scope too.  Every sum lives at its own address for
+// the whole launch (one slice per pass), so no reader can hold an older copy of it.
+template <int N>
+__device__ inline void tile_reduce_publish(double (&v)[N], double *lds, double *P,
+                                           unsigned *cnt)
+{
+    constexpr int NW = kTileBS / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int j = 0; j < N; j++) v[j] = wave_sum_last(v[j]);
+    if (lane == 63)
+#pragma unroll
+        for (int j = 0; j < N; j++) lds[wave * N + j] = v[j];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        if (threadIdx.x < N) {
+            double t = 0.;
+#pragma unroll
+            for (int q = 0; q < NW; q++) t += lds[q * N + threadIdx.x];
+            __hip_atomic_store(P + threadIdx.x, t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (threadIdx.x == 0)
+            __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();  // lds is reused by the next chunk
+}
+
+// A task's chunk sums of one pass as task_sums, loaded at agent scope (published by
+// tile_reduce_publish in other work-groups of the launch).
+__device__ inline void task_sums_published(const double *P0, unsigned nch, int nr, int lane,
+                                           double (&v)[7])
+{
+#pragma unroll
+    for (int j = 0; j < 7; j++) v[j] = 0.;
+    for (unsigned c = lane; c < nch; c += 64)
+#pragma unroll
+        for (int j = 0; j < 7; j++)
+            v[j] += __hip_atomic_load(P0 + (size_t)c * nr * 8 + j, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int j = 0; j < 7; j++) v[j] = wave_sum_last(v[j]);
+}
+
+// One block's share of an evaluation: a task, and the run of its chunks the block sums.
+struct PersistTask {
+    int item, r;
+    unsigned c0, c1;       // chunks [c0, c1) of the task
+    unsigned nch;          // the task's chunk count
+    unsigned first;        // its item's first work-group index (chunk-major layout)
+    int active;
+    int pad;
+    TileTask S;            // the block's copy of the task's Newton state
+};
+
+// Every Newton pass of one likelihood evaluation, to convergence, in one cooperative launch
+// after the pass-0 launch, so that no Newton pass pays a launch, a ramp and a tail.  Work layout:
+// XCD x (blocks b with b % 8 == x) owns chunks [x nc / 8, (x + 1) nc / 8) of every task
+// (nc: the task's chunk count), so the tasks that read the same cells share that XCD's L2;
+// within the XCD the (task, chunk) pairs are listed task by task and each block takes an
+// equal contiguous run of the list, so a block holds one or a few tasks and takes their
+// Newton steps and builds their phase tables once per pass.  The chunks, their sums and
+// the steps are those of the fused passes, bit for bit.
+// Between passes a task's chunk sums of pass q >= 1 are complete when its counter reaches
+// q nc (pass 0's are complete at launch).  A block waits on that count only for its own task's previous pass, and it
+// has finished all its work of every earlier pass, so a waiting block always depends on
+// blocks that are running or done: with every block resident (cooperative launch) the
+// waits end.  Each wait is bounded all the same (p.spin_limit polls): a block that gives
+// up leaves its tasks marked active, which the host reports as an error.
+
+template <int KIND>
+__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_persist_kernel(PersistLaunch p)
+{
+    __shared__ double red[(kTileBS / 64) * 8];
+    __shared__ PersistTask s_task[kPersistTasks];
+    __shared__ int s_ntask, s_go, s_fail;
+    __shared__ double s_off[2];
+    __shared__ TabRot<1024 / kTabRun> R;
+    extern __shared__ double2 tab_lds[];  // t.tab_lds_facet records
+    RecEntry *rec = reinterpret_cast<RecEntry *>(tab_lds);
+    const TileLaunch &t = p.t;
+    const LikLaunch &a = t.lk;
+    const LikItem *items = KIND == 2 ? t.inl : a.items;
+    const int nr = t.nr_g;
+    const unsigned x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    // items that travel in the arguments are stored for the r-marginalisation
+    if (KIND == 2 && blockIdx.x == 0 && threadIdx.x < t.n_inl)
+        t.items_w[threadIdx.x] = t.inl[threadIdx.x];
+    if (threadIdx.x == 0) {
+        // the XCD's list: item by item, task by task, the task's chunks in the XCD's eighth
+        auto chunks = [&](int i) {
+            const unsigned c1 = i + 1 < a.nitems ? items[i + 1].chunk_first : t.nwg / nr;
+            return c1 - items[i].chunk_first;
+        };
+        unsigned long long W = 0;
+        for (int i = 0; i < a.nitems; i++) {
+            const unsigned nc = chunks(i);
+            W += (unsigned long long)nr * ((x + 1) * nc / kXcds - x * nc / kXcds);
+        }
+        const unsigned long long q0 = j * W / p.nbx, q1 = (j + 1) * W / p.nbx;
+        int nt = 0;
+        unsigned long long P = 0;
+        for (int i = 0; i < a.nitems && P < q1 && nt >= 0; i++) {
+            const unsigned nc = chunks(i);
+            const unsigned lo = x * nc / kXcds, m = (x + 1) * nc / kXcds - lo;
+            for (int r = 1; r <= nr && m; r++, P += m) {
+                if (P + m <= q0 || P >= q1) continue;
+                if (nt == kPersistTasks) {
+                    nt = -1;
+                    break;
+                }
+                PersistTask &k = s_task[nt++];
+                k.item = i;
+                k.r = r;
+                k.c0 = lo + (unsigned)(q0 > P ? q0 - P : 0);
+                k.c1 = lo + (unsigned)(q1 < P + m ? q1 - P : m);
+                k.nch = nc;
+                k.first = items[i].chunk_first * (unsigned)nr;
+                k.active = 1;
+            }
+        }
+        s_ntask = nt;
+        s_fail = 0;
+    }
+    __syncthreads();
+    const int ntask = s_ntask;  // < 0: more tasks than a block holds (the host prevents it)
+    // the owner of a task's chunk 0 publishes its state: active until it finishes
+    for (int k = 0; k < ntask; k++)
+        if (threadIdx.x == 0 && s_task[k].c0 == 0)
+            p.t.tasks[(size_t)s_task[k].item * kMaxR + s_task[k].r].active = 1;
+    // pass 0's sums are in slice 0 already (the pass-0 launch before this one, stream-ordered);
+    // passes 1 .. kPersistPasses - 1 sum; the last iteration only takes the final step
+    for (int pass = 1; pass <= kPersistPasses && ntask > 0; pass++) {
+        bool any = false;
+        for (int k = 0; k < ntask; k++) {
+            PersistTask &K = s_task[k];
+            if (!K.active) continue;  // uniform: read after a barrier
+            const LikItem it = items[K.item];
+            const FacetDev &F = a.facets[it.slot];
+            const unsigned gf = F.grid_facet;
+            const size_t ti = (size_t)K.item * kMaxR + K.r;
+            const double ro = a.rg.ro[K.r];
+            {
+                // the previous pass's sums of this task, then its Newton step
+                if (threadIdx.x == 0 && pass > 1) {
+                    const unsigned want = (unsigned)(pass - 1) * K.nch;
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(p.cnt + ti, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT) < want) {
+                        if (++spins > p.spin_limit) {
+                            s_fail = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                __syncthreads();
+                if (s_fail) return;  // the task stays marked active: the host reports it
+                if (threadIdx.x < 64) {
+                    const int lane = threadIdx.x;
+                    double v[7];
+                    task_sums_published(p.part + ((size_t)(pass - 1) * t.nwg + K.first +
+                                                  (unsigned)(K.r - 1)) * 8,
+                                        K.nch, nr, lane, v);
+                    if (lane == 63) {
+                        TileTask S = pass == 1 ? TileTask{} : K.S;
+                        double ho, det;
+                        const int done = pass == 1 ? newton_update<0>(S, v, &ho, &det)
+                                                   : newton_update<1>(S, v, &ho, &det);
+                        K.S = S;
+                        K.active = !done;
+                        if (done && K.c0 == 0) {
+                            p.t.tasks[ti] = S;
+                            finish_lr(a, K.item, K.r, ho, det, S.iter);
+                        }
+                        s_off[0] = S.xo;
+                        s_off[1] = S.yo;
+                    }
+                }
+                __syncthreads();
+                if (!K.active) continue;
+                // phase tables: every column, the rows this block's cells span, row 0
+                const unsigned beg = K.c0 * it.chunk_cells;
+                const unsigned end = min(F.n, K.c1 * it.chunk_cells);
+                const uint32_t *cid = F.cid;
+                const unsigned row_lo = end > beg ? gload(cid, beg) >> 16 : 0u;
+                const unsigned row_hi = end > beg ? gload(cid, end - 1) >> 16 : 0u;
+                const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
+                tab_rot_stage(R, F.ctr, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
+                __syncthreads();
+                for (unsigned q = threadIdx.x; q < gf; q += kTileBS) {
+                    const double2 xx = tab_x(R, q);
+                    rec[q].xr = xx.x;
+                    rec[q].xi = xx.y;
+                    rec[q].c = gload(F.ctr, q);
+                }
+                const unsigned span = row_hi - row_lo + 1;
+                for (unsigned q = threadIdx.x; q < span + (row_lo > 0 ? 1u : 0u); q += kTileBS) {
+                    const unsigned kk = q < span ? row_lo + q : 0u;
+                    const double2 y = tab_y(R, kk);
+                    rec[kk].yr = y.x;
+                    rec[kk].yi = y.y;
+                }
+                __syncthreads();
+            }
+            any = true;
+            for (unsigned c = K.c0; c < K.c1; c++) {
+                const unsigned beg = c * it.chunk_cells;
+                const unsigned end = min(F.n, beg + it.chunk_cells);
+                double s[7];
+                auto none = []() { return true; };  // the records are already staged
+                tile_sums64_rec(F, it, ro, none, reinterpret_cast<const char *>(rec), beg, end,
+                                s);
+                tile_reduce_publish<7>(
+                    s, red, p.part + ((size_t)pass * t.nwg + K.first + c * nr + (K.r - 1)) * 8,
+                    p.cnt + ti);
+            }
+            __syncthreads();  // the records are restaged for the next task
+        }
+        if (!any) break;
+    }
+}
+
+// Largest block count of a cooperative persistent launch with `lds` bytes of dynamic LDS
+// (0: the kernel cannot run).
+unsigned persist_capacity(unsigned lds, int kind)
+{
+    static std::map<std::pair<unsigned, int>, unsigned> cache;
+    const auto key = std::make_pair(lds, kind);
+    auto f = cache.find(key);
+    if (f != cache.end()) return f->second;
+    int per = 0, dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (kind == 2)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<2>, kTileBS, lds);
+    else
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<1>, kTileBS, lds);
+    const unsigned cap = (unsigned)std::max(0, per) * (unsigned)std::max(0, cus);
+    cache[key] = cap;
+    return cap;
+}
+
+bool launch_persist(hipStream_t s, const PersistLaunch &p, unsigned blocks)
+{
+    const unsigned lds = p.t.tab_lds_facet * kRecBytes;
+    void *args[] = {const_cast<PersistLaunch *>(&p)};
+    const void *fn = p.t.grid == 2 ? (const void *)lik_persist_kernel<2>
+                                   : (const void *)lik_persist_kernel<1>;
+    return hipLaunchCooperativeKernel(fn, dim3(blocks), dim3(kTileBS), args, lds, s) ==
+           hipSuccess;
 }
 
 void launch_tile_fused(hipStream_t s, const TileLaunch &t)

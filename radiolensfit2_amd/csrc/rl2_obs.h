@@ -50,11 +50,15 @@ struct LikBuffers {
     TileTask *d_tasks = nullptr;                         // cap * kMaxR
     double *d_partials = nullptr;
     size_t partials_cap = 0;                             // work-groups
+    double *d_ppart = nullptr;                           // one-launch evaluations: chunk
+    size_t ppart_cap = 0;                                // sums per pass (work-groups)
+    unsigned *d_cnt = nullptr;                           // cap * kMaxR chunk counters
     double2 *d_tab = nullptr;                            // Newton phase tables
     size_t tab_cap = 0;                                  // entries
     int tile_passes = 6;     // Newton passes queued before the first status check
     bool ensure(int n);
     bool ensure_partials(size_t nwg);
+    bool ensure_ppart(size_t nwg);
     bool ensure_tab(size_t entries);
     void release();
 };

@@ -162,7 +162,8 @@ bool LikBuffers::ensure(int n)
     bool ok = dmalloc(&d_items, c) && dmalloc(&d_Lr, (size_t)c * kMaxR) &&
               dmalloc(&d_nit, (size_t)c * kMaxR) &&
               dmalloc(&d_ws, (size_t)c * kQagDoubles) && dmalloc(&d_iws, (size_t)c * kQagInts) &&
-              dmalloc(&d_tasks, (size_t)c * kMaxR * 2);  // two generations of task state
+              dmalloc(&d_tasks, (size_t)c * kMaxR * 2) &&  // two generations of task state
+              dmalloc(&d_cnt, (size_t)c * kMaxR);
     ok = ok && hip_ok(hipMemsetAsync(d_tasks, 0, (size_t)c * kMaxR * 2 * sizeof(TileTask),
                                      g_stream),
                       "memset");
@@ -194,6 +195,7 @@ void LikBuffers::release()
     dfree(d_ws);
     dfree(d_iws);
     dfree(d_tasks);
+    dfree(d_cnt);
     if (h_out) hipHostFree(h_out);
     if (h_items) hipHostFree(h_items);
     if (h_Lr) hipHostFree(h_Lr);
@@ -219,6 +221,20 @@ bool LikBuffers::ensure_partials(size_t nwg)
         return false;
     }
     partials_cap = c;
+    return true;
+}
+
+bool LikBuffers::ensure_ppart(size_t nwg)
+{
+    if (nwg <= ppart_cap) return true;
+    dfree(d_ppart);
+    size_t c = 4096;
+    while (c < nwg) c *= 2;
+    if (!dmalloc(&d_ppart, c * 8 * kPersistPasses)) {
+        ppart_cap = 0;
+        return false;
+    }
+    ppart_cap = c;
     return true;
 }
 
@@ -278,6 +294,8 @@ Obs::~Obs()
     lik.release();
     dfree(lik.d_partials);
     lik.partials_cap = 0;
+    dfree(lik.d_ppart);
+    lik.ppart_cap = 0;
     dfree(lik.d_tab);
     lik.tab_cap = 0;
 }
@@ -828,6 +846,79 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // alternate between two generations (index si / pi: the latest).  They need every
         // item on a grid facet with its row table in LDS, and one stream.
         const bool fused = tuning().fuse_step && ng == 1 && tg[0].grid && tg[0].tab_y_lds;
+        // One launch per evaluation (RL2_TUNE_FUSE_STEP 2): every block resident, XCD x
+        // owning an eighth of each task's chunks, each block an equal run of its XCD's
+        // (task, chunk) list; runs that would span more tasks than a block holds fall
+        // back to the fused passes.
+        if (fused && tuning().fuse_step == 2) {
+            TileLaunch &t0 = tg[0];
+            const unsigned cap = persist_capacity(t0.tab_lds_facet * kRecBytes, (int)t0.grid);
+            unsigned long long W[kXcds] = {}, minm[kXcds];
+            unsigned long long wmax = 0;
+            for (unsigned x = 0; x < kXcds; x++) {
+                minm[x] = ~0ull;
+                for (int i = 0; i < n; i++) {
+                    const unsigned c1 = i + 1 < n ? b.h_items[i + 1].chunk_first : chunks;
+                    const unsigned nc = c1 - b.h_items[i].chunk_first;
+                    const unsigned m = (x + 1) * nc / kXcds - x * nc / kXcds;
+                    W[x] += (unsigned long long)nr * m;
+                    if (m) minm[x] = std::min<unsigned long long>(minm[x], m);
+                }
+                wmax = std::max(wmax, W[x]);
+            }
+            unsigned long long nbx = std::min<unsigned long long>(cap / kXcds, wmax);
+            bool ok = nbx > 0;
+            for (unsigned x = 0; x < kXcds && ok; x++) {
+                if (!W[x]) continue;
+                const unsigned long long L = (W[x] + nbx - 1) / nbx;
+                ok = (L + minm[x] - 1) / minm[x] + 1 <= (unsigned long long)kPersistTasks;
+            }
+            if (ok) {
+                if (!b.ensure_ppart(total)) return false;
+                hipMemsetAsync(b.d_cnt, 0, (size_t)n * kMaxR * sizeof(unsigned), g_stream);
+                PersistLaunch P;
+                P.t = t0;
+                P.t.tasks = b.d_tasks;
+                P.part = b.d_ppart;
+                P.cnt = b.d_cnt;
+                P.nbx = (unsigned)nbx;
+                P.spin_limit = 1u << 15;
+                {
+                    KTimer k(RL2_K_LIKELIHOOD);  // pass 0 into the sums' first slice
+                    TileLaunch p0 = t0;
+                    p0.partials = b.d_ppart;
+                    launch_tile_sums(g_stream, p0, 0);
+                }
+                {
+                    KTimer k(RL2_K_LIKELIHOOD);
+                    if (!launch_persist(g_stream, P, (unsigned)nbx * kXcds)) {
+                        set_error("likelihood: cooperative launch failed");
+                        return false;
+                    }
+                }
+                {
+                    KTimer k(RL2_K_RMARG);
+                    LikLaunch ra = a;
+                    ra.tasks = b.d_tasks;
+                    ra.status = b.d_status;
+                    ra.flops = b.d_flops;
+                    launch_rmarg(g_stream, ra);
+                }
+                copy_out();
+                if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+                if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+                if (b.h_status[0] != 0) {  // a task left unfinished: a wait gave up
+                    set_error("likelihood: a device wait exceeded its bound");
+                    return false;
+                }
+                if (g_stats_on) stats_add_work(RL2_K_LIKELIHOOD, *b.h_flops, 0.);
+                memcpy(Le, b.h_Le, n * sizeof(double));
+                memcpy(err, b.h_err, n * sizeof(int));
+                if (Lr) memcpy(Lr, b.h_Lr, (size_t)n * kMaxR * sizeof(double));
+                if (nit) memcpy(nit, b.h_nit, (size_t)n * kMaxR * sizeof(int));
+                return true;
+            }
+        }
         if (fused) {
             TileTask *st[2] = {b.d_tasks, b.d_tasks + (size_t)b.cap * kMaxR};
             double *pt[2] = {b.d_partials, b.d_partials + (size_t)total * 8};
@@ -1807,7 +1898,7 @@ int rl2_tune_set(int knob, int value)
         tuning().item_groups = (unsigned)value;
         return RL2_OK;
     case RL2_TUNE_FUSE_STEP:
-        if (value < 0 || value > 1) break;
+        if (value < 0 || value > 2) break;
         tuning().fuse_step = value;
         return RL2_OK;
     case RL2_TUNE_TILE_PASSES:

@@ -142,11 +142,12 @@ def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
     np.testing.assert_array_equal(nit1, nit)
     np.testing.assert_array_equal(Lr1[:, 1:24], Lr[:, 1:24])
     np.testing.assert_array_equal(Le1, Le)
-    # Newton steps in a step kernel after every pass (RL2_TUNE_FUSE_STEP 0) instead of in the
-    # next pass's work-groups, and the fused passes queued one at a time (every evaluation
-    # then resumes from the step kernel's state after each status check): the same sums,
-    # steps and tables, so the same values bit for bit
-    for fuse, passes in ((0, 0), (1, 1), (0, 1)):
+    # Newton steps in a step kernel after every pass (RL2_TUNE_FUSE_STEP 0), in the next
+    # pass's work-groups (1), or every Newton pass of an evaluation in one launch (2); the
+    # fused passes also queued one at a time (every evaluation then resumes from the step
+    # kernel's state after each status check): the same sums, steps and tables, so the same
+    # values bit for bit
+    for fuse, passes in ((0, 0), (1, 1), (0, 1), (2, 0), (1, 0)):
         N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, fuse), "rl2_tune_set")
         N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_PASSES, passes), "rl2_tune_set")
         try:

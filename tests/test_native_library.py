@@ -227,7 +227,7 @@ def test_tune_knobs_validate_and_round_trip():
              (N.RL2_TUNE_BATCH_GROUPS, [512, 0], [-1, 70000]),
              (N.RL2_TUNE_BATCH_GRAPH, [0, 1], [2, -1]),
              (N.RL2_TUNE_ITEM_GROUPS, [256, 0], [-1, 70000]),
-             (N.RL2_TUNE_FUSE_STEP, [0, 1], [2, -1]),
+             (N.RL2_TUNE_FUSE_STEP, [0, 2, 1], [3, -1]),
              (N.RL2_TUNE_TILE_PASSES, [1, 50, 0], [-1, 51])]
     try:
         for knob, good, bad in cases:
