@@ -541,7 +541,7 @@ def main():
                     help="0: tiled likelihood; 64..1024: one work-group per (e, r) task")
     ap.add_argument("--tile-cells", type=int, default=0,
                     help="tiled likelihood cells per work-group (0: chosen per launch)")
-    ap.add_argument("--grid-variant", type=int, default=3, choices=[0, 1, 2, 3, 4, 5, 6],
+    ap.add_argument("--grid-variant", type=int, default=5, choices=[0, 1, 2, 3, 4, 5, 6],
                     help="extraction gridding: runs of 2 / 2 / 8 / 4 / 4 rows per thread with "
                          "tables of 512 / 1024 / 2048 / 1024 / 2048 slots flushed per channel "
                          "(0-4), or runs of 2 / 4 rows with 1024 / 2048 slots kept across "

@@ -196,7 +196,8 @@ int rl2_synchronize(void);
  *   before one LDS table update: 0 / 1 / 2 / 3 / 4 runs of 2 / 2 / 8 / 4 / 4 rows with
  *   tables of 512 / 1024 / 2048 / 1024 / 2048 slots, flushed to the facet after every
  *   channel; 5 / 6 runs of 2 / 4 rows with 1024 / 2048 slots that keep their cells across
- *   channels and are flushed only when the next channel might not fit; default 3.  Every
+ *   channels and are flushed only when the next channel might not fit; default 5 (the
+ *   fastest with the subtraction fused into the pass).  Every
  *   variant accumulates in integers, so the facet does not depend on the order of the
  *   work-groups (variants with the same rows per work-group give the same bits).
  * RL2_TUNE_TILE_CELLS: facet cells per work-group of the tiled likelihood, a power of two

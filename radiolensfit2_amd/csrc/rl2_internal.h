@@ -80,15 +80,13 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 
 // Power-of-two scales of the gridding's integer accumulators: a value x is carried as
 // rint(x * fx_scale) (visibility components) or rint(x * fs_scale) (sigma^2), with
-// |x| * scale * (rows of a gridding work-group) <= 2^62 (fx_scales(), for the gridding
-// variant selected when it is called).
+// |x| * scale * (terms a table slot takes between flushes, in any gridding path) <= 2^62.
 struct AccScale {
     double fx_scale = 1., fs_scale = 1.;
 };
 // The scales for visibility components bounded by vis_bound and variances by s2_bound.
-// rows_log2 < 0: the terms a slot of the current gridding variant takes between flushes
-// (the rows of a work-group, times the nch channels for the variants that keep cells across
-// channels).
+// rows_log2 < 0: the bound common to every gridding path for nch channels (the most terms
+// a slot takes between flushes in any of them), so that all paths give the same facets.
 AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2 = -1, unsigned nch = 1);
 // Words (8 bytes) of a facet's gridding accumulators.
 size_t grid_acc_words(unsigned facet);
@@ -196,8 +194,8 @@ void launch_subtract_batch(hipStream_t s, int prec, unsigned nch, unsigned long 
 
 // Launch-shape knobs (rl2_tune_set): read by the launchers on the host.
 struct Tuning {
-    int grid_variant = 3;      // RL2_TUNE_GRID_VARIANT: 0 per-visibility LDS hash,
-                               // 1 wave-aggregated cells, 2 / 3 per-thread runs of 8 / 4
+    int grid_variant = 5;      // RL2_TUNE_GRID_VARIANT (runs of rows per thread, LDS table
+                               // slots, flush per channel or deferred; see rl2.h)
     unsigned tile_cells = 0;   // RL2_TUNE_TILE_CELLS (0: chosen per launch)
     int phase_tables = 2;      // RL2_TUNE_PHASE_TABLES: separable Newton phases on grid facets
     int fuse_subtract = 1;     // RL2_TUNE_FUSE_SUBTRACT: a subtraction rides on the next extraction

@@ -204,9 +204,12 @@ void launch_add_noise(hipStream_t s, int prec, unsigned long nvis, double sigma,
 //    32 bits, added by integer atomics into two 64-bit accumulators per cell and component
 //    (no carries are lost: the low sums stay below 2^64 for fewer than 2^32 flushes);
 //  - compaction recombines hi * 2^32 + lo and scales by 2^-S.
-// A term's rounding (one per run of up to 4 visibilities) is at most B * 2^-53 for 1024-row
-// work-groups, and the sum itself is exact, so the facet agrees with the reference's sequential FP64 sums to
-// their own rounding and is bit-identical from run to run.
+// Each visibility is rounded on its own (runs of a cell are then added as integers), and
+// every gridding path takes S from one bound (the most terms any of them puts in a slot
+// between flushes), so the facet is the same bit for bit whatever the variant, the batch
+// path, the order of work-groups or the run.  A term's rounding is at most 2^-(S+1), below
+// B * 2^-48 for 16 channels, and the sum itself is exact, so the facet agrees with the
+// reference's sequential FP64 sums to their own rounding.
 // Accumulator layout: 7 arrays of facet^2 words: re hi, re lo, im hi, im lo, sigma^2 hi,
 // sigma^2 lo (uint64, two's complement), weight (double, integer-valued).
 enum { kAccReHi = 0, kAccReLo, kAccImHi, kAccImLo, kAccS2Hi, kAccS2Lo, kAccW, kAccWords };
@@ -264,7 +267,9 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
     const double inv_inc = 1.0 / a.inc;
     unsigned long long *acc = (unsigned long long *)a.acc;
     double *acc_w = a.acc + kAccW * ncell;
-    auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {
+    // a run's sums are already integers (each visibility rounded on its own, so the run
+    // length, and with it the variant, does not change a bit)
+    auto emit = [&](uint32_t key, long long xr, long long xi, long long xs, uint32_t n) {
         uint32_t h = (key * 2654435761u) >> (32 - HBITS);
         while (true) {
             const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
@@ -273,10 +278,10 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
             h = (h + 1) & (HS - 1);
         }
         if (MODE != 2) {
-            atomicAdd(&hre[h], (unsigned long long)fx_round(xr, a.fx_scale));
-            atomicAdd(&him[h], (unsigned long long)fx_round(xi, a.fx_scale));
+            atomicAdd(&hre[h], (unsigned long long)xr);
+            atomicAdd(&him[h], (unsigned long long)xi);
         }
-        atomicAdd(&hs2[h], (unsigned long long)fx_round(xs, a.fs_scale));
+        atomicAdd(&hs2[h], (unsigned long long)xs);
         atomicAdd(&hcnt[h], n);
     };
     // the rows' u, v and phase numerator p = ul + vm + wn stay in registers for all
@@ -297,7 +302,7 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
         const double wn = a.wn[c], inv_lambda = a.inv_lambda[c];
         const double wn2 = wn * wn, spectra = a.spec[c];
         uint32_t rkey = kEmpty, rn = 0;
-        double rre = 0., rim = 0., rs2 = 0.;
+        long long rre = 0, rim = 0, rs2 = 0;
         // the thread's RPT consecutive visibilities are loaded together, so each load
         // instruction's cache lines are still resident for the next one (issued one row
         // at a time between the gridding work, they were fetched from HBM ~3 times over)
@@ -350,17 +355,20 @@ __global__ __launch_bounds__(256) void grid_run_kernel(GridLaunch a)
                 xr = r.x;
                 xi = r.y;
             }
+            const long long ir = MODE != 2 ? fx_round(xr, a.fx_scale) : 0;
+            const long long ii = MODE != 2 ? fx_round(xi, a.fx_scale) : 0;
+            const long long is = fx_round((double)s2, a.fs_scale);
             if (key == rkey) {
-                rre += xr;
-                rim += xi;
-                rs2 += (double)s2;
+                rre += ir;
+                rim += ii;
+                rs2 += is;
                 rn++;
             } else {
                 if (rn) emit(rkey, rre, rim, rs2, rn);
                 rkey = key;
-                rre = xr;
-                rim = xi;
-                rs2 = (double)s2;
+                rre = ir;
+                rim = ii;
+                rs2 = is;
                 rn = 1;
             }
         }
@@ -825,8 +833,9 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
             for (unsigned t = 0; t < ntarget; t++) {
                 const BatchTarget T = a.desc->targets[t];
                 uint32_t rkey = kEmpty, rn = 0;
-                double rre = 0., rim = 0., rs2 = 0.;
-                auto emit = [&](uint32_t key, double xr, double xi, double xs, uint32_t n) {
+                long long rre = 0, rim = 0, rs2 = 0;  // integer run sums, as grid_run_kernel
+                auto emit = [&](uint32_t key, long long xr, long long xi, long long xs,
+                                uint32_t n) {
                     uint32_t h = (key * 2654435761u) >> (32 - HBITS);
                     while (true) {
                         const uint32_t prev = atomicCAS(&hkey[h], kEmpty, key);
@@ -834,9 +843,9 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
                         if (prev == kEmpty || prev == key) break;
                         h = (h + 1) & (HS - 1);
                     }
-                    atomicAdd(&hre[h], (unsigned long long)fx_round(xr, T.fx_scale));
-                    atomicAdd(&him[h], (unsigned long long)fx_round(xi, T.fx_scale));
-                    atomicAdd(&hs2[h], (unsigned long long)fx_round(xs, a.fs_scale));
+                    atomicAdd(&hre[h], (unsigned long long)xr);
+                    atomicAdd(&him[h], (unsigned long long)xi);
+                    atomicAdd(&hs2[h], (unsigned long long)xs);
                     atomicAdd(&hcnt[h], n);
                 };
 #pragma unroll
@@ -859,17 +868,20 @@ __global__ __launch_bounds__(256) void grid_batch_kernel(GridBatchLaunch a)
                     const double gim = shape * sn + vres[k].y;
                     const double xr = gre * cs - gim * (-sn);
                     const double xi = gre * (-sn) + gim * cs;
+                    const long long ir = fx_round(xr, T.fx_scale);
+                    const long long ii = fx_round(xi, T.fx_scale);
+                    const long long is = fx_round((double)s2, a.fs_scale);
                     if (key == rkey) {
-                        rre += xr;
-                        rim += xi;
-                        rs2 += (double)s2;
+                        rre += ir;
+                        rim += ii;
+                        rs2 += is;
                         rn++;
                     } else {
                         if (rn) emit(rkey, rre, rim, rs2, rn);
                         rkey = key;
-                        rre = xr;
-                        rim = xi;
-                        rs2 = (double)s2;
+                        rre = ir;
+                        rim = ii;
+                        rs2 = is;
                         rn = 1;
                     }
                 }
@@ -963,18 +975,15 @@ void launch_subtract_batch(hipStream_t s, int prec, unsigned nch, unsigned long 
 // Rows of a gridding work-group (256 threads x rows per thread) for the current variant.
 // log2 of the visibilities one LDS slot can receive between flushes: the rows of a
 // work-group, times the channels for the variants that keep cells across channels.
+// The most terms any gridding path puts in one table slot between flushes, as a power of
+// two: 2048 rows (runs of 8), or 1024 rows times every channel (runs of 4 kept across
+// channels).  Every path (each variant, the batched kernel) takes its scale from this one
+// bound, so their integer sums, and the facets, are the same bit for bit.
 static int grid_rows_log2(unsigned nch)
 {
-    int lazy = 0;
-    if (tuning().grid_variant >= 5)
-        while ((1u << lazy) < nch) lazy++;
-    switch (tuning().grid_variant) {
-    case 2: return 11;                 // runs of 8 rows
-    case 3: case 4: return 10;         // runs of 4 rows
-    case 6: return 10 + lazy;
-    case 5: return 9 + lazy;
-    default: return 9;                 // runs of 2 rows
-    }
+    int ch = 0;
+    while ((1u << ch) < nch) ch++;
+    return std::max(11, 10 + ch);
 }
 
 AccScale fx_scales(double vis_bound, double s2_bound, int rows_log2, unsigned nch)

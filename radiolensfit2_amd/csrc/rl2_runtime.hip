@@ -534,7 +534,7 @@ static void batch_launches(Obs &o, unsigned ncap, unsigned groups)
     a.res = o.d_res;
     a.s2f = o.d_s2f;
     a.len = o.len;
-    a.fs_scale = fx_scales(1., o.s2_max, 10).fs_scale;
+    a.fs_scale = fx_scales(1., o.s2_max, -1, o.nch).fs_scale;
     a.desc = o.d_desc;
     a.queue = o.d_queue;
     a.ntiles = (unsigned)((o.nrow + kBatchTileRows - 1) / kBatchTileRows);
@@ -575,7 +575,8 @@ bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
         T.g = make_geom(r.l0, r.m0, r.flux, r.R_mu, 0., 0.);
         T.inc = 2 * o.len / r.facet;
         T.inv_inc = 1.0 / T.inc;
-        const AccScale sc = fx_scales(fabs(r.flux) * o.spec_max + o.res_bound, o.s2_max, 10);
+        const AccScale sc =
+            fx_scales(fabs(r.flux) * o.spec_max + o.res_bound, o.s2_max, -1, o.nch);
         T.fx_scale = sc.fx_scale;
         T.acc = sl.acc;
         T.facet = r.facet;
@@ -1784,7 +1785,7 @@ int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double 
         knobs.push_back({RL2_TUNE_BATCH_GROUPS, {(int)num_cus(), 2 * (int)num_cus(),
                                                  3 * (int)num_cus(), 4 * (int)num_cus()}});
     else
-        knobs.push_back({RL2_TUNE_GRID_VARIANT, {3, 4, 2, 1, 0}});
+        knobs.push_back({RL2_TUNE_GRID_VARIANT, {5, 3, 6, 4, 2, 1, 0}});
     knobs.push_back({RL2_TUNE_ITEM_GROUPS, {0, 256, 1024}});
     knobs.push_back({RL2_TUNE_TILE_CELLS, {0, 2048, 4096, 8192, 16384}});
     knobs.push_back({RL2_TUNE_TILE_STREAMS, {1, 2}});

@@ -253,23 +253,30 @@ def test_mixed_likelihood_tracks_fp64(dev, case):
 
 
 def test_gridding_is_deterministic(dev, case):
-    """Integer accumulation: the same extraction twice, and with another table size for the
-    same runs (variants 3 and 4: runs of 4 rows, 1024 / 2048 slots), gives bit-identical
-    facets; the whole loop run twice gives bit-identical results and residual."""
+    """Integer accumulation with one fixed-point scale for every gridding path: the same
+    extraction twice, and with other variants (5: runs of 2 rows kept across channels, the
+    default; 3 and 4: runs of 4 rows flushed per channel, 1024 / 2048 slots), gives
+    bit-identical facets; the whole loop run twice gives bit-identical results and
+    residual."""
     obs, cat, vs = case
     H = N.hip()
     g = 4
     facet = H.facet_size(cat.scale[g], obs.len)
-    a = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
-    b = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
-    N.check(H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 4), "rl2_tune_set")
+    d0 = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
+    d1 = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
     try:
+        N.check(H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 3), "rl2_tune_set")
+        a = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
+        b = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
+        N.check(H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 4), "rl2_tune_set")
         c = dev.extract(cat.l[g], cat.m[g], cat.flux[g], cat.scale[g], facet, slot=2)
     finally:
-        N.check(H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 3), "rl2_tune_set")
-    for x, y, z in zip(a, b, c):
+        N.check(H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 5), "rl2_tune_set")
+    for x, y, z, p, q in zip(d0, d1, a, b, c):
         np.testing.assert_array_equal(x, y)
         np.testing.assert_array_equal(x, z)
+        np.testing.assert_array_equal(x, p)
+        np.testing.assert_array_equal(x, q)
     runs = []
     for k in range(2):
         d = P.DeviceObservation(obs, vis_sky=vs)
@@ -340,7 +347,7 @@ def test_autotune_keeps_residual_and_results(gpu, case, batch):
         assert np.abs(got["e2"] - ref["e2"]).max() <= 1e-6
     finally:
         d.close()
-        for k, v in ((N.RL2_TUNE_GRID_VARIANT, 3), (N.RL2_TUNE_TILE_CELLS, 0),
+        for k, v in ((N.RL2_TUNE_GRID_VARIANT, 5), (N.RL2_TUNE_TILE_CELLS, 0),
                      (N.RL2_TUNE_TILE_STREAMS, 1), (N.RL2_TUNE_BATCH_GROUPS, 0),
                      (N.RL2_TUNE_ITEM_GROUPS, 0)):
             N.check(H.rl2_tune_set(k, v), "rl2_tune_set")

@@ -239,7 +239,7 @@ def test_tune_knobs_validate_and_round_trip():
                 assert v.value == good[-1]
         assert H.rl2_tune_set(99, 0) == N.RL2_ERR_ARG
     finally:  # defaults
-        H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 3)
+        H.rl2_tune_set(N.RL2_TUNE_GRID_VARIANT, 5)
         H.rl2_tune_set(N.RL2_TUNE_TILE_CELLS, 0)
         H.rl2_tune_set(N.RL2_TUNE_PHASE_TABLES, 2)
         H.rl2_tune_set(N.RL2_TUNE_FUSE_SUBTRACT, 1)
