@@ -433,7 +433,7 @@ def run_c3(args):
                                                     out["paths"]["fp64"]["galaxies_per_s"], 3),
                   "galaxies_per_s_within_gate": round(p["galaxies_per_s"] * float(ok.mean()), 2)})
         log(f"[c3] {name}: within 1e-4 of FP64 {ok.mean():.3f}, median |de| {np.median(de):.2e}")
-    if args.cpu_sample > 0:
+    if args.cpu_sample > 0 and args.c3_oracle > 0:
         n = min(ngal, args.c3_oracle)
         dev.restore()
         resid0 = dev.residual()

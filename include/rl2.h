@@ -223,7 +223,8 @@ int rl2_synchronize(void);
  *   them one by one.  Both give identical results.
  * RL2_TUNE_ITEM_GROUPS: work-groups the per-facet chunk choice aims at for one likelihood
  *   item (RL2_TUNE_TILE_CELLS 0): the chunk is the largest power of two from 2048 cells that
- *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 256.
+ *   still gives the item's 23 scalelength tasks that many work-groups; 0 (default): 256 for
+ *   FP64 items, 512 for FP32 items.
  * RL2_TUNE_FUSE_STEP: 1 (default) folds each Newton step of the likelihood into the next
  *   pass's launch (its work-groups take the step and make their own phase tables), so a pass
  *   is one launch; 2 runs every Newton pass of an evaluation in one cooperative launch (the

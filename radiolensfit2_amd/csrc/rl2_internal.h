@@ -284,7 +284,8 @@ void launch_rmarg(hipStream_t s, const LikLaunch &a);
 constexpr int kTileBS = 256;
 constexpr unsigned kTileMinCells = 2048, kTileMaxCells = 65536;
 constexpr unsigned kTileKnobMinCells = 256;   // smallest chunk RL2_TUNE_TILE_CELLS accepts
-constexpr unsigned long long kTileItemGroups = 256;  // per item (fastest measured with the fused passes)
+constexpr unsigned long long kTileItemGroups = 256;    // per FP64 item (fused Newton passes)
+constexpr unsigned long long kTileItemGroups32 = 512;  // per FP32 item (two-kernel passes)
 constexpr int kInlItems = 8;
 // LDS record of one grid index in the fused Newton passes: {X (16 bytes), Y (16), ctr (8)}.
 constexpr unsigned kRecBytes = 40;

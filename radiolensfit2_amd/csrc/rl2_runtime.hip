@@ -749,13 +749,17 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // launch typically carries 3 to 8 items (the simplex's candidates, a line search's
         // steps), which then fill the device with a few thousand work-groups.
         const int nr = o.rg.numr - 1;
-        auto item_chunk = [&](unsigned cells) {
+        // FP32 items run the two-kernel passes, whose shorter work-groups favour the
+        // smaller chunks of kTileItemGroups32
+        auto item_chunk = [&](unsigned cells, int prec) {
             unsigned chunk = tuning().tile_cells;
             if (chunk == 0) {
+                const unsigned long long target =
+                    tuning().item_groups ? tuning().item_groups
+                                         : (prec == 0 ? kTileItemGroups : kTileItemGroups32);
                 chunk = kTileMinCells;
                 while (chunk < kTileMaxCells &&
-                       (unsigned long long)nr * ((cells + 2 * chunk - 1) / (2 * chunk)) >=
-                           (tuning().item_groups ? tuning().item_groups : kTileItemGroups))
+                       (unsigned long long)nr * ((cells + 2 * chunk - 1) / (2 * chunk)) >= target)
                     chunk *= 2;
             }
             return chunk;
@@ -763,7 +767,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         unsigned chunks = 0;
         for (int i = 0; i < n; i++) {
             const unsigned cells = o.slots[items[i].slot].n;
-            const unsigned chunk = item_chunk(cells);
+            const unsigned chunk = item_chunk(cells, items[i].prec);
             b.h_items[i].chunk_first = chunks;
             b.h_items[i].chunk_cells = chunk;
             chunks += std::max(1u, (cells + chunk - 1) / chunk);
