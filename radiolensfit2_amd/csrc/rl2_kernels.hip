@@ -2827,6 +2827,55 @@ __device__ void tile_status(const LikLaunch &a)
 }
 
 // ------------------------------------------------------------------ Newton passes, fused
+// The step a fused pass's work-group takes before its sums (wave 0; the caller
+// synchronises): the task's chunk sums of the previous pass added in chunk order, the
+// Newton step, the state stored by the task's chunk-0 work-group (to the other
+// generation) with L_r when the task ends; *go and off[2] receive whether the task goes on
+// and its new offset.
+__device__ __attribute__((always_inline)) inline void fused_step(const TileLaunch &t, unsigned first,
+                                                                 unsigned nch,
+                                  unsigned chunk, int item, int r, int *go_out, double *off)
+{
+    if (threadIdx.x >= 64) return;
+    const LikLaunch &a = t.lk;
+    const size_t ti = (size_t)item * kMaxR + r;
+    const int lane = threadIdx.x;
+    const TileTask *T = t.tasks + ti;
+    int go = 0;
+    double xo = 0., yo = 0.;
+    if (t.mode == 2) {
+        if (lane == 63) {
+            go = T->active;
+            xo = T->xo;
+            yo = T->yo;
+        }
+    } else if (t.mode == 0 || T->active) {
+        double v[7];
+        task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch, t.nr_g, lane,
+                  v);
+        if (lane == 63) {
+            TileTask S = t.mode == 0 ? TileTask{} : *T;
+            double ho, det;
+            const int done = t.mode == 0 ? newton_update<0>(S, v, &ho, &det)
+                                         : newton_update<1>(S, v, &ho, &det);
+            if (chunk == 0) {
+                t.tasks_out[ti] = S;
+                if (done) finish_lr(a, item, r, ho, det, S.iter);
+            }
+            go = !done;
+            xo = S.xo;
+            yo = S.yo;
+        }
+    } else if (lane == 63 && chunk == 0) {
+        t.tasks_out[ti] = *T;  // a finished task keeps its state in both generations
+    }
+    if (lane == 63) {
+        *go_out = go;
+        off[0] = xo;
+        off[1] = yo;
+    }
+}
+
 // One Newton pass with the previous pass's step folded in, so that a pass is one launch
 // instead of two.  Each work-group first adds its task's chunk sums of the previous pass
 // (wave 0, in the step kernel's order) and takes the Newton step itself, so every
@@ -2879,43 +2928,7 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         }
         const unsigned row_lo = end > beg ? gload(cid, beg) >> 16 : 0u;
         const unsigned row_hi = end > beg ? gload(cid, end - 1) >> 16 : 0u;
-        if (threadIdx.x < 64) {
-            const int lane = threadIdx.x;
-            const TileTask *T = t.tasks + ti;
-            int go = 0;
-            double xo = 0., yo = 0.;
-            if (t.mode == 2) {
-                if (lane == 63) {
-                    go = T->active;
-                    xo = T->xo;
-                    yo = T->yo;
-                }
-            } else if (t.mode == 0 || T->active) {
-                double v[7];
-                task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch,
-                          t.nr_g, lane, v);
-                if (lane == 63) {
-                    TileTask S = t.mode == 0 ? TileTask{} : *T;
-                    double ho, det;
-                    const int done = t.mode == 0 ? newton_update<0>(S, v, &ho, &det)
-                                                 : newton_update<1>(S, v, &ho, &det);
-                    if (chunk == 0) {
-                        t.tasks_out[ti] = S;
-                        if (done) finish_lr(a, item, r, ho, det, S.iter);
-                    }
-                    go = !done;
-                    xo = S.xo;
-                    yo = S.yo;
-                }
-            } else if (lane == 63 && chunk == 0) {
-                t.tasks_out[ti] = *T;  // a finished task keeps its state in both generations
-            }
-            if (lane == 63) {
-                s_go = go;
-                s_off[0] = xo;
-                s_off[1] = yo;
-            }
-        }
+        fused_step(t, first, nch, chunk, item, r, &s_go, s_off);
         __syncthreads();
         if (!s_go) return false;
         // the column runs, the runs of the rows the chunk spans (cells are in row-major
@@ -3206,9 +3219,175 @@ bool launch_persist(hipStream_t s, const PersistLaunch &p, unsigned blocks)
            hipSuccess;
 }
 
+// Fused Newton passes on facets wider than kRecFacet (up to kWideFacet columns): a 24-byte
+// record {X, ctr} per column in LDS, and the row-table entries of the rows the work-group's
+// chunk spans in a small cache (entry `span` holds row 0, which the zero pad cell reads;
+// a span beyond the cache evaluates the row entries per cell from the run tables).  Cell
+// ids from cid; the arithmetic and its order are tile_sums64_tab's.
+constexpr unsigned kWideRows = 256;
+struct ColEntry {
+    double xr, xi, c;
+};
+template <bool CACHED, class Stage>
+__device__ inline bool tile_sums64_col(const FacetDev &F, const LikItem &it, double ro,
+                                       const Stage &stage, const ColEntry *col,
+                                       const double2 *sY, const TabRot<128> &R,
+                                       unsigned row_lo, unsigned span, unsigned beg,
+                                       unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    const uint32_t *cid = F.cid;
+    const double2 *dc = F.dc;
+    const unsigned pad = F.n;
+    auto fetch = [&](RecCells &b, unsigned base) {
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const unsigned i = base + q * kTileBS;
+            const unsigned k = i < end ? i : pad;
+            b.o[q] = gload(cid, k);
+            const double2 d = gload(dc, k);
+            b.dr[q] = d.x;
+            b.di[q] = d.y;
+        }
+    };
+    auto eval = [&](const RecCells &b) {
+        double uu[kTabU], vv[kTabU];
+        double2 x[kTabU], y[kTabU];
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const unsigned ci = b.o[q] & 0xffffu, cj = b.o[q] >> 16;
+            const ColEntry &cc = col[ci];
+            uu[q] = cc.c;
+            vv[q] = col[cj].c;
+            x[q] = make_double2(cc.xr, cc.xi);
+            // rows below row_lo (only row 0, the pad cell's) wrap past span: entry span
+            y[q] = CACHED ? sY[min(cj - row_lo, span)] : tab_y(R, cj);
+        }
+#pragma unroll
+        for (int q = 0; q < kTabU; q++) {
+            const double k1 = fma(e2, vv[q], a1 * uu[q]);
+            const double k2 = fma(a2, vv[q], e2 * uu[q]);
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = b.dr[q] * M;
+            const double hi = -b.di[q] * M;
+            const double cs = fma(x[q].x, y[q].x, -(x[q].y * y[q].y));
+            const double sn = fma(x[q].y, y[q].x, x[q].x * y[q].y);
+            const double re = fma(hr, cs, -hi * sn);
+            const double im = fma(hr, sn, hi * cs);
+            s[0] += re;
+            s[1] = fma(-im, uu[q], s[1]);
+            s[2] = fma(-im, vv[q], s[2]);
+            const double rU = re * uu[q];
+            s[3] = fma(-rU, uu[q], s[3]);
+            s[4] = fma(-re * vv[q], vv[q], s[4]);
+            s[5] = fma(-rU, vv[q], s[5]);
+        }
+    };
+    constexpr unsigned S = kTabU * kTileBS;
+    RecCells A, B;
+    fetch(A, beg + threadIdx.x);
+    if (!stage()) return false;
+#pragma unroll
+    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.o[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
+    for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
+        fetch(B, base + S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(A);
+        if (base + S >= end) break;
+        fetch(A, base + 2 * S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(B);
+    }
+    s[1] *= kTwoPi;
+    s[2] *= kTwoPi;
+    constexpr double kTwoPi2 = kTwoPi * kTwoPi;
+    s[3] *= kTwoPi2;
+    s[4] *= kTwoPi2;
+    s[5] *= kTwoPi2;
+    return true;
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel(TileLaunch t)
+{
+    __shared__ double red[(kTileBS / 64) * 8];
+    __shared__ int s_go;
+    __shared__ double s_off[2];
+    __shared__ TabRot<128> R;  // gf <= kWideFacet < 128 runs of kTabRun
+    const unsigned per = (t.nwg + kXcds - 1) / kXcds;
+    const unsigned wg = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+    if (wg >= t.nwg) return;
+    const LikLaunch &a = t.lk;
+    const LikItem *items = KIND == 2 ? t.inl : a.items;
+    const int item = tile_item(items, a.nitems, wg, t.nr_g);
+    unsigned first, nch;
+    tile_range(t, items, item, &first, &nch);
+    const unsigned local = wg - first;
+    const unsigned chunk = local / (unsigned)t.nr_g;
+    const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
+    const LikItem it = items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned beg = chunk * it.chunk_cells;
+    const unsigned end = min(F.n, beg + it.chunk_cells);
+    const unsigned gf = F.grid_facet;
+    // t.tab_lds_facet (rounded up to even) column records, then kWideRows row entries
+    extern __shared__ double2 tab_lds[];
+    ColEntry *col = reinterpret_cast<ColEntry *>(tab_lds);
+    double2 *sY = tab_lds + (size_t)((t.tab_lds_facet + 1) & ~1u) * 3 / 2;
+    const uint32_t *cid = F.cid;
+    const double *ctr = F.ctr;
+    const unsigned row_lo = end > beg ? gload(cid, beg) >> 16 : 0u;
+    const unsigned row_hi = end > beg ? gload(cid, end - 1) >> 16 : 0u;
+    const unsigned span = row_hi - row_lo + 1;
+    const bool cached = span < kWideRows;  // entry span holds row 0
+    auto make = [&]() -> bool {
+        fused_step(t, first, nch, chunk, item, r, &s_go, s_off);
+        __syncthreads();
+        if (!s_go) return false;
+        const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
+        tab_rot_stage(R, ctr, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
+        __syncthreads();
+        for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
+            const double2 x = tab_x(R, k);
+            col[k].xr = x.x;
+            col[k].xi = x.y;
+            col[k].c = gload(ctr, k);
+        }
+        if (cached)
+            for (unsigned q = threadIdx.x; q <= span; q += kTileBS)
+                sY[q] = tab_y(R, q < span ? row_lo + q : 0u);
+        __syncthreads();
+        return true;
+    };
+    double s[7];
+    const bool ran =
+        cached ? tile_sums64_col<true>(F, it, a.rg.ro[r], make, col, sY, R, row_lo, span, beg,
+                                       end, s)
+               : tile_sums64_col<false>(F, it, a.rg.ro[r], make, col, sY, R, row_lo, span, beg,
+                                        end, s);
+    if (ran) tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
+}
+
 void launch_tile_fused(hipStream_t s, const TileLaunch &t)
 {
     const unsigned g = tile_grid(t.nwg, 0);
+    if (t.tab_lds_facet > kRecFacet) {
+        const unsigned lds =
+            ((t.tab_lds_facet + 1) & ~1u) * (unsigned)sizeof(ColEntry) + kWideRows * 16;
+        if (t.grid == 2)
+            lik_fused_wide_kernel<2><<<g, kTileBS, lds, s>>>(t);
+        else
+            lik_fused_wide_kernel<1><<<g, kTileBS, lds, s>>>(t);
+        return;
+    }
     const unsigned lds = t.tab_lds_facet * kRecBytes;
     if (t.grid == 2)
         lik_fused_kernel<2><<<g, kTileBS, lds, s>>>(t);

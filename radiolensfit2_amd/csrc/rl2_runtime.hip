@@ -850,12 +850,14 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // queued pass finishes the tasks that converged with it.  Task state and chunk sums
         // alternate between two generations (index si / pi: the latest).  They need every
         // item on a grid facet with its row table in LDS, and one stream.
-        const bool fused = tuning().fuse_step && ng == 1 && tg[0].grid && tg[0].tab_y_lds;
+        // (every item on a grid facet with its tables; facets up to kRecFacet wide use the
+        // 40-byte records, wider ones, up to the column tables' LDS limit, column records)
+        const bool fused = tuning().fuse_step && ng == 1 && tg[0].grid;
         // One launch per evaluation (RL2_TUNE_FUSE_STEP 2): every block resident, XCD x
         // owning an eighth of each task's chunks, each block an equal run of its XCD's
         // (task, chunk) list; runs that would span more tasks than a block holds fall
         // back to the fused passes.
-        if (fused && tuning().fuse_step == 2) {
+        if (fused && tuning().fuse_step == 2 && tg[0].tab_y_lds) {
             TileLaunch &t0 = tg[0];
             const unsigned cap = persist_capacity(t0.tab_lds_facet * kRecBytes, (int)t0.grid);
             unsigned long long W[kXcds] = {}, minm[kXcds];

@@ -199,6 +199,59 @@ def test_process_matches_oracle(gpu, case, batch):
     assert np.abs(resid - ores_vis).max() <= 1e-10 * scale
 
 
+def test_wide_facet_fused_passes_match_step_kernels(gpu):
+    """A facet wider than 1024 columns (a 3.4 arcsec galaxy on 500 km baselines: 1150) takes the
+    fused Newton passes with column records and a cache of its chunk's row entries (or, for
+    a chunk spanning more rows than the cache, row entries evaluated per cell); the values
+    equal the step-kernel passes bit for bit and the oracle to 1e-8 with equal Newton step
+    counts, at the default chunks and at 256-cell chunks."""
+    from radiolensfit2_amd import synth
+    cfg = synth.ArrayConfig(n_dish=24, seed=3, r_max=500e3, n_times=40, n_chan=2, bw_hz=15e6)
+    cat = synth.Catalog(*(np.array(x, dtype=np.float64) for x in
+                          ([150.0], [3.4], [0.2], [-0.1], [2e-5], [-1e-5], [0.0])))
+    obs = synth.make_observation(cfg, cat, noise=True)
+    vs = T.oracle_sky_model(obs, cat)
+    H = N.hip()
+    facet = H.facet_size(cat.scale[0], obs.len)
+    assert facet > 1024
+    d = P.DeviceObservation(obs, vis_sky=vs)
+    try:
+        gv, gs, guu, gvv, gsw = d.extract(cat.l[0], cat.m[0], cat.flux[0], cat.scale[0], facet,
+                                          slot=0)
+        d.set_prior(P.rprior_for(cat.flux[0], P.r_grid()), slot=0)
+        params, keep = T.oracle_params(gv, gs, guu, gvv, cat.flux[0])
+        e = np.array([[0.0, 0.0], [0.2, -0.1], [-0.35, 0.25], [0.5, 0.1]])
+        print(f"wide facet {facet}: {len(gv)} cells", flush=True)
+        from oracle import oracle as O
+        import ctypes as C
+        for cells in (0, 256):
+            out = {}
+            try:
+                N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_CELLS, cells), "rl2_tune_set")
+                for fuse in (1, 0):
+                    N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, fuse), "rl2_tune_set")
+                    out[fuse] = d.loglik(e[:, 0], e[:, 1], slot=0, detail=True)
+            finally:
+                N.check(H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, 1), "rl2_tune_set")
+                N.check(H.rl2_tune_set(N.RL2_TUNE_TILE_CELLS, 0), "rl2_tune_set")
+            for a, b in zip(out[1], out[0]):
+                np.testing.assert_array_equal(a, b)
+            Le, err, Lr, nit = out[1]
+            for k in range(len(e)):
+                oLr = np.zeros(40)
+                onit = np.zeros(40, np.int32)
+                oerr = C.c_int()
+                oLe = O.lib().orl2_loglikelihood_detail(C.byref(params), e[k, 0], e[k, 1],
+                                                        C.byref(oerr), O.ptr(oLr),
+                                                        onit.ctypes.data_as(O.c_int_p))
+                assert err[k] == oerr.value
+                np.testing.assert_array_equal(nit[k, 1:24], onit[1:24])
+                np.testing.assert_allclose(Lr[k, 1:24], oLr[1:24], rtol=0, atol=1e-8)
+                assert abs(Le[k] - oLe) <= 1e-8, (cells, k, Le[k], oLe)
+    finally:
+        d.close()
+
+
 @pytest.mark.parametrize("batch", [1, 4])
 def test_fused_subtraction_matches_separate_passes(gpu, case, batch):
     """A subtraction made in the same pass as the next extraction (RL2_TUNE_FUSE_SUBTRACT)
