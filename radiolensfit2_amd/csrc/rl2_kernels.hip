@@ -1700,32 +1700,201 @@ __device__ inline double wave_sum(double v)
     return __hiloint2double(hi, lo);
 }
 
-__device__ inline void qk41_wave(const Spline &sp, double a, double b, double *result,
-                                 double *abserr, double *resabs, double *resasc)
+// Lane values read into scalar registers (l uniform over the wave).
+__device__ inline double read_lane(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// The wave's maximum (MAX) or minimum in every lane, by the lane shifts of wave_sum_last; a
+// lane whose source lies outside its row, or in a row masked off, keeps its own value, which
+// leaves the extreme unchanged.  fmax and fmin are exact (and skip NaN), so the result does
+// not depend on the order.
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_keep_f64(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW_MASK, 0xf, false);
+    const int rhi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW_MASK, 0xf, false);
+    return __hiloint2double(rhi, rlo);
+}
+template <bool MAX>
+__device__ inline double wave_extreme(double v)
+{
+    auto op = [](double p, double q) { return MAX ? fmax(p, q) : fmin(p, q); };
+    v = op(v, dpp_keep_f64<0x111, 0xf>(v));  // row_shr:1
+    v = op(v, dpp_keep_f64<0x112, 0xf>(v));  // row_shr:2
+    v = op(v, dpp_keep_f64<0x114, 0xf>(v));  // row_shr:4
+    v = op(v, dpp_keep_f64<0x118, 0xf>(v));  // row_shr:8
+    v = op(v, dpp_keep_f64<0x142, 0xa>(v));  // row_bcast:15 into rows 1 and 3
+    v = op(v, dpp_keep_f64<0x143, 0xc>(v));  // row_bcast:31 into rows 2 and 3
+    return read_lane(v, 63);
+}
+
+// The spline of the r-marginalisation in LDS with the cubic coefficients b_i and d_i of each
+// interval (cspline_eval's), formed once per spline: spline_exp_lds then takes no division
+// per evaluation and, as b_i and d_i are the same expressions of the same knot data, gives
+// spline_exp's values bit for bit.
+struct SplineLds {
+    Spline s;
+    double b[kMaxR], d[kMaxR];
+};
+
+__device__ inline double spline_exp_lds(const SplineLds &t, double x)
+{
+    const Spline &s = t.s;
+    if (x < s.x[0] || x > s.x[s.n - 1]) return 0.;
+    int lo = 0, hi = s.n - 1;
+    while (hi > lo + 1) {
+        const int i = (hi + lo) / 2;
+        if (s.x[i] > x)
+            hi = i;
+        else
+            lo = i;
+    }
+    const double x_lo = s.x[lo], x_hi = s.x[lo + 1];
+    const double dx = x_hi - x_lo;
+    if (!(dx > 0.0)) return 0.;
+    const double y_lo = s.y[lo];
+    double y;
+    if (s.linear) {
+        y = y_lo + (x - x_lo) / dx * (s.y[lo + 1] - y_lo);
+    } else {
+        const double delx = x - x_lo;
+        y = y_lo + delx * (t.b[lo] + delx * (s.c[lo] + delx * t.d[lo]));
+    }
+    return exp(y);
+}
+
+// spline_init_ws (GSL cspline_init semantics) by each wave: lane p holds knot (xp, yp) for
+// p < n, the knots already in t.s.x and t.s.y.  The rows of the tridiagonal system are formed
+// by their lanes at once; the LDL^T recurrences run over lane values read into scalar
+// registers, in spline_init_ws's order and with its expressions, so c, and the b_i and d_i
+// formed from it, are the same bit for bit.  Wave 0 writes n, linear = 0, c, b and d.
+// Returns spline_init_ws's status, the same in every lane.
+__device__ int spline_build_wave(SplineLds &t, int n, double xp, double yp)
+{
+    const int lane = threadIdx.x & 63;
+    const bool w0 = threadIdx.x < 64;
+    const double x1 = __shfl_down(xp, 1), x2 = __shfl_down(xp, 2);
+    const double y1 = __shfl_down(yp, 1), y2 = __shfl_down(yp, 2);
+    if (__any(lane + 1 < n && !(xp < x1))) return 1;  // knots strictly increasing
+    const int max_index = n - 1, sys = max_index - 1;
+    double off = 0., diag = 0., g = 0.;
+    if (lane < sys) {
+        const double h_i = x1 - xp;
+        const double h_ip1 = x2 - x1;
+        const double yd_i = y1 - yp;
+        const double yd_ip1 = y2 - y1;
+        const double g_i = (h_i != 0.0) ? 1.0 / h_i : 0.0;
+        const double g_ip1 = (h_ip1 != 0.0) ? 1.0 / h_ip1 : 0.0;
+        off = h_ip1;
+        diag = 2.0 * (h_ip1 + h_i);
+        g = 3.0 * (yd_ip1 * g_ip1 - yd_i * g_i);
+    }
+    double c = 0.;  // lane p: c_p, with c_0 = c_{n-1} = 0
+    int status = 0;
+    if (sys == 1) {
+        const double c1 = read_lane(g, 0) / read_lane(diag, 0);
+        if (lane == 1) c = c1;
+    } else if (sys > 1) {
+        double alp_l = 0., gam_l = 0.;  // lane i: alp_i, gam_i
+        double al = read_lane(diag, 0);
+        double gm = read_lane(off, 0) / al;
+        if (al == 0) status = 1;
+        if (lane == 0) {
+            alp_l = al;
+            gam_l = gm;
+        }
+        for (int i = 1; i < sys - 1; i++) {
+            al = read_lane(diag, i) - read_lane(off, i - 1) * gm;
+            gm = read_lane(off, i) / al;
+            if (al == 0) status = 1;
+            if (lane == i) {
+                alp_l = al;
+                gam_l = gm;
+            }
+        }
+        al = read_lane(diag, sys - 1) - read_lane(off, sys - 2) * gm;
+        if (lane == sys - 1) alp_l = al;
+        double z = read_lane(g, 0);
+        double z_l = lane == 0 ? z : 0.;
+        for (int i = 1; i < sys; i++) {
+            z = read_lane(g, i) - read_lane(gam_l, i - 1) * z;
+            if (lane == i) z_l = z;
+        }
+        const double cc = lane < sys ? z_l / alp_l : 0.;
+        double xs = read_lane(cc, sys - 1);
+        if (lane == sys) c = xs;
+        for (int i = sys - 2; i >= 0; i--) {
+            xs = read_lane(cc, i) - read_lane(gam_l, i) * xs;
+            if (lane == i + 1) c = xs;
+        }
+    }
+    const double c1 = __shfl_down(c, 1);
+    if (w0) {
+        if (lane == 0) {
+            t.s.n = n;
+            t.s.linear = 0;
+        }
+        if (lane < n) t.s.c[lane] = c;
+        if (lane + 1 < n) {
+            const double dx = x1 - xp;
+            const double dy = y1 - yp;
+            t.b[lane] = (dy / dx) - dx * (c1 + 2.0 * c) / 3.0;
+            t.d[lane] = (c1 - c) / (3.0 * dx);
+        }
+    }
+    return status;
+}
+
+// Lane j's Gauss-Kronrod-41 node (0..19 left of the centre, 20..39 right, 40 the centre) and
+// its Kronrod and Gauss weights, read once per work-group.
+struct Gk41Node {
+    double x, wk, wg;
+    int side;  // -1 left, 1 right, 0 the centre (and the unused lanes 41..63)
+};
+
+__device__ inline Gk41Node gk41_node()
+{
+    const int lane = threadIdx.x & 63;
+    const int k = lane < 20 ? lane : lane - 20;
+    Gk41Node g{0., 0., 0., 0};
+    if (lane < 41) {
+        g.side = lane < 20 ? -1 : (lane < 40 ? 1 : 0);
+        g.x = lane < 40 ? kXgk41[k] : 0.;
+        g.wk = lane < 40 ? kWgk41[k] : kWgk41[20];
+        g.wg = (lane < 40 && (k & 1)) ? kWg41[k >> 1] : 0.;
+    }
+    return g;
+}
+
+__device__ inline void qk41_wave(const SplineLds &sp, const Gk41Node &g, double a, double b,
+                                 double *result, double *abserr, double *resabs,
+                                 double *resasc)
 {
     const int lane = threadIdx.x & 63;
     const double center = 0.5 * (a + b);
     const double half_length = 0.5 * (b - a);
     const double abs_half_length = fabs(half_length);
-    const int k = lane < 20 ? lane : lane - 20;  // node index of lanes 0..39
-    double f = 0., wk = 0., wg = 0.;
+    double f = 0.;
     if (lane < 41) {
         double x;
-        if (lane < 20)
-            x = center - half_length * kXgk41[k];
-        else if (lane < 40)
-            x = center + half_length * kXgk41[k];
+        if (g.side < 0)
+            x = center - half_length * g.x;
+        else if (g.side > 0)
+            x = center + half_length * g.x;
         else
             x = center;
-        f = spline_exp(sp, x);
-        wk = lane < 40 ? kWgk41[k] : kWgk41[20];
-        wg = (lane < 40 && (k & 1)) ? kWg41[k >> 1] : 0.;
+        f = spline_exp_lds(sp, x);
     }
-    double result_kronrod = wave_sum(wk * f);
-    const double result_gauss = wave_sum(wg * f);
-    double result_abs = wave_sum(wk * fabs(f));
+    double result_kronrod = wave_sum(g.wk * f);
+    const double result_gauss = wave_sum(g.wg * f);
+    double result_abs = wave_sum(g.wk * fabs(f));
     const double mean = result_kronrod * 0.5;
-    double result_asc = wave_sum(lane < 41 ? wk * fabs(f - mean) : 0.);
+    double result_asc = wave_sum(lane < 41 ? g.wk * fabs(f - mean) : 0.);
     const double err = (result_kronrod - result_gauss) * half_length;
     result_kronrod *= half_length;
     result_abs *= abs_half_length;
@@ -1736,8 +1905,12 @@ __device__ inline void qk41_wave(const Spline &sp, double a, double b, double *r
     *abserr = rescale_error(err, result_abs, result_asc);
 }
 
-__device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, int limit,
-                          QagWs &w, double *result)
+// qag41_spline's QUADPACK sequence by the work-group.  Each bisection takes two barriers: one
+// after the halves' results are exchanged, one after lane 0 has updated the lists (qpsrt
+// included) and published the next interval; every thread reads the exchange before the
+// second and the lists after it, so no write can overtake a read of the previous bisection.
+__device__ int qag41_wave(const SplineLds &sp, const Gk41Node &g, double a, double b,
+                          double epsrel, int limit, QagWs &w, double *result)
 {
     double result0, abserr0, resabs0, resasc0;
     int roundoff_type1 = 0, roundoff_type2 = 0, error_type = 0;
@@ -1752,7 +1925,7 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
     }
     int size = 0, nrmax = 0, imax = 0;
     *result = 0;
-    qk41_wave(sp, a, b, &result0, &abserr0, &resabs0, &resasc0);
+    qk41_wave(sp, g, a, b, &result0, &abserr0, &resabs0, &resasc0);
     size = 1;
     if (writer) {
         w.rlist[0] = result0;
@@ -1772,6 +1945,8 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
     }
     double area = result0, errsum = abserr0;
     int iteration = 1;
+    __shared__ double xch[8];
+    __shared__ int sh_imax, sh_nrmax;
     __syncthreads();
     do {
         const double a_i = w.alist[imax], b_i = w.blist[imax];
@@ -1779,13 +1954,11 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
         const int lev = w.level[imax];
         const double a1 = a_i, b1 = 0.5 * (a_i + b_i), a2 = b1, b2 = b_i;
         double area1, area2, error1, error2, resasc1, resasc2;  // |f| integrals unused by qag
-        // the two halves at once: wave 0 integrates [a1, b1], wave 1 [a2, b2], each with
-        // its own node values; the results are exchanged through LDS
-        __shared__ double xch[8];
+        // the two halves at once: wave 0 integrates [a1, b1], wave 1 [a2, b2]
         {
             const int wv = threadIdx.x >> 6;
             double r, e, ra, rc;
-            qk41_wave(sp, wv ? a2 : a1, wv ? b2 : b1, &r, &e, &ra, &rc);
+            qk41_wave(sp, g, wv ? a2 : a1, wv ? b2 : b1, &r, &e, &ra, &rc);
             if ((threadIdx.x & 63) == 0) {
                 xch[4 * wv + 0] = r;
                 xch[4 * wv + 1] = e;
@@ -1815,9 +1988,10 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
             const double tmp = (1 + 100 * kDblEps) * (fabs(a2) + 1000 * kDblMin);
             if (fabs(a1) <= tmp && fabs(b2) <= tmp) error_type = 3;
         }
-        const int i_max = imax, i_new = size;
-        const int new_level = lev + 1;
+        size++;
         if (writer) {
+            const int i_max = imax, i_new = size - 1;
+            const int new_level = lev + 1;
             if (error2 > error1) {
                 w.alist[i_max] = a2;
                 w.rlist[i_max] = area2;
@@ -1839,21 +2013,13 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
                 w.elist[i_new] = error2;
                 w.level[i_new] = new_level;
             }
-        }
-        size++;
-        __syncthreads();
-        if (writer) qpsrt(w, size, limit, nrmax, imax);
-        __syncthreads();
-        // broadcast the list state lane 0 maintains
-        __shared__ int sh_imax, sh_nrmax;
-        if (writer) {
+            qpsrt(w, size, limit, nrmax, imax);
             sh_imax = imax;
             sh_nrmax = nrmax;
         }
         __syncthreads();
         imax = sh_imax;
         nrmax = sh_nrmax;
-        __syncthreads();
         iteration++;
     } while (iteration < limit && !error_type && errsum > tolerance);
     double sum = 0;
@@ -1866,95 +2032,71 @@ __device__ int qag41_wave(const Spline &sp, double a, double b, double epsrel, i
     return 5;
 }
 
-// marginalise_r.c:81-223 for the wave: (x, y) in LDS, y shifted in place.
-__device__ double marginalise_posterior_wave(int num, const double *x, double *y, QagWs &w,
-                                             Spline &sp)
+// likelihood.c:126-146 with marginalise_posterior_r (marginalise_r.c:81-223) for one item, by
+// the two waves of the work-group.  Lane r (1 <= r < numR) of each wave reads L_r[r]; the
+// points above kBad go, in r order, to knots 0..num-1 in LDS; ymax, xmin and xmax are wave
+// reductions (xmin and xmax are the extreme abscissae: the reference's start from the
+// neighbours of the maximum does not change them); the QUADPACK lists live in LDS.
+__device__ void rmarg_item(const LikLaunch &a, int item, SplineLds &sp, double *qd, int *qi)
 {
-    double ymax = -10.0;
-    int ymaxpos = 0;
-    for (int ii = 0; ii < num; ii++)
-        if (y[ii] > ymax) {
-            ymax = y[ii];
-            ymaxpos = ii;
-        }
-    double xmin = x[ymaxpos], xmax = x[ymaxpos];
-    if (ymaxpos < num - 1) xmax = x[ymaxpos + 1];
-    if (ymaxpos > 0) xmin = x[ymaxpos - 1];
-    for (int ii = 0; ii < num; ii++) {
-        if (x[ii] < xmin) xmin = x[ii];
-        if (x[ii] > xmax) xmax = x[ii];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int ii = 0; ii < num; ii++) y[ii] -= ymax;
-    __syncthreads();
-    if (xmax <= xmin) return NAN;
-    __shared__ int st_init;
-    __shared__ double spline_ws[7 * kMaxR];
-    if (threadIdx.x == 0) st_init = spline_init_ws(sp, x, y, num, 0, spline_ws);
-    __syncthreads();
-    if (st_init) return NAN;
-    double sump = 0.;
-    int st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
-    if (st) {
-        __syncthreads();
-        if (threadIdx.x == 0) spline_init_ws(sp, x, y, num, 1, spline_ws);
-        __syncthreads();
-        st = qag41_wave(sp, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
-    }
-    if (st) return kBad;
-    return log(sump) + ymax;
-}
-
-// likelihood.c:126-146 for one item: L_e from its L_r, by every thread of the two-wave
-// work-group; sp, x, y and the QUADPACK lists in LDS.
-__device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *x, double *y,
-                           double *qd, int *qi)
-{
-    const LikItem it = a.items[item];
-    const FacetDev &F = a.facets[it.slot];
-    const double *L_r = a.Lr + (size_t)item * kMaxR;
-    int num = 0;
-    int nRo = 1;
+    const int lane = threadIdx.x & 63;
     const int numR = a.rg.numr;
+    const bool mine = lane >= 1 && lane < numR;
+    const LikItem it = a.items[item];
+    const size_t base = (size_t)item * kMaxR;
+    const double lv = mine ? a.Lr[base + lane] : kBad;
     if (a.tasks) {  // tiled engine: an item with a task still iterating waits for the re-run
-        const int lane = threadIdx.x & 63;  // one task per lane (numR <= kMaxR <= 64)
-        const bool act = lane >= 1 && lane < numR && a.tasks[(size_t)item * kMaxR + lane].active;
+        const bool act = mine && a.tasks[base + lane].active;
         if (__any(act)) return;
     }
-    while (nRo < numR && (L_r[nRo] <= kBad)) nRo++;
-    if (nRo < numR) {
-        for (nRo = 1; nRo < numR; nRo++) {
-            const double lv = L_r[nRo];
-            if (lv > kBad) {
-                if (threadIdx.x == 0) {
-                    x[num] = a.rg.ro[nRo];
-                    y[num] = lv + F.rprior[nRo];
-                }
-                num++;
-            }
-        }
-    }
-    __syncthreads();
+    const bool ok = mine && lv > kBad;
+    const double xr = mine ? a.rg.ro[lane] : 0.;
+    const double yr = ok ? lv + a.facets[it.slot].rprior[lane] : 0.;
+    const uint64_t m = __ballot(ok);
+    const int num = __popcll(m);
     int error = 0;
     double L_e;
     if (num > 4) {
-        QagWs w;
-        w.alist = qd;
-        w.blist = qd + kQagLimit;
-        w.rlist = qd + 2 * kQagLimit;
-        w.elist = qd + 3 * kQagLimit;
-        w.order = qi;
-        w.level = qi + kQagLimit;
-        L_e = marginalise_posterior_wave(num, x, y, w, sp);
+        const double ymax = wave_extreme<true>(ok && yr > -10.0 ? yr : -10.0);
+        const double xmin = wave_extreme<false>(ok ? xr : INFINITY);
+        const double xmax = wave_extreme<true>(ok ? xr : -INFINITY);
+        const int pos = __popcll(m & ((1ull << lane) - 1ull));
+        if (threadIdx.x < 64 && ok) {
+            sp.s.x[pos] = xr;
+            sp.s.y[pos] = yr - ymax;
+        }
+        __syncthreads();
+        const double xp = lane < num ? sp.s.x[lane] : 0.;
+        const double yp = lane < num ? sp.s.y[lane] : 0.;
+        if (xmax <= xmin || spline_build_wave(sp, num, xp, yp)) {
+            L_e = NAN;
+        } else {
+            QagWs w;
+            w.alist = qd;
+            w.blist = qd + kQagLimit;
+            w.rlist = qd + 2 * kQagLimit;
+            w.elist = qd + 3 * kQagLimit;
+            w.order = qi;
+            w.level = qi + kQagLimit;
+            const Gk41Node g = gk41_node();
+            __syncthreads();
+            double sump = 0.;
+            int st = qag41_wave(sp, g, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
+            if (st) {  // linear interpolation (marginalise_r.c: the retry after a failure)
+                __syncthreads();
+                if (threadIdx.x == 0) sp.s.linear = 1;
+                __syncthreads();
+                st = qag41_wave(sp, g, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
+            }
+            L_e = st ? kBad : log(sump) + ymax;
+        }
     } else {
         error = 1;
         L_e = kBad;
     }
     if (L_e == kBad) error = 1;
-    if (a.tasks && it.prec == 1)
-        for (int r = 1; r < numR; r++)
-            if (a.tasks[(size_t)item * kMaxR + r].fragile) error |= kErrFragile;
+    if (a.tasks && it.prec == 1 && __any(mine && a.tasks[base + lane].fragile))
+        error |= kErrFragile;
     if (threadIdx.x == 0) {
         a.Le[item] = L_e;
         a.err[item] = error;
@@ -1963,20 +2105,19 @@ __device__ void rmarg_item(const LikLaunch &a, int item, Spline &sp, double *x, 
 
 __device__ void tile_status(const LikLaunch &a);
 
-// The QUADPACK interval lists live in LDS (40 KB): lane 0 updates them and every lane
-// reads them back after each bisection, a round trip through L2 when they were in global
-// memory.
+// The spline and the QUADPACK interval lists live in LDS (41 KB): lane 0 updates the lists
+// and every lane reads them back after each bisection, a round trip through L2 when they
+// were in global memory.
 __global__ __launch_bounds__(128) void rmarg_kernel(LikLaunch a)
 {
     if (blockIdx.x == (unsigned)a.nitems) {  // the status group (tiled engine)
         tile_status(a);
         return;
     }
-    __shared__ Spline sp;
-    __shared__ double x[kMaxR], y[kMaxR];
+    __shared__ SplineLds sp;
     __shared__ double qd[kQagDoubles];
     __shared__ int qi[kQagInts];
-    rmarg_item(a, blockIdx.x, sp, x, y, qd, qi);
+    rmarg_item(a, blockIdx.x, sp, qd, qi);
 }
 
 // ------------------------------------------------------------------ likelihood, tiled
