@@ -1742,18 +1742,17 @@ struct SplineLds {
     double b[kMaxR], d[kMaxR];
 };
 
+// The interval is found by counting the inner knots at or below x: with strictly increasing
+// knots (spline_build_wave checks them) that is the index cspline_eval's bisection finds, and
+// the reads are independent of each other rather than a chain of dependent LDS round trips.
 __device__ inline double spline_exp_lds(const SplineLds &t, double x)
 {
     const Spline &s = t.s;
-    if (x < s.x[0] || x > s.x[s.n - 1]) return 0.;
-    int lo = 0, hi = s.n - 1;
-    while (hi > lo + 1) {
-        const int i = (hi + lo) / 2;
-        if (s.x[i] > x)
-            hi = i;
-        else
-            lo = i;
-    }
+    const int n = s.n;
+    if (x < s.x[0] || x > s.x[n - 1]) return 0.;
+    int lo = 0;
+#pragma unroll 8
+    for (int j = 1; j < n - 1; j++) lo += s.x[j] <= x ? 1 : 0;
     const double x_lo = s.x[lo], x_hi = s.x[lo + 1];
     const double dx = x_hi - x_lo;
     if (!(dx > 0.0)) return 0.;
@@ -1800,30 +1799,41 @@ __device__ int spline_build_wave(SplineLds &t, int n, double xp, double yp)
         const double c1 = read_lane(g, 0) / read_lane(diag, 0);
         if (lane == 1) c = c1;
     } else if (sys > 1) {
-        double alp_l = 0., gam_l = 0.;  // lane i: alp_i, gam_i
+        // step i: alp_i from gam_{i-1}, the forward substitution's z_i, then gam_i =
+        // off_i / alp_i, whose division is the chain's latency; the next step's lane values
+        // are read before it
+        double alp_l = 0., gam_l = 0., z_l = 0.;  // lane i: alp_i, gam_i, z_i
         double al = read_lane(diag, 0);
-        double gm = read_lane(off, 0) / al;
+        double of = read_lane(off, 0);
+        double z = read_lane(g, 0);
+        double gm = of / al;
         if (al == 0) status = 1;
         if (lane == 0) {
             alp_l = al;
             gam_l = gm;
+            z_l = z;
         }
+        double dg = read_lane(diag, 1), gg = read_lane(g, 1), of_next = read_lane(off, 1);
         for (int i = 1; i < sys - 1; i++) {
-            al = read_lane(diag, i) - read_lane(off, i - 1) * gm;
-            gm = read_lane(off, i) / al;
+            al = dg - of * gm;
+            z = gg - gm * z;
+            of = of_next;
+            dg = read_lane(diag, i + 1);
+            gg = read_lane(g, i + 1);
+            of_next = read_lane(off, i + 1);
+            gm = of / al;
             if (al == 0) status = 1;
             if (lane == i) {
                 alp_l = al;
                 gam_l = gm;
+                z_l = z;
             }
         }
-        al = read_lane(diag, sys - 1) - read_lane(off, sys - 2) * gm;
-        if (lane == sys - 1) alp_l = al;
-        double z = read_lane(g, 0);
-        double z_l = lane == 0 ? z : 0.;
-        for (int i = 1; i < sys; i++) {
-            z = read_lane(g, i) - read_lane(gam_l, i - 1) * z;
-            if (lane == i) z_l = z;
+        al = dg - of * gm;  // alp_{sys-1}: no gam, and no status check, in spline_init_ws
+        z = gg - gm * z;
+        if (lane == sys - 1) {
+            alp_l = al;
+            z_l = z;
         }
         const double cc = lane < sys ? z_l / alp_l : 0.;
         double xs = read_lane(cc, sys - 1);
@@ -2042,13 +2052,20 @@ __device__ void rmarg_item(const LikLaunch &a, int item, SplineLds &sp, double *
     const int lane = threadIdx.x & 63;
     const int numR = a.rg.numr;
     const bool mine = lane >= 1 && lane < numR;
+    // every global read of the item is issued here, the node tables first
+    const Gk41Node g = gk41_node();
     const LikItem it = a.items[item];
     const size_t base = (size_t)item * kMaxR;
     const double lv = mine ? a.Lr[base + lane] : kBad;
-    if (a.tasks) {  // tiled engine: an item with a task still iterating waits for the re-run
-        const bool act = mine && a.tasks[base + lane].active;
-        if (__any(act)) return;
+    bool act = false, frag = false;
+    if (a.tasks && mine) {
+        act = a.tasks[base + lane].active;
+        frag = a.tasks[base + lane].fragile;
     }
+    if (__any(act)) return;  // tiled engine: an item with a task still iterating waits
+    // the node tables have arrived with the loads above: keep them from being re-read later
+    asm volatile("" ::"v"(g.x), "v"(g.wk), "v"(g.wg));
+    const bool fragile = it.prec == 1 && __any(frag);
     const bool ok = mine && lv > kBad;
     const double xr = mine ? a.rg.ro[lane] : 0.;
     const double yr = ok ? lv + a.facets[it.slot].rprior[lane] : 0.;
@@ -2078,7 +2095,6 @@ __device__ void rmarg_item(const LikLaunch &a, int item, SplineLds &sp, double *
             w.elist = qd + 3 * kQagLimit;
             w.order = qi;
             w.level = qi + kQagLimit;
-            const Gk41Node g = gk41_node();
             __syncthreads();
             double sump = 0.;
             int st = qag41_wave(sp, g, xmin, xmax, 1.e-3, kQagLimit, w, &sump);
@@ -2095,8 +2111,7 @@ __device__ void rmarg_item(const LikLaunch &a, int item, SplineLds &sp, double *
         L_e = kBad;
     }
     if (L_e == kBad) error = 1;
-    if (a.tasks && it.prec == 1 && __any(mine && a.tasks[base + lane].fragile))
-        error |= kErrFragile;
+    if (fragile) error |= kErrFragile;
     if (threadIdx.x == 0) {
         a.Le[item] = L_e;
         a.err[item] = error;
