@@ -2984,10 +2984,10 @@ __device__ void tile_status(const LikLaunch &a)
 
 // ------------------------------------------------------------------ Newton passes, fused
 // The step a fused pass's work-group takes before its sums (wave 0; the caller
-// synchronises): the task's chunk sums of the previous pass added in chunk order, the
-// Newton step, the state stored by the task's chunk-0 work-group (to the other
-// generation) with L_r when the task ends; *go and off[2] receive whether the task goes on
-// and its new offset.
+// synchronises), for a task that is still iterating (fused_idle has dealt with the others):
+// the task's chunk sums of the previous pass added in chunk order, the Newton step, the
+// state stored by the task's chunk-0 work-group (to the other generation) with L_r when the
+// task ends; *go and off[2] receive whether the task goes on and its new offset.
 __device__ __attribute__((always_inline)) inline void fused_step(const TileLaunch &t, unsigned first,
                                                                  unsigned nch,
                                   unsigned chunk, int item, int r, int *go_out, double *off)
@@ -3005,7 +3005,7 @@ __device__ __attribute__((always_inline)) inline void fused_step(const TileLaunc
             xo = T->xo;
             yo = T->yo;
         }
-    } else if (t.mode == 0 || T->active) {
+    } else {
         double v[7];
         task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch, t.nr_g, lane,
                   v);
@@ -3022,14 +3022,26 @@ __device__ __attribute__((always_inline)) inline void fused_step(const TileLaunc
             xo = S.xo;
             yo = S.yo;
         }
-    } else if (lane == 63 && chunk == 0) {
-        t.tasks_out[ti] = *T;  // a finished task keeps its state in both generations
     }
     if (lane == 63) {
         *go_out = go;
         off[0] = xo;
         off[1] = yo;
     }
+}
+
+// Whether a fused pass's work-group has a task that ended in an earlier pass (mode 1), in
+// which case the task's chunk-0 work-group keeps its state in both generations.  Every wave
+// reads the flag (the same address: a scalar load, whose wait does not include the cells'
+// vector loads already in flight), so an idle work-group leaves without a barrier, which
+// would wait for those loads.
+__device__ inline bool fused_idle(const TileLaunch &t, unsigned chunk, int item, int r)
+{
+    if (t.mode != 1) return false;
+    const size_t ti = (size_t)item * kMaxR + r;
+    if (t.tasks[ti].active) return false;
+    if (chunk == 0 && threadIdx.x == 63) t.tasks_out[ti] = t.tasks[ti];
+    return true;
 }
 
 // One Newton pass with the previous pass's step folded in, so that a pass is one launch
@@ -3075,6 +3087,7 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     // previous pass's chunk sums and takes the step while those loads are in flight.
     __shared__ TabRot<1024 / kTabRun> R;  // gf <= 1024
     auto make = [&]() -> bool {
+        if (fused_idle(t, chunk, item, r)) return false;
         constexpr int NC = 1024 / kTileBS;
         double c[NC];
 #pragma unroll
@@ -3505,6 +3518,7 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel
     const unsigned span = row_hi - row_lo + 1;
     const bool cached = span < kWideRows;  // entry span holds row 0
     auto make = [&]() -> bool {
+        if (fused_idle(t, chunk, item, r)) return false;
         fused_step(t, first, nch, chunk, item, r, &s_go, s_off);
         __syncthreads();
         if (!s_go) return false;
