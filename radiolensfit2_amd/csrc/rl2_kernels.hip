@@ -2455,7 +2455,8 @@ struct RecCells {
     double dr[kTabU], di[kTabU];
 };
 template <class Stage>
-__device__ inline bool tile_sums64_rec(const FacetDev &F, const LikItem &it, double ro,
+__device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const double2 *dc,
+                                       const LikItem &it, double ro,
                                        const Stage &stage, const char *rec, unsigned beg,
                                        unsigned end, double (&s)[7])
 {
@@ -2467,9 +2468,6 @@ __device__ inline bool tile_sums64_rec(const FacetDev &F, const LikItem &it, dou
     const double a1 = 1. + e1, a2 = 1. - e1;
 #pragma unroll
     for (int j = 0; j < 7; j++) s[j] = 0.;
-    const uint32_t *loff = F.loff;
-    const double2 *dc = F.dc;
-    const unsigned pad = F.n;
     auto fetch = [&](RecCells &b, unsigned base) {
 #pragma unroll
         for (int q = 0; q < kTabU; q++) {
@@ -2983,6 +2981,34 @@ __device__ void tile_status(const LikLaunch &a)
 }
 
 // ------------------------------------------------------------------ Newton passes, fused
+// What a fused pass's work-group reads of its task before anything else, so that these
+// loads are in flight with the facet's and the cells': the task's state S (the same address
+// in every thread: scalar loads) and wave 0's share of the previous pass's chunk sums of the
+// task (lane c: chunk c, when nch <= 64; task_sums reads them itself otherwise).
+struct FusedPre {
+    TileTask S;
+    double p[7];
+};
+
+__device__ __attribute__((always_inline)) inline FusedPre fused_pre(const TileLaunch &t,
+                                                                    unsigned first,
+                                                                    unsigned nch, int item,
+                                                                    int r)
+{
+    FusedPre f;
+    f.S = t.tasks[(size_t)item * kMaxR + r];
+#pragma unroll
+    for (int j = 0; j < 7; j++) f.p[j] = 0.;
+    const unsigned lane = threadIdx.x;
+    if (t.mode != 2 && nch <= 64 && lane < nch) {
+        const double *P = t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8 +
+                          (size_t)lane * t.nr_g * 8;
+#pragma unroll
+        for (int j = 0; j < 7; j++) f.p[j] = P[j];
+    }
+    return f;
+}
+
 // The step a fused pass's work-group takes before its sums (wave 0; the caller
 // synchronises), for a task that is still iterating (fused_idle has dealt with the others):
 // the task's chunk sums of the previous pass added in chunk order, the Newton step, the
@@ -2990,27 +3016,39 @@ __device__ void tile_status(const LikLaunch &a)
 // task ends; *go and off[2] receive whether the task goes on and its new offset.
 __device__ __attribute__((always_inline)) inline void fused_step(const TileLaunch &t, unsigned first,
                                                                  unsigned nch,
-                                  unsigned chunk, int item, int r, int *go_out, double *off)
+                                  unsigned chunk, int item, int r, const FusedPre &pre,
+                                  int *go_out, double *off)
 {
     if (threadIdx.x >= 64) return;
     const LikLaunch &a = t.lk;
     const size_t ti = (size_t)item * kMaxR + r;
     const int lane = threadIdx.x;
-    const TileTask *T = t.tasks + ti;
     int go = 0;
     double xo = 0., yo = 0.;
     if (t.mode == 2) {
         if (lane == 63) {
-            go = T->active;
-            xo = T->xo;
-            yo = T->yo;
+            go = pre.S.active;
+            xo = pre.S.xo;
+            yo = pre.S.yo;
         }
     } else {
         double v[7];
-        task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch, t.nr_g, lane,
-                  v);
+        if (nch <= 64) {  // task_sums' arithmetic on the chunk sums read at the start
+#pragma unroll
+            for (int j = 0; j < 7; j++) {
+                double pj = pre.p[j];
+                asm volatile("" : "+v"(pj));  // consumed here, not where the load was issued
+                v[j] = 0.;
+                v[j] += pj;
+            }
+#pragma unroll
+            for (int j = 0; j < 7; j++) v[j] = wave_sum_last(v[j]);
+        } else {
+            task_sums(t.part_in + (size_t)(first + (unsigned)(r - t.r_lo)) * 8, nch, t.nr_g,
+                      lane, v);
+        }
         if (lane == 63) {
-            TileTask S = t.mode == 0 ? TileTask{} : *T;
+            TileTask S = t.mode == 0 ? TileTask{} : pre.S;
             double ho, det;
             const int done = t.mode == 0 ? newton_update<0>(S, v, &ho, &det)
                                          : newton_update<1>(S, v, &ho, &det);
@@ -3032,15 +3070,14 @@ __device__ __attribute__((always_inline)) inline void fused_step(const TileLaunc
 
 // Whether a fused pass's work-group has a task that ended in an earlier pass (mode 1), in
 // which case the task's chunk-0 work-group keeps its state in both generations.  Every wave
-// reads the flag (the same address: a scalar load, whose wait does not include the cells'
-// vector loads already in flight), so an idle work-group leaves without a barrier, which
-// would wait for those loads.
-__device__ inline bool fused_idle(const TileLaunch &t, unsigned chunk, int item, int r)
+// holds the flag (read with scalar loads, whose wait does not include the cells' vector
+// loads in flight), so an idle work-group leaves without a barrier, which would wait for
+// those loads.
+__device__ inline bool fused_idle(const TileLaunch &t, unsigned chunk, int item, int r,
+                                  const FusedPre &pre)
 {
-    if (t.mode != 1) return false;
-    const size_t ti = (size_t)item * kMaxR + r;
-    if (t.tasks[ti].active) return false;
-    if (chunk == 0 && threadIdx.x == 63) t.tasks_out[ti] = t.tasks[ti];
+    if (t.mode != 1 || pre.S.active) return false;
+    if (chunk == 0 && threadIdx.x == 63) t.tasks_out[(size_t)item * kMaxR + r] = pre.S;
     return true;
 }
 
@@ -3072,34 +3109,45 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)t.nr_g;
     const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
-    const size_t ti = (size_t)item * kMaxR + r;
+    const FusedPre pre = fused_pre(t, first, nch, item, r);
     const LikItem it = items[item];
     const FacetDev &F = a.facets[it.slot];
-    const unsigned beg = chunk * it.chunk_cells;
-    const unsigned end = min(F.n, beg + it.chunk_cells);
+    const unsigned fn = F.n;
     const unsigned gf = F.grid_facet;  // <= 1024: the host checks the LDS tables fit
+    const uint32_t *cid = F.cid, *loff = F.loff;
+    const double *ctr = F.ctr;
+    const double2 *dc = F.dc;
+    // the facet's fields arrive with the task's state: one wait for both, before the
+    // cells' loads are issued
+    asm volatile("" ::"s"(fn), "s"(gf), "s"(cid), "s"(loff), "s"(ctr), "s"(dc));
+    const unsigned beg = chunk * it.chunk_cells;
+    const unsigned end = min(fn, beg + it.chunk_cells);
     extern __shared__ double2 tab_lds[];  // t.tab_lds_facet records of kRecBytes
     RecEntry *rec = reinterpret_cast<RecEntry *>(tab_lds);
-    const uint32_t *cid = F.cid;
-    const double *ctr = F.ctr;
     // Runs after the first cells' loads are issued (inside tile_sums64_tab): the cell
     // centres this thread's table entries need are requested too, then wave 0 adds the
     // previous pass's chunk sums and takes the step while those loads are in flight.
     __shared__ TabRot<1024 / kTabRun> R;  // gf <= 1024
     auto make = [&]() -> bool {
-        if (fused_idle(t, chunk, item, r)) return false;
+        if (fused_idle(t, chunk, item, r, pre)) return false;
         constexpr int NC = 1024 / kTileBS;
         double c[NC];
 #pragma unroll
-        for (int j = 0; j < NC; j++) {
+        for (int j = 0; j < NC; j++) {  // loads without branches: one count of them to wait on
             const unsigned k = threadIdx.x + j * kTileBS;
-            c[j] = k < gf ? gload(ctr, k) : 0.;
+            const double v = gload(ctr, k < gf ? k : 0u);
+            c[j] = k < gf ? v : 0.;
         }
-        const unsigned row_lo = end > beg ? gload(cid, beg) >> 16 : 0u;
-        const unsigned row_hi = end > beg ? gload(cid, end - 1) >> 16 : 0u;
-        fused_step(t, first, nch, chunk, item, r, &s_go, s_off);
+        uint32_t id_lo = end > beg ? gload(cid, beg) : 0u;
+        uint32_t id_hi = end > beg ? gload(cid, end - 1) : 0u;
+        fused_step(t, first, nch, chunk, item, r, pre, &s_go, s_off);
         __syncthreads();
         if (!s_go) return false;
+        // the chunk's first and last cell ids are waited for only here: a wait for them
+        // before the step would also wait for every cell and centre load issued before them
+        asm volatile("" : "+v"(id_lo), "+v"(id_hi));
+        const unsigned row_lo = __builtin_amdgcn_readfirstlane(id_lo >> 16);
+        const unsigned row_hi = __builtin_amdgcn_readfirstlane(id_hi >> 16);
         // the column runs, the runs of the rows the chunk spans (cells are in row-major
         // order) and run 0 (row 0: the zero pad cell's), then the entries
         const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
@@ -3126,8 +3174,8 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         return true;
     };
     double s[7];
-    if (tile_sums64_rec(F, it, a.rg.ro[r], make, reinterpret_cast<const char *>(rec), beg, end,
-                        s))
+    if (tile_sums64_rec(fn, loff, dc, it, a.rg.ro[r], make, reinterpret_cast<const char *>(rec),
+                        beg, end, s))
         tile_reduce<7>(s, red, t.partials + (size_t)wg * 8);
 }
 
@@ -3346,8 +3394,8 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_persist_kernel(Pe
                 const unsigned end = min(F.n, beg + it.chunk_cells);
                 double s[7];
                 auto none = []() { return true; };  // the records are already staged
-                tile_sums64_rec(F, it, ro, none, reinterpret_cast<const char *>(rec), beg, end,
-                                s);
+                tile_sums64_rec(F.n, F.loff, F.dc, it, ro, none,
+                                reinterpret_cast<const char *>(rec), beg, end, s);
                 tile_reduce_publish<7>(
                     s, red, p.part + ((size_t)pass * t.nwg + K.first + c * nr + (K.r - 1)) * 8,
                     p.cnt + ti);
@@ -3502,6 +3550,7 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)t.nr_g;
     const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
+    const FusedPre pre = fused_pre(t, first, nch, item, r);
     const LikItem it = items[item];
     const FacetDev &F = a.facets[it.slot];
     const unsigned beg = chunk * it.chunk_cells;
@@ -3518,8 +3567,8 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel
     const unsigned span = row_hi - row_lo + 1;
     const bool cached = span < kWideRows;  // entry span holds row 0
     auto make = [&]() -> bool {
-        if (fused_idle(t, chunk, item, r)) return false;
-        fused_step(t, first, nch, chunk, item, r, &s_go, s_off);
+        if (fused_idle(t, chunk, item, r, pre)) return false;
+        fused_step(t, first, nch, chunk, item, r, pre, &s_go, s_off);
         __syncthreads();
         if (!s_go) return false;
         const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
