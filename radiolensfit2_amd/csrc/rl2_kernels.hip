@@ -2449,6 +2449,11 @@ __device__ inline bool tile_sums64_tab(const FacetDev &F, const LikItem &it, dou
 struct RecEntry {
     double xr, xi, yr, yi, c;
 };
+// The cell centres as the records hold them (RecEntry::c), indexed like the centre array.
+struct RecCtr {
+    const RecEntry *rec;
+    __device__ double operator[](unsigned k) const { return rec[k].c; }
+};
 static_assert(sizeof(RecEntry) == kRecBytes, "record layout");
 struct RecCells {
     uint32_t o[kTabU];
@@ -2703,9 +2708,10 @@ __device__ inline double2 tab_entry(const double2 &e, const double2 &w)
 }
 // The runs' first entries (columns: every run of the gf columns; rows: runs yb0..yb1 and,
 // with y0, run 0) and both W tables, spread over the block's threads; the caller
-// synchronises before reading R.
-template <unsigned NRUN>
-__device__ inline void tab_rot_stage(TabRot<NRUN> &R, const double *ctr, unsigned gf, double xo,
+// synchronises before reading R.  ctr(k): the cell centre k, from global memory or from a
+// copy in LDS.
+template <unsigned NRUN, class Ctr>
+__device__ inline void tab_rot_stage(TabRot<NRUN> &R, const Ctr &ctr, unsigned gf, double xo,
                                      double yo, unsigned yb0, unsigned yb1, bool y0)
 {
     const unsigned nb = (gf + kTabRun - 1) / kTabRun;
@@ -3141,6 +3147,13 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         uint32_t id_lo = end > beg ? gload(cid, beg) : 0u;
         uint32_t id_hi = end > beg ? gload(cid, end - 1) : 0u;
         fused_step(t, first, nch, chunk, item, r, pre, &s_go, s_off);
+        // the records' centres do not depend on the step: stored while wave 0 takes it, and
+        // the tables read their runs' centres from them
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            const unsigned k = threadIdx.x + j * kTileBS;
+            if (k < gf) rec[k].c = c[j];
+        }
         __syncthreads();
         if (!s_go) return false;
         // the chunk's first and last cell ids are waited for only here: a wait for them
@@ -3151,7 +3164,7 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         // the column runs, the runs of the rows the chunk spans (cells are in row-major
         // order) and run 0 (row 0: the zero pad cell's), then the entries
         const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
-        tab_rot_stage(R, ctr, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
+        tab_rot_stage(R, RecCtr{rec}, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < NC; j++) {
@@ -3160,7 +3173,6 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
                 const double2 x = tab_x(R, k);
                 rec[k].xr = x.x;
                 rec[k].xi = x.y;
-                rec[k].c = c[j];
             }
         }
         const unsigned span = row_hi - row_lo + 1;
@@ -3445,6 +3457,11 @@ constexpr unsigned kWideRows = 256;
 struct ColEntry {
     double xr, xi, c;
 };
+// The cell centres as the column records hold them (ColEntry::c).
+struct ColCtr {
+    const ColEntry *col;
+    __device__ double operator[](unsigned k) const { return col[k].c; }
+};
 template <bool CACHED, class Stage>
 __device__ inline bool tile_sums64_col(const FacetDev &F, const LikItem &it, double ro,
                                        const Stage &stage, const ColEntry *col,
@@ -3569,16 +3586,20 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel
     auto make = [&]() -> bool {
         if (fused_idle(t, chunk, item, r, pre)) return false;
         fused_step(t, first, nch, chunk, item, r, pre, &s_go, s_off);
+        // the records' centres do not depend on the step: stored by waves 1.. while wave 0
+        // takes it, and the tables read their runs' centres from them
+        if (threadIdx.x >= 64)
+            for (unsigned k = threadIdx.x - 64; k < gf; k += kTileBS - 64)
+                col[k].c = gload(ctr, k);
         __syncthreads();
         if (!s_go) return false;
         const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
-        tab_rot_stage(R, ctr, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
+        tab_rot_stage(R, ColCtr{col}, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
         __syncthreads();
         for (unsigned k = threadIdx.x; k < gf; k += kTileBS) {
             const double2 x = tab_x(R, k);
             col[k].xr = x.x;
             col[k].xi = x.y;
-            col[k].c = gload(ctr, k);
         }
         if (cached)
             for (unsigned q = threadIdx.x; q <= span; q += kTileBS)
