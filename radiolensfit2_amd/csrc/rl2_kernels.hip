@@ -2202,6 +2202,9 @@ __device__ inline float4 gload(const float4 *p, size_t i)
 // their terms are exactly zero, and every thread adds its cells in increasing order as
 // before.
 constexpr int kTileU = 4;
+// Cells per thread and batch of the paths with two batches in flight (the Newton passes'
+// record path, pass 0 on extracted facets; 3 and 4 measured slower).
+constexpr int kRecU = 2;
 
 // Chunk sums in FP64.  PASS 0: {S2, value, dx, dy, dxx, dyy, dxy} of the unnormalised
 // model at zero offset (likelihood.c:214-236); PASS 1: {value, dx, dy, dxx, dyy, dxy} at
@@ -2316,6 +2319,88 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
             s[5] = fma(-hr * vv, vv, s[5]);
             s[6] = fma(-hU, vv, s[6]);
         }
+    }
+    s[2] *= kTwoPi;
+    s[3] *= kTwoPi;
+    constexpr double kTwoPi2 = kTwoPi * kTwoPi;
+    s[4] *= kTwoPi2;
+    s[5] *= kTwoPi2;
+    s[6] *= kTwoPi2;
+}
+
+// tile_sums64_p0_grid's sums with two batches of kRecU cells per thread in flight: the next
+// batch's loads are issued before the current batch is summed, the visibility pair comes from
+// the packed copy dc, and `stage` (the centres into sC, ending with a barrier) runs after the
+// first batch's loads are issued.  A thread's cells, and so its sums, are those of
+// tile_sums64_p0_grid in the same order (past the end: the zero pad cell, which adds zeros).
+template <class Stage>
+__device__ inline void tile_sums64_p0_db(const FacetDev &F, const LikItem &it, double ro,
+                                         const double *sC, const Stage &stage, unsigned beg,
+                                         unsigned end, double (&s)[7])
+{
+    const double e1 = it.e1, e2 = it.e2;
+    const double detA = 1. - e1 * e1 - e2 * e2;
+    const double scale = arcs2rad_scale(ro);
+    const double sf = (scale * scale) / (detA * detA);
+    const double sw = sf * (4 * kPi * kPi);
+    const double a1 = 1. + e1, a2 = 1. - e1;
+#pragma unroll
+    for (int j = 0; j < 7; j++) s[j] = 0.;
+    const uint32_t *cid = F.cid;
+    const double2 *dc = F.dc;
+    const double *w2 = F.invs2_64;
+    const unsigned pad = F.n;
+    struct Cells {
+        uint32_t c[kRecU];
+        double dr[kRecU], di[kRecU], s2[kRecU];
+    };
+    auto fetch = [&](Cells &b, unsigned base) {
+#pragma unroll
+        for (int q = 0; q < kRecU; q++) {
+            const unsigned i = base + q * kTileBS;
+            const unsigned k = i < end ? i : pad;
+            b.c[q] = gload(cid, k);
+            const double2 d = gload(dc, k);
+            b.dr[q] = d.x;
+            b.di[q] = d.y;
+            b.s2[q] = gload(w2, k);
+        }
+    };
+    auto eval = [&](const Cells &b) {
+#pragma unroll
+        for (int q = 0; q < kRecU; q++) {
+            const double uu = sC[b.c[q] & 0xffffu], vv = sC[b.c[q] >> 16];
+            const double k1 = fma(e2, vv, a1 * uu);
+            const double k2 = fma(a2, vv, e2 * uu);
+            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
+            const double M = rr * rr * rr;
+            const double hr = b.dr[q] * M;
+            const double hi = -b.di[q] * M;
+            s[0] = fma(M * M, b.s2[q], s[0]);
+            s[1] += hr;
+            s[2] = fma(-hi, uu, s[2]);
+            s[3] = fma(-hi, vv, s[3]);
+            const double hU = hr * uu;
+            s[4] = fma(-hU, uu, s[4]);
+            s[5] = fma(-hr * vv, vv, s[5]);
+            s[6] = fma(-hU, vv, s[6]);
+        }
+    };
+    constexpr unsigned S = kRecU * kTileBS;
+    Cells A, B;
+    fetch(A, beg + threadIdx.x);
+    stage();
+#pragma unroll
+    for (int q = 0; q < kRecU; q++)
+        asm volatile("" : "+v"(A.c[q]), "+v"(A.dr[q]), "+v"(A.di[q]), "+v"(A.s2[q]));
+    for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
+        fetch(B, base + S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(A);
+        if (base + S >= end) break;
+        fetch(A, base + 2 * S);
+        __builtin_amdgcn_sched_barrier(0);
+        eval(B);
     }
     s[2] *= kTwoPi;
     s[3] *= kTwoPi;
@@ -2456,8 +2541,8 @@ struct RecCtr {
 };
 static_assert(sizeof(RecEntry) == kRecBytes, "record layout");
 struct RecCells {
-    uint32_t o[kTabU];
-    double dr[kTabU], di[kTabU];
+    uint32_t o[kRecU];
+    double dr[kRecU], di[kRecU];
 };
 template <class Stage>
 __device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const double2 *dc,
@@ -2475,7 +2560,7 @@ __device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const
     for (int j = 0; j < 7; j++) s[j] = 0.;
     auto fetch = [&](RecCells &b, unsigned base) {
 #pragma unroll
-        for (int q = 0; q < kTabU; q++) {
+        for (int q = 0; q < kRecU; q++) {
             const unsigned i = base + q * kTileBS;
             const unsigned k = i < end ? i : pad;
             b.o[q] = gload(loff, k);
@@ -2485,10 +2570,10 @@ __device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const
         }
     };
     auto eval = [&](const RecCells &b) {
-        double uu[kTabU], vv[kTabU];
-        double2 x[kTabU], y[kTabU];
+        double uu[kRecU], vv[kRecU];
+        double2 x[kRecU], y[kRecU];
 #pragma unroll
-        for (int q = 0; q < kTabU; q++) {
+        for (int q = 0; q < kRecU; q++) {
             const RecEntry &col = *reinterpret_cast<const RecEntry *>(rec + (b.o[q] & 0xffffu));
             const RecEntry &row = *reinterpret_cast<const RecEntry *>(rec + (b.o[q] >> 16));
             uu[q] = col.c;
@@ -2497,7 +2582,7 @@ __device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const
             y[q] = make_double2(row.yr, row.yi);
         }
 #pragma unroll
-        for (int q = 0; q < kTabU; q++) {
+        for (int q = 0; q < kRecU; q++) {
             const double k1 = fma(e2, vv[q], a1 * uu[q]);
             const double k2 = fma(a2, vv[q], e2 * uu[q]);
             const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
@@ -2517,12 +2602,12 @@ __device__ inline bool tile_sums64_rec(unsigned pad, const uint32_t *loff, const
             s[5] = fma(-rU, vv[q], s[5]);
         }
     };
-    constexpr unsigned S = kTabU * kTileBS;
+    constexpr unsigned S = kRecU * kTileBS;
     RecCells A, B;
     fetch(A, beg + threadIdx.x);
     if (!stage()) return false;
 #pragma unroll
-    for (int q = 0; q < kTabU; q++) asm volatile("" : "+v"(A.o[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
+    for (int q = 0; q < kRecU; q++) asm volatile("" : "+v"(A.o[q]), "+v"(A.dr[q]), "+v"(A.di[q]));
     for (unsigned base = beg + threadIdx.x; base < end; base += 2 * S) {
         fetch(B, base + S);
         __builtin_amdgcn_sched_barrier(0);
@@ -2853,10 +2938,12 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
                     tile_sums64_tab<false>(F, it, a.rg.ro[r], copy, Y, sX, sY, sC, beg, end, s);
             } else {
                 double *sC = reinterpret_cast<double *>(tab_lds);  // tab_lds_facet * 8 bytes
-                for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS)
-                    sC[k] = gload(F.ctr, k);
-                __syncthreads();
-                tile_sums64_p0_grid<kTileU>(F, it, a.rg.ro[r], sC, beg, end, s);
+                auto stage = [&]() {
+                    for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS)
+                        sC[k] = gload(F.ctr, k);
+                    __syncthreads();
+                };
+                tile_sums64_p0_db(F, it, a.rg.ro[r], sC, stage, beg, end, s);
             }
         };
         grid_sums(a.facets[it.slot]);
