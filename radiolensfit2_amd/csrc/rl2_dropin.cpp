@@ -106,10 +106,18 @@ Obs *facet_context(size_t n_needed)
     return &g_fctx->o;
 }
 
-void set_grid(Obs &o, int numr, const double *ro)
+// The scalelength grid of a call: ro[0..numr-1] (ro[0] unused, as in the reference).  The
+// device buffers hold kMaxR points per request, so a larger grid is refused.
+bool set_grid(Obs &o, int numr, const double *ro)
 {
+    if (numr < 2 || numr > kMaxR || !ro) {
+        set_error("likelihood_params: numr must be in [2, " + std::to_string(kMaxR) +
+                  "] with ro given (got " + std::to_string(numr) + ")");
+        return false;
+    }
     o.rg.numr = numr;
     for (int k = 0; k < kMaxR; k++) o.rg.ro[k] = (k < numr) ? ro[k] : 0.;
+    return true;
 }
 
 // Facet slot holding par's facet: the resident extraction result when par points at it,
@@ -131,8 +139,7 @@ bool facet_for(const likelihood_params *par, Obs **out, int *slot)
         *slot = 0;
     }
     Obs &o = **out;
-    set_grid(o, par->numr, par->ro);
-    if (!o.ensure_slots(1)) return false;
+    if (!set_grid(o, par->numr, par->ro) || !o.ensure_slots(1)) return false;
     for (int k = 0; k < kMaxR; k++) o.slots[0].rprior[k] = (k < par->numr) ? par->rprior[k] : 0.;
     o.slots[0].n = n;
     o.facet_upload(0);
@@ -682,7 +689,10 @@ void data_processing(bool re_fitting, unsigned int *bad_list, int nsources, doub
         report("data_processing");
         return;
     }
-    set_grid(*o, par->numr, par->ro);
+    if (!set_grid(*o, par->numr, par->ro)) {
+        report("data_processing");
+        return;
+    }
     std::vector<unsigned> todo(nsources);
     unsigned maxg = 0;
     for (int g = 0; g < nsources; g++) {

@@ -345,3 +345,26 @@ def test_data_processing_writes_the_reference_lines(case):
     assert np.all(vsk == 0)
     assert np.abs(vd - ores_vis).max() <= 1e-10 * np.abs(ores_vis).max()
     assert fit_t.value > 0
+
+
+def test_loglikelihood_refuses_a_scalelength_grid_beyond_the_device_buffers(case):
+    """numr outside [2, 32] (the device holds 32 scalelength points per request) is refused
+    with an error instead of being indexed past the buffers; a valid call still works after."""
+    obs, cat, vs = case
+    H = N.hip()
+    fmax = H.facet_size(3.5, obs.len)
+    par = Par(obs, fmax)
+    facet, n, lp = _extract_dropin(obs, cat, 3, par, obs.vis, vs)
+    for bad in (40, 33, 1):
+        lp.numr = bad
+        err = C.c_int(-1)
+        L = H.loglikelihood(C.byref(lp), 0.1, -0.05, C.byref(err))
+        assert L == -1.0e10 and err.value == 1, (bad, L, err.value)
+        assert b"numr" in H.rl2_last_error()
+    lp.numr = par.numr
+    err = C.c_int(-1)
+    L = H.loglikelihood(C.byref(lp), 0.1, -0.05, C.byref(err))
+    op = par.oracle(facet, n)
+    oerr = C.c_int(-1)
+    oL = O.lib().orl2_loglikelihood(C.byref(op), 0.1, -0.05, C.byref(oerr))
+    assert err.value == oerr.value and abs(L - oL) <= 1e-8
