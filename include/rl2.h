@@ -169,6 +169,16 @@ int rl2_process(rl2_obs *obs, unsigned int ngal, const double *l, const double *
 int rl2_marginalise_posterior_r_noalloc(int num_marvals, const double *xmarvals,
                                         const double *ymarvals, double *out, void *stream);
 
+/* ---- verification entry point ----
+ * No counterpart in the reference: the step of likelihood.c:126-146 that turns a request's
+ * L_r into L_e (set_posterior_values, then marginalise_posterior_r when more than 4 points
+ * remain), run by the kernel the likelihood engine launches (rmarg_kernel), for nitems rows
+ * of L_r (kMaxR = 32 doubles each, L_r[0] unused) under one prior and scalelength grid
+ * (numr in 2..32).  Le, err: nitems each (err 1 as loglikelihood's error flag).  For tests
+ * and for callers that compute L_r themselves; synchronous. */
+int rl2_rmarg_eval(int nitems, int numr, const double *ro, const double *rprior,
+                   const double *Lr, double *Le, int *err);
+
 /* ---- scalelength grid and the fixed constants (RadioLensfit2.cpp:370-392) ---- */
 int rl2_r_grid(double *ro /* >= 29 */);
 

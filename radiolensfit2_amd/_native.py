@@ -131,6 +131,8 @@ _HIP_SIGS = {
     "rl2_r_grid": (C.c_int, [c_double_p]),
     "rl2_marginalise_posterior_r_noalloc": (C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                       C.c_void_p]),
+    "rl2_rmarg_eval": (C.c_int, [C.c_int, C.c_int, c_double_p, c_double_p, c_double_p,
+                                 c_double_p, C.POINTER(C.c_int)]),
     "rl2_stats_enable": (C.c_int, [C.c_int]),
     "rl2_stats_reset": (C.c_int, []),
     "rl2_stats_get": (C.c_int, [C.c_int, C.POINTER(KernelStat)]),

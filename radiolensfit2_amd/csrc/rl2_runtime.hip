@@ -1649,6 +1649,61 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
     return hip_ok(hipStreamSynchronize(g_stream), "load_facet") ? RL2_OK : RL2_ERR_HIP;
 }
 
+int rl2_rmarg_eval(int nitems, int numr, const double *ro, const double *rprior,
+                   const double *Lr, double *Le, int *err)
+{
+    if (nitems <= 0 || numr < 2 || numr > kMaxR || !ro || !rprior || !Lr || !Le || !err) {
+        set_error("rl2_rmarg_eval: nitems > 0, numr in [2, 32] and every array required");
+        return RL2_ERR_ARG;
+    }
+    if (!ensure_init()) return RL2_ERR_NO_DEVICE;
+    FacetDev f;
+    memset(&f, 0, sizeof f);
+    for (int k = 0; k < kMaxR; k++) f.rprior[k] = k < numr ? rprior[k] : 0.;
+    std::vector<LikItem> items(nitems);
+    for (auto &it : items) {
+        memset(&it, 0, sizeof it);
+        it.slot = 0;
+    }
+    LikItem *d_items = nullptr;
+    FacetDev *d_f = nullptr;
+    double *d_Lr = nullptr, *d_Le = nullptr;
+    int *d_err = nullptr;
+    const size_t nlr = (size_t)nitems * kMaxR;
+    bool ok = dmalloc(&d_items, nitems) && dmalloc(&d_f, 1) && dmalloc(&d_Lr, nlr) &&
+              dmalloc(&d_Le, nitems) && dmalloc(&d_err, nitems) &&
+              hip_ok(hipMemcpyAsync(d_items, items.data(), nitems * sizeof(LikItem),
+                                    hipMemcpyHostToDevice, g_stream), "rmarg items") &&
+              hip_ok(hipMemcpyAsync(d_f, &f, sizeof f, hipMemcpyHostToDevice, g_stream),
+                     "rmarg prior") &&
+              hip_ok(hipMemcpyAsync(d_Lr, Lr, nlr * sizeof(double), hipMemcpyHostToDevice,
+                                    g_stream), "rmarg L_r");
+    if (ok) {
+        LikLaunch a{};
+        a.nitems = nitems;
+        a.items = d_items;
+        a.facets = d_f;
+        a.rg.numr = numr;
+        for (int k = 0; k < kMaxR; k++) a.rg.ro[k] = k < numr ? ro[k] : 0.;
+        a.Lr = d_Lr;
+        a.Le = d_Le;
+        a.err = d_err;
+        launch_rmarg(g_stream, a);
+        ok = hip_ok(hipGetLastError(), "rmarg launch") &&
+             hip_ok(hipMemcpyAsync(Le, d_Le, nitems * sizeof(double), hipMemcpyDeviceToHost,
+                                   g_stream), "rmarg L_e") &&
+             hip_ok(hipMemcpyAsync(err, d_err, nitems * sizeof(int), hipMemcpyDeviceToHost,
+                                   g_stream), "rmarg err") &&
+             hip_ok(hipStreamSynchronize(g_stream), "rmarg sync");
+    }
+    dfree(d_items);
+    dfree(d_f);
+    dfree(d_Lr);
+    dfree(d_Le);
+    dfree(d_err);
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
 int rl2_obs_set_prior(rl2_obs *obs, int slot, const double *rprior)
 {
     if (!obs || slot < 0 || !rprior) return RL2_ERR_ARG;
