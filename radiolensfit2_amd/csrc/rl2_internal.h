@@ -325,6 +325,8 @@ struct TileLaunch {
     const double *part_in = nullptr;
     int mode = 0;              // fused passes: 0 first step (from the pass-0 sums), 1 Newton
                                // step, 2 resume from the state in `tasks`
+    unsigned item_chunks = 0;  // chunks of every item when all items have the same count
+                               // (0: they differ; the kernels then search chunk_first)
 };
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass);
 // One Newton pass with the previous pass's step folded in (grid items, row tables in LDS).

@@ -2156,11 +2156,27 @@ __device__ inline int tile_item(const LikItem *items, int nitems, unsigned wg, i
 __device__ inline void tile_range(const TileLaunch &t, const LikItem *items, int item,
                                   unsigned *first, unsigned *nch)
 {
+    if (t.item_chunks) {  // every item of the launch has item_chunks chunks
+        *first = (unsigned)item * t.item_chunks * (unsigned)t.nr_g;
+        *nch = t.item_chunks;
+        return;
+    }
     const unsigned c0 = items[item].chunk_first;
     const unsigned c1 = (item + 1 < t.lk.nitems) ? items[item + 1].chunk_first
                                                   : t.nwg / (unsigned)t.nr_g;
     *first = c0 * (unsigned)t.nr_g;
     *nch = c1 - c0;
+}
+
+// The item, first work-group and chunk count of logical work-group wg: a division when every
+// item of the launch has the same chunk count (one galaxy's points: the serial loop), else
+// the search over the items' first chunks.
+__device__ inline void tile_locate(const TileLaunch &t, const LikItem *items, unsigned wg,
+                                   int *item, unsigned *first, unsigned *nch)
+{
+    *item = t.item_chunks ? (int)(wg / (t.item_chunks * (unsigned)t.nr_g))
+                          : tile_item(items, t.lk.nitems, wg, t.nr_g);
+    tile_range(t, items, *item, first, nch);
 }
 
 // Work-groups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8) and each XCD
@@ -2898,9 +2914,9 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
     // there whenever they travel there: its block 0 is what stores them to a.items.
     const LikItem *items =
         KIND == 2 ? t.inl : ((PASS == 0 && t.n_inl) ? t.inl : a.items);
-    const int item = tile_item(items, a.nitems, wg, t.nr_g);
+    int item;
     unsigned first, nch;
-    tile_range(t, items, item, &first, &nch);
+    tile_locate(t, items, wg, &item, &first, &nch);
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)t.nr_g;
     const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
@@ -3196,9 +3212,9 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     if (wg >= t.nwg) return;
     const LikLaunch &a = t.lk;
     const LikItem *items = KIND == 2 ? t.inl : a.items;
-    const int item = tile_item(items, a.nitems, wg, t.nr_g);
+    int item;
     unsigned first, nch;
-    tile_range(t, items, item, &first, &nch);
+    tile_locate(t, items, wg, &item, &first, &nch);
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)t.nr_g;
     const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;
@@ -3648,9 +3664,9 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_wide_kernel
     if (wg >= t.nwg) return;
     const LikLaunch &a = t.lk;
     const LikItem *items = KIND == 2 ? t.inl : a.items;
-    const int item = tile_item(items, a.nitems, wg, t.nr_g);
+    int item;
     unsigned first, nch;
-    tile_range(t, items, item, &first, &nch);
+    tile_locate(t, items, wg, &item, &first, &nch);
     const unsigned local = wg - first;
     const unsigned chunk = local / (unsigned)t.nr_g;
     const int r = (int)(local - chunk * (unsigned)t.nr_g) + t.r_lo;

@@ -774,6 +774,12 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         }
         const unsigned total = chunks * (unsigned)nr;
         if (!b.ensure_partials(2 * (size_t)total)) return false;  // two generations
+        // one chunk count for every item (the points of one galaxy): the kernels then find a
+        // work-group's item by a division instead of a search
+        unsigned item_chunks = n > 0 ? chunks / (unsigned)n : 0u;
+        for (int i = 0; i < n && item_chunks; i++)
+            if (b.h_items[i].chunk_first != (unsigned)i * item_chunks) item_chunks = 0;
+        if (item_chunks * (unsigned)n != chunks) item_chunks = 0;
         // phase tables for FP64 items on grid facets: 2 * grid_facet entries per task
         size_t ntab = 0;
         unsigned tab_facet = 0;
@@ -814,6 +820,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             // with the items in the kernel arguments
             t.grid = (ntab && ntab_items == n) ? (n <= kInlItems ? 2u : 1u) : 0u;
             t.items_w = b.d_items;
+            t.item_chunks = item_chunks;
             t.n_inl = n <= kInlItems ? (unsigned)n : 0u;
             for (unsigned i = 0; i < t.n_inl; i++) t.inl[i] = b.h_items[i];
         }
