@@ -2819,19 +2819,29 @@ __device__ inline void tab_rot_stage(TabRot<NRUN> &R, const Ctr &ctr, unsigned g
     const unsigned ny = yb1 - yb0 + 1 + (y0 ? 1u : 0u);
     const unsigned nw = min(gf, kTabRun);
     const double c0 = ctr[0];
+    static_assert(sizeof(TabRot<NRUN>) == (2 * NRUN + 2 * kTabRun) * sizeof(double2),
+                  "ex, ey, wx, wy contiguous");
+    double2 *E = reinterpret_cast<double2 *>(&R);  // ex, ey, wx, wy in that order
     for (unsigned q = threadIdx.x; q < nb + ny + 2 * nw; q += blockDim.x) {
+        // the entry's argument and slot first, then one sin/cos whatever the kind of entry:
+        // with a sin/cos in each branch, a wave holding several kinds ran them one after
+        // the other
+        double arg;
+        unsigned slot;
         if (q < nb) {
-            R.ex[q] = cis_rev(xo * ctr[q * kTabRun]);
+            arg = xo * ctr[q * kTabRun];
+            slot = q;
         } else if (q < nb + ny) {
             const unsigned b = q - nb <= yb1 - yb0 ? yb0 + (q - nb) : 0u;
-            R.ey[b] = cis_rev(yo * ctr[b * kTabRun]);
+            arg = yo * ctr[b * kTabRun];
+            slot = NRUN + b;
         } else {
-            const unsigned w = q - nb - ny, j = w % nw;
-            if (w < nw)
-                R.wx[j] = cis_rev(xo * (ctr[j] - c0));
-            else
-                R.wy[j] = cis_rev(yo * (ctr[j] - c0));
+            const unsigned w = q - nb - ny;  // < 2 nw
+            const unsigned j = w < nw ? w : w - nw;
+            arg = (w < nw ? xo : yo) * (ctr[j] - c0);
+            slot = 2 * NRUN + (w < nw ? 0u : kTabRun) + j;
         }
+        E[slot] = cis_rev(arg);
     }
 }
 template <unsigned NRUN>
@@ -3233,9 +3243,9 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
     const unsigned end = min(fn, beg + it.chunk_cells);
     extern __shared__ double2 tab_lds[];  // t.tab_lds_facet records of kRecBytes
     RecEntry *rec = reinterpret_cast<RecEntry *>(tab_lds);
-    // Runs after the first cells' loads are issued (inside tile_sums64_tab): the cell
-    // centres this thread's table entries need are requested too, then wave 0 adds the
-    // previous pass's chunk sums and takes the step while those loads are in flight.
+    // Runs after the first cells' loads are issued (inside tile_sums64_rec): the cell
+    // centres this thread's records need are requested too, then wave 0 adds the previous
+    // pass's chunk sums and takes the step while those loads are in flight.
     __shared__ TabRot<1024 / kTabRun> R;  // gf <= 1024
     auto make = [&]() -> bool {
         if (fused_idle(t, chunk, item, r, pre)) return false;
