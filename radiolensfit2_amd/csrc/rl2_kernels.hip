@@ -2823,25 +2823,20 @@ __device__ inline void tab_rot_stage(TabRot<NRUN> &R, const Ctr &ctr, unsigned g
                   "ex, ey, wx, wy contiguous");
     double2 *E = reinterpret_cast<double2 *>(&R);  // ex, ey, wx, wy in that order
     for (unsigned q = threadIdx.x; q < nb + ny + 2 * nw; q += blockDim.x) {
-        // the entry's argument and slot first, then one sin/cos whatever the kind of entry:
-        // with a sin/cos in each branch, a wave holding several kinds ran them one after
-        // the other
-        double arg;
-        unsigned slot;
-        if (q < nb) {
-            arg = xo * ctr[q * kTabRun];
-            slot = q;
-        } else if (q < nb + ny) {
-            const unsigned b = q - nb <= yb1 - yb0 ? yb0 + (q - nb) : 0u;
-            arg = yo * ctr[b * kTabRun];
-            slot = NRUN + b;
-        } else {
-            const unsigned w = q - nb - ny;  // < 2 nw
-            const unsigned j = w < nw ? w : w - nw;
-            arg = (w < nw ? xo : yo) * (ctr[j] - c0);
-            slot = 2 * NRUN + (w < nw ? 0u : kTabRun) + j;
-        }
-        E[slot] = cis_rev(arg);
+        // the entry's centre index, factor and slot by selects, then one centre read and one
+        // sin/cos whatever the kind of entry: with them in branches, a wave holding several
+        // kinds ran each branch's read and sin/cos one after the other
+        const bool isx = q < nb, isy = !isx && q < nb + ny;
+        const unsigned b = q - nb <= yb1 - yb0 ? yb0 + (q - nb) : 0u;  // a row run (isy)
+        const unsigned w = q - nb - ny;                                  // < 2 nw (W entries)
+        const unsigned j = w < nw ? w : w - nw;
+        const unsigned ci = isx ? q * kTabRun : (isy ? b * kTabRun : j);
+        const double o = isx ? xo : (isy ? yo : (w < nw ? xo : yo));
+        const unsigned slot =
+            isx ? q : (isy ? NRUN + b : 2 * NRUN + (w < nw ? 0u : kTabRun) + j);
+        const double c = ctr[ci];
+        const double run = o * c, off = o * (c - c0);  // a run's first entry; a W entry
+        E[slot] = cis_rev(isx || isy ? run : off);
     }
 }
 template <unsigned NRUN>
@@ -3279,11 +3274,20 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_fused_kernel(Tile
         const unsigned yb0 = row_lo / kTabRun, yb1 = row_hi / kTabRun;
         tab_rot_stage(R, RecCtr{rec}, gf, s_off[0], s_off[1], yb0, yb1, yb0 > 0);
         __syncthreads();
+        // the column entries: every run and W value read before any is used
+        double2 ex[NC], wx[NC];
+#pragma unroll
+        for (int j = 0; j < NC; j++) {
+            const unsigned k = threadIdx.x + j * kTileBS;
+            const unsigned kk = k < gf ? k : 0u;
+            ex[j] = R.ex[kk / kTabRun];
+            wx[j] = R.wx[kk % kTabRun];
+        }
 #pragma unroll
         for (int j = 0; j < NC; j++) {
             const unsigned k = threadIdx.x + j * kTileBS;
             if (k < gf) {
-                const double2 x = tab_x(R, k);
+                const double2 x = tab_entry(ex[j], wx[j]);
                 rec[k].xr = x.x;
                 rec[k].xi = x.y;
             }
