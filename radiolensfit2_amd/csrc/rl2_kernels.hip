@@ -2349,19 +2349,29 @@ __device__ inline void tile_sums64_p0_grid(const FacetDev &F, const LikItem &it,
 // the packed copy dc, and `stage` (the centres into sC, ending with a barrier) runs after the
 // first batch's loads are issued.  A thread's cells, and so its sums, are those of
 // tile_sums64_p0_grid in the same order (past the end: the zero pad cell, which adds zeros).
-template <class Stage>
-__device__ inline void tile_sums64_p0_db(const FacetDev &F, const LikItem &it, double ro,
-                                         const double *sC, const Stage &stage, unsigned beg,
-                                         unsigned end, double (&s)[7])
+// NT scalelength tasks of the same item and chunk at once (ro[t], s[t]): the cell's loads,
+// centres and the shape terms k1, k2 and k1^2 + k2^2 are shared, the rest is each task's own,
+// with the expressions of the one-task form, so every task's sums are the same bit for bit.
+template <int NT, class Stage>
+__device__ inline void tile_sums64_p0_db(const FacetDev &F, const LikItem &it,
+                                         const double (&ro)[NT], const double *sC,
+                                         const Stage &stage, unsigned beg, unsigned end,
+                                         double (&s)[NT][7])
 {
     const double e1 = it.e1, e2 = it.e2;
     const double detA = 1. - e1 * e1 - e2 * e2;
-    const double scale = arcs2rad_scale(ro);
-    const double sf = (scale * scale) / (detA * detA);
-    const double sw = sf * (4 * kPi * kPi);
+    double sw[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const double scale = arcs2rad_scale(ro[t]);
+        const double sf = (scale * scale) / (detA * detA);
+        sw[t] = sf * (4 * kPi * kPi);
+    }
     const double a1 = 1. + e1, a2 = 1. - e1;
 #pragma unroll
-    for (int j = 0; j < 7; j++) s[j] = 0.;
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int j = 0; j < 7; j++) s[t][j] = 0.;
     const uint32_t *cid = F.cid;
     const double2 *dc = F.dc;
     const double *w2 = F.invs2_64;
@@ -2388,18 +2398,22 @@ __device__ inline void tile_sums64_p0_db(const FacetDev &F, const LikItem &it, d
             const double uu = sC[b.c[q] & 0xffffu], vv = sC[b.c[q] >> 16];
             const double k1 = fma(e2, vv, a1 * uu);
             const double k2 = fma(a2, vv, e2 * uu);
-            const double rr = rsqrt_ge1(fma(sw, fma(k1, k1, k2 * k2), 1.));
-            const double M = rr * rr * rr;
-            const double hr = b.dr[q] * M;
-            const double hi = -b.di[q] * M;
-            s[0] = fma(M * M, b.s2[q], s[0]);
-            s[1] += hr;
-            s[2] = fma(-hi, uu, s[2]);
-            s[3] = fma(-hi, vv, s[3]);
-            const double hU = hr * uu;
-            s[4] = fma(-hU, uu, s[4]);
-            s[5] = fma(-hr * vv, vv, s[5]);
-            s[6] = fma(-hU, vv, s[6]);
+            const double kk = fma(k1, k1, k2 * k2);
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const double rr = rsqrt_ge1(fma(sw[t], kk, 1.));
+                const double M = rr * rr * rr;
+                const double hr = b.dr[q] * M;
+                const double hi = -b.di[q] * M;
+                s[t][0] = fma(M * M, b.s2[q], s[t][0]);
+                s[t][1] += hr;
+                s[t][2] = fma(-hi, uu, s[t][2]);
+                s[t][3] = fma(-hi, vv, s[t][3]);
+                const double hU = hr * uu;
+                s[t][4] = fma(-hU, uu, s[t][4]);
+                s[t][5] = fma(-hr * vv, vv, s[t][5]);
+                s[t][6] = fma(-hU, vv, s[t][6]);
+            }
         }
     };
     constexpr unsigned S = kRecU * kTileBS;
@@ -2418,12 +2432,15 @@ __device__ inline void tile_sums64_p0_db(const FacetDev &F, const LikItem &it, d
         __builtin_amdgcn_sched_barrier(0);
         eval(B);
     }
-    s[2] *= kTwoPi;
-    s[3] *= kTwoPi;
     constexpr double kTwoPi2 = kTwoPi * kTwoPi;
-    s[4] *= kTwoPi2;
-    s[5] *= kTwoPi2;
-    s[6] *= kTwoPi2;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        s[t][2] *= kTwoPi;
+        s[t][3] *= kTwoPi;
+        s[t][4] *= kTwoPi2;
+        s[t][5] *= kTwoPi2;
+        s[t][6] *= kTwoPi2;
+    }
 }
 
 // PASS 1 sums of tile_sums64 on a facet with grid structure.  The Newton phase is
@@ -2964,7 +2981,9 @@ __device__ inline void tile_chunk(const TileLaunch &t, unsigned wg, double *red)
                         sC[k] = gload(F.ctr, k);
                     __syncthreads();
                 };
-                tile_sums64_p0_db(F, it, a.rg.ro[r], sC, stage, beg, end, s);
+                const double ro1[1] = {a.rg.ro[r]};
+                tile_sums64_p0_db<1>(F, it, ro1, sC, stage, beg, end,
+                                     reinterpret_cast<double(&)[1][7]>(s));
             }
         };
         grid_sums(a.facets[it.slot]);
@@ -3752,12 +3771,78 @@ void launch_tile_fused(hipStream_t s, const TileLaunch &t)
         lik_fused_kernel<1><<<g, kTileBS, lds, s>>>(t);
 }
 
+// Pass 0 on extracted facets with two scalelength tasks per work-group (tasks r_lo + 2g and
+// r_lo + 2g + 1 of one item and chunk; the last group of an odd task count has one): the two
+// share each cell's loads, centres and shape terms (tile_sums64_p0_db<2>), and each task's
+// chunk sums go where its own work-group of lik_tile_kernel<0, KIND> puts them, with the same
+// values bit for bit.  Logical work-group wg2 = chunk * groups + g (chunk counted over the
+// launch's items), in the XCD-aware order of lik_tile_kernel.
+template <int KIND, bool TWO>
+__device__ inline void p0_pair_chunk(const TileLaunch &t, const LikItem *items, int item,
+                                     unsigned cg, unsigned chunk, int ra, double *red)
+{
+    const LikLaunch &a = t.lk;
+    const LikItem it = items[item];
+    const FacetDev &F = a.facets[it.slot];
+    const unsigned beg = chunk * it.chunk_cells;
+    const unsigned end = min(F.n, beg + it.chunk_cells);
+    extern __shared__ double2 tab_lds[];
+    double *sC = reinterpret_cast<double *>(tab_lds);  // tab_lds_facet * 8 bytes
+    auto stage = [&]() {
+        for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
+        __syncthreads();
+    };
+    constexpr int NT = TWO ? 2 : 1;
+    double ro[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) ro[q] = a.rg.ro[ra + q];
+    double s[NT][7];
+    tile_sums64_p0_db<NT>(F, it, ro, sC, stage, beg, end, s);
+    const size_t w0 = (size_t)cg * (unsigned)t.nr_g + (unsigned)(ra - t.r_lo);
+#pragma unroll
+    for (int q = 0; q < NT; q++) tile_reduce<7>(s[q], red, t.partials + (w0 + q) * 8);
+}
+
+template <int KIND>
+__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_p0_pair_kernel(TileLaunch t)
+{
+    __shared__ double red[(kTileBS / 64) * 8];
+    if (KIND == 2 && blockIdx.x == 0 && threadIdx.x < t.n_inl)
+        t.items_w[threadIdx.x] = t.inl[threadIdx.x];
+    const LikItem *items = KIND == 2 ? t.inl : t.lk.items;
+    const unsigned nr = (unsigned)t.nr_g, groups = (nr + 1) / 2;
+    const unsigned nwg2 = t.nwg / nr * groups;
+    const unsigned per = (nwg2 + kXcds - 1) / kXcds;
+    const unsigned nbx = gridDim.x / kXcds;
+    for (unsigned j = blockIdx.x / kXcds; j < per; j += nbx) {
+        const unsigned wg2 = (blockIdx.x % kXcds) * per + j;
+        if (wg2 >= nwg2) break;
+        const unsigned cg = wg2 / groups, g = wg2 - cg * groups;
+        const int item = t.item_chunks ? (int)(cg / t.item_chunks)
+                                       : tile_item(items, t.lk.nitems, cg * nr, (int)nr);
+        const unsigned c0 = t.item_chunks ? (unsigned)item * t.item_chunks
+                                          : items[item].chunk_first;
+        const int ra = t.r_lo + 2 * (int)g;
+        if (2 * g + 1 < nr)
+            p0_pair_chunk<KIND, true>(t, items, item, cg, cg - c0, ra, red);
+        else
+            p0_pair_chunk<KIND, false>(t, items, item, cg, cg - c0, ra, red);
+    }
+}
+
 void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
     if (pass == 0) {
         const unsigned lds = t.tab_lds_facet * 8;
-        if (t.grid == 2)
+        // the grid kernels' pass 0 by pairs of tasks (RL2_TUNE_P0_PAIRS 1, the default)
+        const unsigned g2 =
+            tile_grid(t.nwg / (unsigned)t.nr_g * (((unsigned)t.nr_g + 1) / 2), tuning().tile_blocks);
+        if (t.grid == 2 && tuning().p0_pairs)
+            lik_p0_pair_kernel<2><<<g2, kTileBS, lds, s>>>(t);
+        else if (t.grid == 1 && tuning().p0_pairs)
+            lik_p0_pair_kernel<1><<<g2, kTileBS, lds, s>>>(t);
+        else if (t.grid == 2)
             lik_tile_kernel<0, 2><<<g, kTileBS, lds, s>>>(t);
         else if (t.grid == 1)
             lik_tile_kernel<0, 1><<<g, kTileBS, lds, s>>>(t);
