@@ -244,14 +244,14 @@ int rl2_synchronize(void);
  *   (1..50); 0 (default): the previous evaluation's largest step count + 1.  Passes beyond a
  *   task's last step leave it at once, and missing ones are queued after the check, so the
  *   results do not depend on it.
- * RL2_TUNE_P0_PAIRS: 1 (default) runs the zero-offset pass of FP64 items on extracted facets
- *   with two scalelength tasks per work-group (they share each cell's loads and shape
- *   terms); 0 with one.  Identical results. */
+ * RL2_TUNE_P0_TASKS: scalelength tasks per work-group (1..3, default 3) in the zero-offset
+ *   pass of FP64 items on extracted facets; the tasks of a work-group share each cell's
+ *   loads and shape terms.  Identical results. */
 enum { RL2_TUNE_GRID_VARIANT = 0, RL2_TUNE_TILE_CELLS = 1, RL2_TUNE_PHASE_TABLES = 2,
        RL2_TUNE_FUSE_SUBTRACT = 3, RL2_TUNE_TILE_BLOCKS = 4, RL2_TUNE_TILE_STREAMS = 5,
        RL2_TUNE_BATCH_EXTRACT = 6, RL2_TUNE_BATCH_GROUPS = 7, RL2_TUNE_BATCH_GRAPH = 8,
        RL2_TUNE_ITEM_GROUPS = 9, RL2_TUNE_FUSE_STEP = 10, RL2_TUNE_TILE_PASSES = 11,
-       RL2_TUNE_P0_PAIRS = 12 };
+       RL2_TUNE_P0_TASKS = 12 };
 int rl2_tune_set(int knob, int value);
 int rl2_tune_get(int knob, int *value);
 

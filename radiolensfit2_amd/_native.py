@@ -38,7 +38,7 @@ RL2_TUNE_BATCH_GRAPH = 8
 RL2_TUNE_ITEM_GROUPS = 9
 RL2_TUNE_FUSE_STEP = 10
 RL2_TUNE_TILE_PASSES = 11
-RL2_TUNE_P0_PAIRS = 12
+RL2_TUNE_P0_TASKS = 12
 (RL2_K_EXTRACT, RL2_K_COMPACT, RL2_K_LIKELIHOOD, RL2_K_RMARG, RL2_K_SUBTRACT,
  RL2_K_SOURCES, RL2_K_STEP) = range(7)
 KERNEL_NAMES = ["extract", "compact", "likelihood", "rmarg", "subtract", "sources", "step"]

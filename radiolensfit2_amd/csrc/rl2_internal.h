@@ -214,8 +214,8 @@ struct Tuning {
                                // (1) or a step kernel after every pass (0)
     int tile_passes = 0;       // RL2_TUNE_TILE_PASSES: Newton passes queued before the first
                                // status check (0: the previous evaluation's largest count + 1)
-    int p0_pairs = 1;          // RL2_TUNE_P0_PAIRS: pass 0 of the grid kernels by pairs of
-                               // tasks per work-group (1) or one task per work-group (0)
+    int p0_tasks = 3;          // RL2_TUNE_P0_TASKS: scalelength tasks per work-group in pass
+                               // 0 of the grid kernels (1..3)
 };
 Tuning &tuning();
 

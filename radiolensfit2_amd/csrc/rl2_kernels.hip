@@ -3771,15 +3771,15 @@ void launch_tile_fused(hipStream_t s, const TileLaunch &t)
         lik_fused_kernel<1><<<g, kTileBS, lds, s>>>(t);
 }
 
-// Pass 0 on extracted facets with two scalelength tasks per work-group (tasks r_lo + 2g and
-// r_lo + 2g + 1 of one item and chunk; the last group of an odd task count has one): the two
-// share each cell's loads, centres and shape terms (tile_sums64_p0_db<2>), and each task's
-// chunk sums go where its own work-group of lik_tile_kernel<0, KIND> puts them, with the same
-// values bit for bit.  Logical work-group wg2 = chunk * groups + g (chunk counted over the
-// launch's items), in the XCD-aware order of lik_tile_kernel.
-template <int KIND, bool TWO>
-__device__ inline void p0_pair_chunk(const TileLaunch &t, const LikItem *items, int item,
-                                     unsigned cg, unsigned chunk, int ra, double *red)
+// Pass 0 on extracted facets with TPW scalelength tasks per work-group (tasks r_lo + TPW g ..
+// of one item and chunk; the last group holds what is left): they share each cell's loads,
+// centres and shape terms (tile_sums64_p0_db<NT>), and each task's chunk sums go where its
+// own work-group of lik_tile_kernel<0, KIND> puts them, with the same values bit for bit.
+// Logical work-group wg2 = chunk * groups + g (chunk counted over the launch's items), in the
+// XCD-aware order of lik_tile_kernel.
+template <int KIND, int NTT>
+__device__ inline void p0_group_chunk(const TileLaunch &t, const LikItem *items, int item,
+                                      unsigned cg, unsigned chunk, int ra, double *red)
 {
     const LikLaunch &a = t.lk;
     const LikItem it = items[item];
@@ -3792,7 +3792,7 @@ __device__ inline void p0_pair_chunk(const TileLaunch &t, const LikItem *items, 
         for (unsigned k = threadIdx.x; k < F.grid_facet; k += kTileBS) sC[k] = gload(F.ctr, k);
         __syncthreads();
     };
-    constexpr int NT = TWO ? 2 : 1;
+    constexpr int NT = NTT;
     double ro[NT];
 #pragma unroll
     for (int q = 0; q < NT; q++) ro[q] = a.rg.ro[ra + q];
@@ -3803,14 +3803,14 @@ __device__ inline void p0_pair_chunk(const TileLaunch &t, const LikItem *items, 
     for (int q = 0; q < NT; q++) tile_reduce<7>(s[q], red, t.partials + (w0 + q) * 8);
 }
 
-template <int KIND>
-__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_p0_pair_kernel(TileLaunch t)
+template <int KIND, int TPW>
+__global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_p0_group_kernel(TileLaunch t)
 {
     __shared__ double red[(kTileBS / 64) * 8];
     if (KIND == 2 && blockIdx.x == 0 && threadIdx.x < t.n_inl)
         t.items_w[threadIdx.x] = t.inl[threadIdx.x];
     const LikItem *items = KIND == 2 ? t.inl : t.lk.items;
-    const unsigned nr = (unsigned)t.nr_g, groups = (nr + 1) / 2;
+    const unsigned nr = (unsigned)t.nr_g, groups = (nr + TPW - 1) / TPW;
     const unsigned nwg2 = t.nwg / nr * groups;
     const unsigned per = (nwg2 + kXcds - 1) / kXcds;
     const unsigned nbx = gridDim.x / kXcds;
@@ -3822,11 +3822,14 @@ __global__ __launch_bounds__(kTileBS, kTileGridWaves) void lik_p0_pair_kernel(Ti
                                        : tile_item(items, t.lk.nitems, cg * nr, (int)nr);
         const unsigned c0 = t.item_chunks ? (unsigned)item * t.item_chunks
                                           : items[item].chunk_first;
-        const int ra = t.r_lo + 2 * (int)g;
-        if (2 * g + 1 < nr)
-            p0_pair_chunk<KIND, true>(t, items, item, cg, cg - c0, ra, red);
+        const int ra = t.r_lo + TPW * (int)g;
+        const unsigned left = nr - TPW * g;
+        if (left >= (unsigned)TPW)
+            p0_group_chunk<KIND, TPW>(t, items, item, cg, cg - c0, ra, red);
+        else if (TPW == 3 && left == 2)
+            p0_group_chunk<KIND, 2>(t, items, item, cg, cg - c0, ra, red);
         else
-            p0_pair_chunk<KIND, false>(t, items, item, cg, cg - c0, ra, red);
+            p0_group_chunk<KIND, 1>(t, items, item, cg, cg - c0, ra, red);
     }
 }
 
@@ -3835,14 +3838,21 @@ void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass)
     const unsigned g = tile_grid(t.nwg, tuning().tile_blocks);
     if (pass == 0) {
         const unsigned lds = t.tab_lds_facet * 8;
-        // the grid kernels' pass 0 by pairs of tasks (RL2_TUNE_P0_PAIRS 1, the default)
-        const unsigned g2 =
-            tile_grid(t.nwg / (unsigned)t.nr_g * (((unsigned)t.nr_g + 1) / 2), tuning().tile_blocks);
-        if (t.grid == 2 && tuning().p0_pairs)
-            lik_p0_pair_kernel<2><<<g2, kTileBS, lds, s>>>(t);
-        else if (t.grid == 1 && tuning().p0_pairs)
-            lik_p0_pair_kernel<1><<<g2, kTileBS, lds, s>>>(t);
-        else if (t.grid == 2)
+        // the grid kernels' pass 0 with RL2_TUNE_P0_TASKS tasks per work-group (default 3)
+        const unsigned tpw = (unsigned)tuning().p0_tasks;
+        const unsigned g2 = tile_grid(
+            t.nwg / (unsigned)t.nr_g * (((unsigned)t.nr_g + tpw - 1) / tpw), tuning().tile_blocks);
+        if (t.grid && tpw == 3) {
+            if (t.grid == 2)
+                lik_p0_group_kernel<2, 3><<<g2, kTileBS, lds, s>>>(t);
+            else
+                lik_p0_group_kernel<1, 3><<<g2, kTileBS, lds, s>>>(t);
+        } else if (t.grid && tpw == 2) {
+            if (t.grid == 2)
+                lik_p0_group_kernel<2, 2><<<g2, kTileBS, lds, s>>>(t);
+            else
+                lik_p0_group_kernel<1, 2><<<g2, kTileBS, lds, s>>>(t);
+        } else if (t.grid == 2)
             lik_tile_kernel<0, 2><<<g, kTileBS, lds, s>>>(t);
         else if (t.grid == 1)
             lik_tile_kernel<0, 1><<<g, kTileBS, lds, s>>>(t);

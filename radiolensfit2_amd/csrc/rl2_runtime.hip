@@ -1974,9 +1974,9 @@ int rl2_tune_set(int knob, int value)
         if (value < 0 || value > 50) break;
         tuning().tile_passes = value;
         return RL2_OK;
-    case RL2_TUNE_P0_PAIRS:
-        if (value != 0 && value != 1) break;
-        tuning().p0_pairs = value;
+    case RL2_TUNE_P0_TASKS:
+        if (value < 1 || value > 3) break;
+        tuning().p0_tasks = value;
         return RL2_OK;
     case RL2_TUNE_TILE_STREAMS:
         if (value != 1 && value != 2) break;
@@ -2006,7 +2006,7 @@ int rl2_tune_get(int knob, int *value)
     case RL2_TUNE_ITEM_GROUPS: *value = (int)tuning().item_groups; return RL2_OK;
     case RL2_TUNE_FUSE_STEP: *value = tuning().fuse_step; return RL2_OK;
     case RL2_TUNE_TILE_PASSES: *value = tuning().tile_passes; return RL2_OK;
-    case RL2_TUNE_P0_PAIRS: *value = tuning().p0_pairs; return RL2_OK;
+    case RL2_TUNE_P0_TASKS: *value = tuning().p0_tasks; return RL2_OK;
     default: set_error("rl2_tune_get: unknown knob"); return RL2_ERR_ARG;
     }
 }

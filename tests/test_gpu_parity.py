@@ -142,17 +142,19 @@ def test_likelihood_on_extracted_facet_matches_oracle(dev, case, g):
     np.testing.assert_array_equal(nit1, nit)
     np.testing.assert_array_equal(Lr1[:, 1:24], Lr[:, 1:24])
     np.testing.assert_array_equal(Le1, Le)
-    # pass 0 with one task per work-group instead of pairs (RL2_TUNE_P0_PAIRS 0): each task's
-    # chunk sums are the same expressions in the same order, so the same values bit for bit
-    N.check(H.rl2_tune_set(N.RL2_TUNE_P0_PAIRS, 0), "rl2_tune_set")
-    try:
-        Le2, err2, Lr2, nit2 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
-    finally:
-        N.check(H.rl2_tune_set(N.RL2_TUNE_P0_PAIRS, 1), "rl2_tune_set")
-    np.testing.assert_array_equal(err2, err)
-    np.testing.assert_array_equal(nit2, nit)
-    np.testing.assert_array_equal(Lr2[:, 1:24], Lr[:, 1:24])
-    np.testing.assert_array_equal(Le2, Le)
+    # pass 0 with one or two tasks per work-group instead of three (RL2_TUNE_P0_TASKS): each
+    # task's chunk sums are the same expressions in the same order, so the same values bit
+    # for bit
+    for tpw in (1, 2):
+        N.check(H.rl2_tune_set(N.RL2_TUNE_P0_TASKS, tpw), "rl2_tune_set")
+        try:
+            Le2, err2, Lr2, nit2 = dev.loglik(e[:, 0], e[:, 1], slot=3, detail=True)
+        finally:
+            N.check(H.rl2_tune_set(N.RL2_TUNE_P0_TASKS, 3), "rl2_tune_set")
+        np.testing.assert_array_equal(err2, err)
+        np.testing.assert_array_equal(nit2, nit)
+        np.testing.assert_array_equal(Lr2[:, 1:24], Lr[:, 1:24])
+        np.testing.assert_array_equal(Le2, Le)
     # Newton steps in a step kernel after every pass (RL2_TUNE_FUSE_STEP 0), in the next
     # pass's work-groups (1), or every Newton pass of an evaluation in one launch (2); the
     # fused passes also queued one at a time (every evaluation then resumes from the step

@@ -229,7 +229,7 @@ def test_tune_knobs_validate_and_round_trip():
              (N.RL2_TUNE_ITEM_GROUPS, [256, 0], [-1, 70000]),
              (N.RL2_TUNE_FUSE_STEP, [0, 2, 1], [3, -1]),
              (N.RL2_TUNE_TILE_PASSES, [1, 50, 0], [-1, 51]),
-             (N.RL2_TUNE_P0_PAIRS, [0, 1], [2, -1])]
+             (N.RL2_TUNE_P0_TASKS, [1, 2, 3], [0, 4])]
     try:
         for knob, good, bad in cases:
             for x in good:
@@ -252,7 +252,7 @@ def test_tune_knobs_validate_and_round_trip():
         H.rl2_tune_set(N.RL2_TUNE_ITEM_GROUPS, 0)
         H.rl2_tune_set(N.RL2_TUNE_FUSE_STEP, 1)
         H.rl2_tune_set(N.RL2_TUNE_TILE_PASSES, 0)
-        H.rl2_tune_set(N.RL2_TUNE_P0_PAIRS, 1)
+        H.rl2_tune_set(N.RL2_TUNE_P0_TASKS, 3)
 
 
 def _philox_py(ctr, key, rounds=10):
