@@ -485,9 +485,9 @@ def run_config1(ngal=100):
 
 def roofline(kstats, lik_peak, fused=True):
     """Rooflines of the two kernels that bound a galaxy: the likelihood's chunk-sum kernels
-    (lik_tile_kernel for pass 0, and lik_fused_kernel / lik_fused_wide_kernel for the Newton
-    passes when the steps are folded into them, else lik_tile_kernel; FP64 VALU) and the
-    extraction's gridding
+    (lik_p0_group_kernel, or lik_tile_kernel, for pass 0, and lik_fused_kernel /
+    lik_fused_wide_kernel for the Newton passes when the steps are folded into them, else
+    lik_tile_kernel; FP64 VALU) and the extraction's gridding
     kernel (grid_run_kernel, HBM).
     achieved = the device-counted algorithmic FLOP-equivalents (bytes) of the timed launches
     / their summed durations from HIP events around each launch on the library stream."""
@@ -495,8 +495,8 @@ def roofline(kstats, lik_peak, fused=True):
     tr = measured_traffic()
     lik_tf = lik["flops"] / (lik["ms"] * 1e-3) / 1e12 if lik["ms"] > 0 else 0.0
     ext_gbs = ext["bytes"] / (ext["ms"] * 1e-3) / 1e9 if ext["ms"] > 0 else 0.0
-    r_lik = {"kernel": ("lik_tile_kernel + lik_fused_kernel / lik_fused_wide_kernel (chunk sums "
-                        "of the zero-offset and Newton passes, each Newton pass taking the "
+    r_lik = {"kernel": ("lik_p0_group_kernel + lik_fused_kernel / lik_fused_wide_kernel (chunk "
+                        "sums of the zero-offset and Newton passes, each Newton pass taking the "
                         "previous step)" if fused
                         else "lik_tile_kernel (chunk sums of the zero-offset and Newton passes)"),
              "bound": "valu", "achieved": round(lik_tf, 4),
