@@ -3872,10 +3872,13 @@ void launch_tile_sums(hipStream_t s, const TileLaunch &t, int pass)
 void launch_tile_step(hipStream_t s, const TileLaunch &t, int pass)
 {
     const unsigned ntask = (unsigned)t.lk.nitems * t.nr_g;
+    // the step is wave 0's; the other waves only write the next pass's tables, so a step
+    // without tables (the one after an evaluation's last fused pass) runs one wave
+    const unsigned bs = t.tab ? kStepBS : 64u;
     if (pass == 0)
-        lik_step_kernel<0><<<ntask, kStepBS, 0, s>>>(t);
+        lik_step_kernel<0><<<ntask, bs, 0, s>>>(t);
     else
-        lik_step_kernel<1><<<ntask, kStepBS, 0, s>>>(t);
+        lik_step_kernel<1><<<ntask, bs, 0, s>>>(t);
 }
 
 void launch_tile_pass(hipStream_t s, const TileLaunch &t, int pass)
