@@ -190,8 +190,9 @@ typedef struct {
     double flops;         /* algorithmic FLOP-equivalents (likelihood) */
     double bytes;         /* algorithmic HBM bytes (streaming kernels) */
 } rl2_kernel_stat;
-/* RL2_K_LIKELIHOOD: the likelihood's chunk-sum kernels (lik_tile_kernel); RL2_K_STEP: the
- * per-task step kernels that follow each of them (lik_step_kernel). */
+/* RL2_K_LIKELIHOOD: the likelihood's chunk-sum kernels (pass 0 and the Newton passes, which
+ * take the previous pass's step when steps are fused); RL2_K_STEP: the per-task step kernels
+ * (after every pass with RL2_TUNE_FUSE_STEP 0, else once after an evaluation's last pass). */
 enum { RL2_K_EXTRACT = 0, RL2_K_COMPACT = 1, RL2_K_LIKELIHOOD = 2, RL2_K_RMARG = 3,
        RL2_K_SUBTRACT = 4, RL2_K_SOURCES = 5, RL2_K_STEP = 6, RL2_K_COUNT = 7 };
 int rl2_stats_enable(int on);

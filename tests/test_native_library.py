@@ -330,3 +330,19 @@ def test_catalog_draws_follow_the_priors():
                                         for k in range(len(grid) - 1)])])
     F /= F[-1]
     assert ks(mod, lambda x: np.interp(x, grid, F)) < crit
+
+
+def test_header_constants_equal_the_bindings():
+    """Every enumerator of include/rl2.h has the value radiolensfit2_amd/_native.py binds."""
+    text = open(os.path.join(ROOT, "include", "rl2.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    consts = {}
+    for body in re.findall(r"enum\s*\{([^}]*)\}", text):
+        for name, val in re.findall(r"(RL2_\w+)\s*=\s*(-?\d+)", body):
+            consts[name] = int(val)
+    assert len(consts) >= 29  # status codes, precisions, kernel timers, launch knobs
+    for name, val in consts.items():
+        if name == "RL2_K_COUNT":
+            assert val == len(N.KERNEL_NAMES)
+            continue
+        assert getattr(N, name) == val, name
