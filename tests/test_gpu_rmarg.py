@@ -2,7 +2,8 @@
 against the oracle's set_posterior_values + marginalise_posterior_r (likelihood.c:126-146,
 marginalise_r.c:29-223) on caller-given L_r rows, including the cases the fits reach only
 rarely: points at kBad or NaN, fewer than 5 points, all values below the -10 floor of ymax,
-a flat posterior, a rough one.  Tolerance: L_e within 1e-8 absolute (the wave's Gauss-Kronrod
+a flat posterior, a rough one, one whose cubic spline overflows exp (the retry on the linear
+interpolant).  Tolerance: L_e within 1e-8 absolute (the wave's Gauss-Kronrod
 sums differ from QUADPACK's sequential order only by rounding), error flags equal."""
 import ctypes as C
 
@@ -49,6 +50,9 @@ def _rows(numr, ro, rng):
         h = rng.uniform(-50.0, 5e4)
         rows.append(h - 0.5 * ((lro - c) / w) ** 2 * rng.uniform(1.0, 400.0)
                     + rng.normal(0.0, 0.02, numr - 1))
+    # alternating 0 / -40000: the natural cubic overshoots beyond exp's range, QAG on it fails
+    # and the integral is retried on the linear interpolant (marginalise_r.c:171-210)
+    rows.append(np.where(np.arange(numr - 1) % 2 == 0, 0.0, -40000.0))
     base = 200.0 - 30.0 * (lro - lro.mean()) ** 2
     r = base.copy(); r[[0, 3, 7]] = KBAD; rows.append(r)               # scattered kBad
     r = base.copy(); r[:4] = KBAD; r[-3:] = KBAD; rows.append(r)        # ends at kBad

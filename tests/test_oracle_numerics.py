@@ -150,6 +150,32 @@ def test_marginalise_posterior_r_ymax_floor():
     assert got == pytest.approx(-50.0 + math.log(x[-1] - x[0]), rel=1e-12)
 
 
+def _qag(f, a, b):
+    cb = O.F1_CB(lambda t, p: f(t))
+    r, e, n = C.c_double(), C.c_double(), C.c_ulong()
+    st = L.orl2_qag41(cb, None, a, b, 0.0, 1e-3, 1000, C.byref(r), C.byref(e), C.byref(n))
+    return st, r.value
+
+
+def test_marginalise_posterior_r_retries_with_linear_interpolation():
+    # marginalise_r.c:171-210: when QAG on the cubic spline fails, the integral is taken again
+    # on the linear interpolant.  Log-posteriors alternating 0 / -40000 make the natural cubic
+    # overshoot beyond exp's range (scipy's spline, not the oracle's, shows it), so QAG on it
+    # fails; QAG on numpy's linear interpolant succeeds, and the oracle returns that integral.
+    ro, numr = O.r_grid()
+    x = ro[1:numr]
+    y = np.where(np.arange(len(x)) % 2 == 0, 0.0, -40000.0)
+    spl = sint.CubicSpline(x, y, bc_type="natural")
+    assert spl(np.linspace(x[0], x[-1], 20001)).max() > math.log(np.finfo(float).max)
+    with np.errstate(over="ignore"):
+        st_cubic, _ = _qag(lambda t: float(np.exp(spl(t))), x[0], x[-1])
+    assert st_cubic != 0
+    st_lin, lin = _qag(lambda t: math.exp(float(np.interp(t, x, y))), x[0], x[-1])
+    assert st_lin == 0 and lin > 0
+    got, _ = _oracle_rmarg(x, y)
+    assert got == pytest.approx(math.log(lin), abs=1e-9)  # ymax = 0
+
+
 def test_set_posterior_values_drops_invalid():
     ro, numr = O.r_grid()
     Lr = np.linspace(-5, 5, 40)
