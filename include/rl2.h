@@ -178,6 +178,11 @@ int rl2_marginalise_posterior_r_noalloc(int num_marvals, const double *xmarvals,
  * and for callers that compute L_r themselves; synchronous. */
 int rl2_rmarg_eval(int nitems, int numr, const double *ro, const double *rprior,
                    const double *Lr, double *Le, int *err);
+/* No counterpart in the reference: marginalise_over_position_shift (likelihood.c:376-421,
+ * declared in likelihood.h) evaluated by the device function the likelihood kernels apply,
+ * for n host values x (out: n values, -1e10 for x <= Delta as the reference).  For tests;
+ * synchronous. */
+int rl2_marg_position_eval(int n, const double *x, double *out);
 
 /* ---- scalelength grid and the fixed constants (RadioLensfit2.cpp:370-392) ---- */
 int rl2_r_grid(double *ro /* >= 29 */);

@@ -134,6 +134,7 @@ _HIP_SIGS = {
                                                       C.c_void_p]),
     "rl2_rmarg_eval": (C.c_int, [C.c_int, C.c_int, c_double_p, c_double_p, c_double_p,
                                  c_double_p, C.POINTER(C.c_int)]),
+    "rl2_marg_position_eval": (C.c_int, [C.c_int, c_double_p, c_double_p]),
     "rl2_stats_enable": (C.c_int, [C.c_int]),
     "rl2_stats_reset": (C.c_int, []),
     "rl2_stats_get": (C.c_int, [C.c_int, C.POINTER(KernelStat)]),

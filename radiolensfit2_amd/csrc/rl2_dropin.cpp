@@ -21,6 +21,9 @@ using namespace rl2;
 namespace {
 
 // ---------------------------------------------------------------- small device buffers
+// A device array of one call.  A failed allocation leaves p null and a failed transfer
+// returns false, each with the library's error message set: callers check `allocated(...)`
+// and the transfers before a launch takes the pointers.
 template <class T>
 struct DBuf {
     T *p = nullptr;
@@ -28,25 +31,41 @@ struct DBuf {
     explicit DBuf(size_t count)
     {
         n = count ? count : 1;
-        if (hipMalloc((void **)&p, n * sizeof(T)) != hipSuccess) p = nullptr;
+        if (!hip_ok(hipMalloc((void **)&p, n * sizeof(T)), "hipMalloc")) p = nullptr;
     }
     ~DBuf()
     {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
     bool up(const T *h, size_t count)
     {
-        return p && hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, stream()) ==
-                        hipSuccess;
+        return p && hip_ok(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice,
+                                          stream()),
+                           "upload");
     }
     bool down(T *h, size_t count) const
     {
-        return p && hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, stream()) ==
-                        hipSuccess;
+        return p && hip_ok(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost,
+                                          stream()),
+                           "download");
     }
 };
 
-bool sync_ok() { return hipStreamSynchronize(stream()) == hipSuccess; }
+template <class... B>
+bool allocated(const B &...b)
+{
+    return ((b.p != nullptr) && ...);
+}
+
+bool sync_ok() { return hip_ok(hipStreamSynchronize(stream()), "hipStreamSynchronize"); }
+
+// Queues a device-to-host copy on the library's stream.
+bool copy_down(void *h, const void *d, size_t bytes)
+{
+    return hip_ok(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, stream()), "download");
+}
 
 void report(const char *fn)
 {
@@ -142,8 +161,7 @@ bool facet_for(const likelihood_params *par, Obs **out, int *slot)
     if (!set_grid(o, par->numr, par->ro) || !o.ensure_slots(1)) return false;
     for (int k = 0; k < kMaxR; k++) o.slots[0].rprior[k] = (k < par->numr) ? par->rprior[k] : 0.;
     o.slots[0].n = n;
-    o.facet_upload(0);
-    return true;
+    return o.facet_upload(0);
 }
 
 // Last arrays given to marginalise_posterior_r, for marf().
@@ -201,12 +219,14 @@ double marginalise_posterior_r(int num_marvals, double *xmarvals, double *ymarva
     DBuf<double> dx(num_marvals), dy(num_marvals), dout(num_marvals + 1);
     DBuf<double> ws(kQagDoubles);
     DBuf<int> iws(kQagInts);
-    dx.up(xmarvals, num_marvals);
-    dy.up(ymarvals, num_marvals);
+    if (!allocated(dx, dy, dout, ws, iws) || !dx.up(xmarvals, num_marvals) ||
+        !dy.up(ymarvals, num_marvals)) {
+        report("marginalise_posterior_r");
+        return kBad;
+    }
     launch_marginalise_r(stream(), num_marvals, dx.p, dy.p, dout.p, ws.p, iws.p);
     std::vector<double> out(num_marvals + 1);
-    dout.down(out.data(), num_marvals + 1);
-    if (!sync_ok()) {
+    if (!dout.down(out.data(), num_marvals + 1) || !sync_ok()) {
         report("marginalise_posterior_r");
         return kBad;
     }
@@ -222,12 +242,13 @@ void model_galaxy_visibilities_at_zero(double e1, double e2, double scale, unsig
     if (!ensure_device() || num_coords == 0) return;
     DBuf<double> du(num_coords), dv(num_coords), ds(num_coords), dm(num_coords),
         part((num_coords + 255) / 256);
-    du.up(grid_u, num_coords);
-    dv.up(grid_v, num_coords);
-    ds.up(sigma2, num_coords);
+    if (!allocated(du, dv, ds, dm, part) || !du.up(grid_u, num_coords) ||
+        !dv.up(grid_v, num_coords) || !ds.up(sigma2, num_coords)) {
+        report("model_galaxy_visibilities_at_zero");
+        return;
+    }
     launch_model_at_zero(stream(), e1, e2, scale, num_coords, du.p, dv.p, ds.p, dm.p, part.p);
-    dm.down(Modvis, num_coords);
-    if (!sync_ok()) report("model_galaxy_visibilities_at_zero");
+    if (!dm.down(Modvis, num_coords) || !sync_ok()) report("model_galaxy_visibilities_at_zero");
 }
 
 void data_galaxy_visibilities(double spectra, double wavenumber, double, double, double e1,
@@ -237,13 +258,15 @@ void data_galaxy_visibilities(double spectra, double wavenumber, double, double,
 {
     if (!ensure_device() || num_coords == 0) return;
     DBuf<double> du(num_coords), dv(num_coords), dw(num_coords), dvis(2 * (size_t)num_coords);
-    du.up(uu_metres, num_coords);
-    dv.up(vv_metres, num_coords);
-    dw.up(ww_metres, num_coords);
+    if (!allocated(du, dv, dw, dvis) || !du.up(uu_metres, num_coords) ||
+        !dv.up(vv_metres, num_coords) || !dw.up(ww_metres, num_coords)) {
+        report("data_galaxy_visibilities");
+        return;
+    }
     GalaxyGeom g = make_geom(l, m, flux, scale, e1, e2);
     launch_galaxy_vis(stream(), spectra, wavenumber, g, num_coords, du.p, dv.p, dw.p, dvis.p);
-    dvis.down((double *)vis, 2 * (size_t)num_coords);
-    if (!sync_ok()) report("data_galaxy_visibilities");
+    if (!dvis.down((double *)vis, 2 * (size_t)num_coords) || !sync_ok())
+        report("data_galaxy_visibilities");
 }
 
 void data_visibilities_phase_shift(double wavenumber, double l, double m, unsigned int num_coords,
@@ -252,13 +275,15 @@ void data_visibilities_phase_shift(double wavenumber, double l, double m, unsign
 {
     if (!ensure_device() || num_coords == 0) return;
     DBuf<double> du(num_coords), dv(num_coords), dw(num_coords), dvis(2 * (size_t)num_coords);
-    du.up(uu_metres, num_coords);
-    dv.up(vv_metres, num_coords);
-    dw.up(ww_metres, num_coords);
-    dvis.up((const double *)vis, 2 * (size_t)num_coords);
+    if (!allocated(du, dv, dw, dvis) || !du.up(uu_metres, num_coords) ||
+        !dv.up(vv_metres, num_coords) || !dw.up(ww_metres, num_coords) ||
+        !dvis.up((const double *)vis, 2 * (size_t)num_coords)) {
+        report("data_visibilities_phase_shift");
+        return;
+    }
     launch_phase_shift(stream(), wavenumber, l, m, num_coords, du.p, dv.p, dw.p, dvis.p);
-    dvis.down((double *)vis, 2 * (size_t)num_coords);
-    if (!sync_ok()) report("data_visibilities_phase_shift");
+    if (!dvis.down((double *)vis, 2 * (size_t)num_coords) || !sync_ok())
+        report("data_visibilities_phase_shift");
 }
 
 // Shared by the three gridding entry points: grid (mode 1/2) natural-order rows.
@@ -283,16 +308,18 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     DBuf<double> acc(grid_acc_words(sizeg)), cu(ncell), cv(ncell), cre(ncell), cim(ncell),
         cs2(ncell), csw(ncell);
     DBuf<unsigned> scan(compact_scratch_words(sizeg) + 1), dn(1);
-    du.up(u, ncoords);
-    dv.up(v, ncoords);
-    dwn.up(wavenumbers, nch);
-    dinv.up(invl.data(), nch);
-    dsp.up(spec.data(), nch);
-    dperm.up(iota.data(), ncoords);
-    ds2.up(s2.data(), nvis);
-    if (flag) dfl.up((const unsigned char *)flag, nvis);
-    if (mode == 1) dres.up((const double *)vis, 2 * nvis);
-    hipMemsetAsync(acc.p, 0, grid_acc_words(sizeg) * sizeof(double), stream());
+    if (!allocated(du, dv, dwn, dinv, dsp, dres, dperm, ds2, ds2f, dfl, acc, cu, cv, cre, cim,
+                   cs2, csw, scan, dn) ||
+        !du.up(u, ncoords) || !dv.up(v, ncoords) || !dwn.up(wavenumbers, nch) ||
+        !dinv.up(invl.data(), nch) || !dsp.up(spec.data(), nch) ||
+        !dperm.up(iota.data(), ncoords) || !ds2.up(s2.data(), nvis) ||
+        (flag && !dfl.up((const unsigned char *)flag, nvis)) ||
+        (mode == 1 && !dres.up((const double *)vis, 2 * nvis)) ||
+        !hip_ok(hipMemsetAsync(acc.p, 0, grid_acc_words(sizeg) * sizeof(double), stream()),
+                "hipMemsetAsync")) {
+        report("gridding");
+        return 0;
+    }
     // scales of the integer accumulators from the largest gridded value and variance
     double vb = 0., sb = 0.;
     if (mode == 1)
@@ -324,21 +351,21 @@ static unsigned grid_natural(int mode, const double *wavenumbers, unsigned nch, 
     launch_compact(stream(), acc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
                    dn.p, csw.p, scan.p, &sc);
     unsigned n = 0;
-    dn.down(&n, 1);
-    if (!sync_ok()) {
+    if (!dn.down(&n, 1) || !sync_ok()) {
         report("gridding");
         return 0;
     }
     std::vector<double> re(n), im(n);
-    if (new_vis) {
-        cre.down(re.data(), n);
-        cim.down(im.data(), n);
+    bool ok = true;
+    if (new_vis) ok = cre.down(re.data(), n) && cim.down(im.data(), n);
+    if (new_sigma2) ok = ok && cs2.down(new_sigma2, n);
+    if (sum_w) ok = ok && csw.down(sum_w, ncell);
+    if (gu) ok = ok && cu.down(gu, n);
+    if (gv) ok = ok && cv.down(gv, n);
+    if (!sync_ok() || !ok) {
+        report("gridding copy");
+        return 0;
     }
-    if (new_sigma2) cs2.down(new_sigma2, n);
-    if (sum_w) csw.down(sum_w, ncell);
-    if (gu) cu.down(gu, n);
-    if (gv) cv.down(gv, n);
-    if (!sync_ok()) report("gridding copy");
     if (new_vis)
         for (unsigned i = 0; i < n; i++) {
             new_vis[i].real = re[i];
@@ -380,21 +407,19 @@ static unsigned compact_host(unsigned sizeg, double len, const double *re, const
     }
     DBuf<double> dacc(4 * ncell), cu(ncell), cv(ncell), cre(ncell), cim(ncell), cs2(ncell);
     DBuf<unsigned> scan(compact_scratch_words(sizeg) + 1), dn(1);
-    dacc.up(acc.data(), 4 * ncell);
+    if (!allocated(dacc, cu, cv, cre, cim, cs2, scan, dn) || !dacc.up(acc.data(), 4 * ncell))
+        return 0;
     launch_compact(stream(), dacc.p, sizeg, len, cu.p, cv.p, cre.p, cim.p, cs2.p, nullptr, nullptr,
                    dn.p, nullptr, scan.p, nullptr);
     unsigned n = 0;
-    dn.down(&n, 1);
-    if (!sync_ok()) return 0;
+    if (!dn.down(&n, 1) || !sync_ok()) return 0;
     std::vector<double> vr(n), vi(n);
-    if (gu) cu.down(gu, n);
-    if (gv) cv.down(gv, n);
-    if (out_vis) {
-        cre.down(vr.data(), n);
-        cim.down(vi.data(), n);
-    }
-    if (out_s2) cs2.down(out_s2, n);
-    if (!sync_ok()) return 0;
+    bool ok = true;
+    if (gu) ok = cu.down(gu, n);
+    if (gv) ok = ok && cv.down(gv, n);
+    if (out_vis) ok = ok && cre.down(vr.data(), n) && cim.down(vi.data(), n);
+    if (out_s2) ok = ok && cs2.down(out_s2, n);
+    if (!sync_ok() || !ok) return 0;
     if (out_vis)
         for (unsigned i = 0; i < n; i++) {
             out_vis[i].real = vr[i];
@@ -409,9 +434,11 @@ unsigned int evaluate_facet_coords(double *grid_u, double *grid_v, double len, u
     if (g_obs && g_last.sum_w == count_w && g_last.facet == sizeg) {
         Obs &o = g_obs->o;
         FacetSlot &s = o.slots[0];
-        hipMemcpyAsync(grid_u, s.uu, g_last.n * 8, hipMemcpyDeviceToHost, stream());
-        hipMemcpyAsync(grid_v, s.vv, g_last.n * 8, hipMemcpyDeviceToHost, stream());
-        if (!sync_ok()) report("evaluate_facet_coords");
+        if (!copy_down(grid_u, s.uu, g_last.n * 8) || !copy_down(grid_v, s.vv, g_last.n * 8) ||
+            !sync_ok()) {
+            report("evaluate_facet_coords");
+            return 0;
+        }
         g_last.uu = grid_u;
         g_last.vv = grid_v;
         return g_last.n;
@@ -454,8 +481,7 @@ void source_extraction(int, unsigned int facet, likelihood_params *par, complexd
         return;
     }
     unsigned n = 0;
-    hipMemcpyAsync(&n, &o->d_facets[0].n, sizeof n, hipMemcpyDeviceToHost, stream());
-    if (!sync_ok()) {
+    if (!copy_down(&n, &o->d_facets[0].n, sizeof n) || !sync_ok()) {
         report("source_extraction");
         return;
     }
@@ -463,11 +489,9 @@ void source_extraction(int, unsigned int facet, likelihood_params *par, complexd
     s.n = n;
     s.facet = facet;
     std::vector<double> re(n), im(n);
-    hipMemcpyAsync(re.data(), s.dre, n * 8, hipMemcpyDeviceToHost, stream());
-    hipMemcpyAsync(im.data(), s.dim, n * 8, hipMemcpyDeviceToHost, stream());
-    hipMemcpyAsync(facet_sigma2, s.s2, n * 8, hipMemcpyDeviceToHost, stream());
-    hipMemcpyAsync(sum_w, s.sumw, (size_t)facet * facet * 8, hipMemcpyDeviceToHost, stream());
-    if (!sync_ok()) {
+    if (!copy_down(re.data(), s.dre, n * 8) || !copy_down(im.data(), s.dim, n * 8) ||
+        !copy_down(facet_sigma2, s.s2, n * 8) ||
+        !copy_down(sum_w, s.sumw, (size_t)facet * facet * 8) || !sync_ok()) {
         report("source_extraction");
         return;
     }
@@ -522,7 +546,7 @@ double loglikelihood_r(double ee1, double ee2, double scale, unsigned long int n
         return kBad;
     }
     const double ro[2] = {0., scale};
-    set_grid(*o, 2, ro);
+    set_grid(*o, 2, ro);  // numr 2: always accepted
     std::vector<LikItem> items{{0, 0, ee1, ee2}};
     double Le;
     int err;
@@ -548,36 +572,40 @@ int cross_correlation(unsigned long int n_uv_coords, double *grid_u, double *gri
         return 1;
     }
     DBuf<double> dmod(n_uv_coords), dho(kMaxR), ddet(kMaxR);
-    dmod.up(visMod, n_uv_coords);
+    if (!allocated(dmod, dho, ddet) || !dmod.up(visMod, n_uv_coords)) {
+        report("cross_correlation");
+        return 1;
+    }
     const double ro[2] = {0., 1.};
-    set_grid(*o, 2, ro);
-    o->h_facets[0].mod = dmod.p;
-    o->facet_upload(0);
+    set_grid(*o, 2, ro);  // numr 2: always accepted
     LikBuffers &b = o->lik;
-    if (!b.ensure(1)) return 1;
-    LikItem it{0, 0, 0., 0.};
-    hipMemcpyAsync(b.d_items, &it, sizeof it, hipMemcpyHostToDevice, stream());
-    LikLaunch a = {};
-    a.nitems = 1;
-    a.items = b.d_items;
-    a.facets = o->d_facets;
-    a.rg = o->rg;
-    a.Lr = b.d_Lr;
-    a.nit = b.d_nit;
-    a.Le = b.d_Le;
-    a.err = b.d_err;
-    a.qag_ws = b.d_ws;
-    a.qag_iws = b.d_iws;
-    a.block = 256;
-    a.ho = dho.p;
-    a.det = ddet.p;
-    launch_likelihood(stream(), a);
+    const LikItem it{0, 0, 0., 0.};
+    o->h_facets[0].mod = dmod.p;
+    bool ok = o->facet_upload(0) && b.ensure(1) &&
+              hip_ok(hipMemcpyAsync(b.d_items, &it, sizeof it, hipMemcpyHostToDevice, stream()),
+                     "upload");
     double h[kMaxR], d[kMaxR];
-    dho.down(h, kMaxR);
-    ddet.down(d, kMaxR);
-    bool ok = sync_ok();
+    if (ok) {
+        LikLaunch a = {};
+        a.nitems = 1;
+        a.items = b.d_items;
+        a.facets = o->d_facets;
+        a.rg = o->rg;
+        a.Lr = b.d_Lr;
+        a.nit = b.d_nit;
+        a.Le = b.d_Le;
+        a.err = b.d_err;
+        a.qag_ws = b.d_ws;
+        a.qag_iws = b.d_iws;
+        a.block = 256;
+        a.ho = dho.p;
+        a.det = ddet.p;
+        launch_likelihood(stream(), a);
+        ok = dho.down(h, kMaxR) && ddet.down(d, kMaxR);
+    }
+    ok = sync_ok() && ok;  // also before dmod is freed: nothing may still read it
     o->h_facets[0].mod = nullptr;
-    o->facet_upload(0);
+    ok = o->facet_upload(0) && ok;
     if (!ok) {
         report("cross_correlation");
         return 1;
@@ -661,17 +689,16 @@ void sky_model(double *wavenumbers, double *spec, double, int, unsigned int num_
     DBuf<GalaxyGeom> dg(n_gal);
     std::vector<GalaxyGeom> g(n_gal);
     for (unsigned i = 0; i < n_gal; i++) g[i] = make_geom(l[i], m[i], gflux[i], gscale[i], 0., 0.);
-    du.up(uu_metres, num_coords);
-    dv.up(vv_metres, num_coords);
-    dw.up(ww_metres, num_coords);
-    dwn.up(wavenumbers, num_channels);
-    dsp.up(spec, num_channels);
-    dres.up((const double *)visMod, 2 * nvis);
-    dg.up(g.data(), n_gal);
+    if (!allocated(du, dv, dw, dwn, dsp, dres, dg) || !du.up(uu_metres, num_coords) ||
+        !dv.up(vv_metres, num_coords) || !dw.up(ww_metres, num_coords) ||
+        !dwn.up(wavenumbers, num_channels) || !dsp.up(spec, num_channels) ||
+        !dres.up((const double *)visMod, 2 * nvis) || !dg.up(g.data(), n_gal)) {
+        report("sky_model");
+        return;
+    }
     launch_add_sources(stream(), 0, num_channels, num_coords, du.p, dv.p, dw.p, dwn.p, dsp.p, dg.p,
                        n_gal, 1.0, dres.p);
-    dres.down((double *)visMod, 2 * nvis);
-    if (!sync_ok()) report("sky_model");
+    if (!dres.down((double *)visMod, 2 * nvis) || !sync_ok()) report("sky_model");
 }
 
 void data_processing(bool re_fitting, unsigned int *bad_list, int nsources, double len,

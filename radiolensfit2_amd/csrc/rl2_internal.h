@@ -369,6 +369,7 @@ void launch_phase_shift(hipStream_t s, double wavenumber, double l, double m, un
                         const double *u, const double *v, const double *w, double *vis);
 void launch_marginalise_r(hipStream_t s, int num, const double *x, const double *y,
                           double *out, double *qag_ws, int *qag_iws);
+void launch_marg_position(hipStream_t s, int n, const double *x, double *out);
 
 // Row permutation / packing at observation set-up.
 void launch_gather_rows(hipStream_t s, unsigned long nrow, const uint32_t *perm,
@@ -381,7 +382,7 @@ void launch_residual_out(hipStream_t s, int prec, unsigned nch, unsigned long nr
                          const uint32_t *perm, const void *res, double *out);
 void launch_morton_keys(hipStream_t s, unsigned long nrow, const double *u, const double *v,
                         double umax, uint32_t *keys, uint32_t *idx);
-void sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
+bool sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
                uint32_t *keys_alt, uint32_t *idx_alt, void *tmp, size_t tmp_bytes,
                uint32_t **sorted_idx);
 size_t sort_rows_tmp_bytes(unsigned long nrow);

@@ -3563,12 +3563,14 @@ unsigned persist_capacity(unsigned lds, int kind)
     auto f = cache.find(key);
     if (f != cache.end()) return f->second;
     int per = 0, dev = 0, cus = 0;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (kind == 2)
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<2>, kTileBS, lds);
-    else
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<1>, kTileBS, lds);
+    const bool queried =
+        hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        (kind == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<2>,
+                                                                  kTileBS, lds)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, lik_persist_kernel<1>,
+                                                                  kTileBS, lds)) == hipSuccess;
+    if (!queried) return 0;  // not cached: the next call asks again
     const unsigned cap = (unsigned)std::max(0, per) * (unsigned)std::max(0, cus);
     cache[key] = cap;
     return cap;
@@ -3912,6 +3914,20 @@ void launch_marginalise_r(hipStream_t s, int num, const double *x, const double 
     marginalise_r_kernel<<<1, 1, 0, s>>>(num, x, y, out, qag_ws, qag_iws);
 }
 
+// marg_position (likelihood.c:376-421) on n given arguments, one per thread: the device
+// function finish_lr applies to B = ho^2 / 2.
+__global__ void marg_position_kernel(int n, const double *x, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = marg_position(x[i]);
+}
+
+void launch_marg_position(hipStream_t s, int n, const double *x, double *out)
+{
+    if (n <= 0) return;
+    marg_position_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, x, out);
+}
+
 // ------------------------------------------------------------------ subtraction (K2)
 // data_processing.c:109-135: visSkyMod -= round model, visData -= fitted model, i.e.
 // R = (R - fit) + round with one phase rotor per visibility.
@@ -4158,17 +4174,21 @@ size_t sort_rows_tmp_bytes(unsigned long nrow)
 {
     size_t bytes = 0;
     hipcub::DoubleBuffer<uint32_t> k(nullptr, nullptr), v(nullptr, nullptr);
-    hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)nrow);
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, v, (int)nrow) != hipSuccess)
+        return 0;  // the sort itself then reports the failure
     return bytes;
 }
 
-void sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
+bool sort_rows(hipStream_t s, unsigned long nrow, uint32_t *keys, uint32_t *idx,
                uint32_t *keys_alt, uint32_t *idx_alt, void *tmp, size_t tmp_bytes,
                uint32_t **sorted_idx)
 {
     hipcub::DoubleBuffer<uint32_t> k(keys, keys_alt), v(idx, idx_alt);
-    hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k, v, (int)nrow, 0, 32, s);
+    const bool ok = hip_ok(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k, v, (int)nrow,
+                                                              0, 32, s),
+                           "row sort");
     *sorted_idx = v.Current();
+    return ok;
 }
 
 }  // namespace rl2

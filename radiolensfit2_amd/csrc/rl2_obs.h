@@ -99,7 +99,7 @@ struct Obs {
     std::vector<std::pair<unsigned long long, hipGraphExec_t>> batch_graphs;
     ~Obs();
     bool ensure_slots(int n);
-    void facet_upload(int slot);  // push host descriptor (pointers, prior) to the device
+    bool facet_upload(int slot);  // push host descriptor (pointers, prior) to the device
 };
 
 // Batched likelihood evaluation for items over the observation's facet slots.

@@ -38,8 +38,8 @@ hipEvent_t take_event()
         g_event_pool.pop_back();
         return e;
     }
-    hipEvent_t e;
-    hipEventCreate(&e);
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;  // the timer then records nothing
     return e;
 }
 void drain_stats()
@@ -47,9 +47,9 @@ void drain_stats()
     for (auto &k : g_stats) {
         for (auto &p : k.pending) {
             float ms = 0.f;
-            hipEventSynchronize(p.second);
-            hipEventElapsedTime(&ms, p.first, p.second);
-            k.ms += ms;
+            if (hipEventSynchronize(p.second) == hipSuccess &&
+                hipEventElapsedTime(&ms, p.first, p.second) == hipSuccess)
+                k.ms += ms;
             g_event_pool.push_back(p.first);
             g_event_pool.push_back(p.second);
         }
@@ -82,13 +82,21 @@ KTimer::KTimer(int k) : kernel(k)
     if (g_stats_on) {
         a = take_event();
         b = take_event();
-        hipEventRecord(a, g_stream);
+        if (!a || !b || hipEventRecord(a, g_stream) != hipSuccess) {
+            if (a) g_event_pool.push_back(a);
+            if (b) g_event_pool.push_back(b);
+            a = b = nullptr;
+        }
     }
 }
 KTimer::~KTimer()
 {
-    if (g_stats_on && a) {
-        hipEventRecord(b, g_stream);
+    if (g_stats_on && a && b) {
+        if (hipEventRecord(b, g_stream) != hipSuccess) {
+            g_event_pool.push_back(a);
+            g_event_pool.push_back(b);
+            return;
+        }
         g_stats[kernel].pending.push_back({a, b});
         g_stats[kernel].launches++;
     }
@@ -147,7 +155,7 @@ static bool dmalloc(T **p, size_t count)
 template <class T>
 static void dfree(T *&p)
 {
-    if (p) hipFree((void *)p);
+    if (p) (void)hipFree((void *)p);
     p = nullptr;
 }
 
@@ -196,10 +204,10 @@ void LikBuffers::release()
     dfree(d_iws);
     dfree(d_tasks);
     dfree(d_cnt);
-    if (h_out) hipHostFree(h_out);
-    if (h_items) hipHostFree(h_items);
-    if (h_Lr) hipHostFree(h_Lr);
-    if (h_nit) hipHostFree(h_nit);
+    if (h_out) (void)hipHostFree(h_out);
+    if (h_items) (void)hipHostFree(h_items);
+    if (h_Lr) (void)hipHostFree(h_Lr);
+    if (h_nit) (void)hipHostFree(h_nit);
     h_out = nullptr;
     h_items = nullptr;
     h_Lr = nullptr;
@@ -255,7 +263,7 @@ bool LikBuffers::ensure_tab(size_t entries)
 // ------------------------------------------------------------------ observation
 Obs::~Obs()
 {
-    hipStreamSynchronize(g_stream);
+    (void)hipStreamSynchronize(g_stream);  // no kernel may still use what is freed below
     dfree(d_u);
     dfree(d_v);
     dfree(d_w);
@@ -285,9 +293,9 @@ Obs::~Obs()
     }
     dfree(d_facets);
     dfree(d_feat);
-    for (auto &g : batch_graphs) hipGraphExecDestroy(g.second);
+    for (auto &g : batch_graphs) (void)hipGraphExecDestroy(g.second);
     batch_graphs.clear();
-    if (h_desc) hipHostFree(h_desc);
+    if (h_desc) (void)hipHostFree(h_desc);
     h_desc = nullptr;
     dfree(d_desc);
     dfree(d_queue);
@@ -325,9 +333,15 @@ bool Obs::ensure_slots(int n)
     FacetDev *nd = nullptr;
     if (!dmalloc(&nd, n)) return false;
     if (d_facets) {
-        hipMemcpyAsync(nd, d_facets, old * sizeof(FacetDev), hipMemcpyDeviceToDevice, g_stream);
-        hipStreamSynchronize(g_stream);
-        hipFree(d_facets);
+        const bool copied = hip_ok(hipMemcpyAsync(nd, d_facets, old * sizeof(FacetDev),
+                                                  hipMemcpyDeviceToDevice, g_stream),
+                                   "facet descriptors") &&
+                            hip_ok(hipStreamSynchronize(g_stream), "facet descriptors");
+        if (!copied) {
+            dfree(nd);
+            return false;
+        }
+        dfree(d_facets);
     }
     d_facets = nd;
     h_facets.resize(n);
@@ -356,14 +370,17 @@ bool Obs::ensure_slots(int n)
 }
 
 // Pushes the slot descriptor except the device-written count n (updated by compaction).
-void Obs::facet_upload(int i)
+bool Obs::facet_upload(int i)
 {
     FacetDev &f = h_facets[i];
     for (int k = 0; k < kMaxR; k++) f.rprior[k] = slots[i].rprior[k];
     const size_t off = offsetof(FacetDev, rprior);
-    hipMemcpyAsync((char *)(d_facets + i) + off, (char *)&f + off, sizeof(FacetDev) - off,
-                   hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(d_facets + i, &f, offsetof(FacetDev, n), hipMemcpyHostToDevice, g_stream);
+    return hip_ok(hipMemcpyAsync((char *)(d_facets + i) + off, (char *)&f + off,
+                                 sizeof(FacetDev) - off, hipMemcpyHostToDevice, g_stream),
+                  "facet descriptor") &&
+           hip_ok(hipMemcpyAsync(d_facets + i, &f, offsetof(FacetDev, n), hipMemcpyHostToDevice,
+                                 g_stream),
+                  "facet descriptor");
 }
 
 bool Obs::measure_res_bound()
@@ -439,7 +456,7 @@ static bool finish_extract(Obs &o, int slot, unsigned facet, bool want_sumw, con
         f.cid = gf ? s.cid : nullptr;
         f.ctr = gf ? s.ctr : nullptr;
         f.grid_facet = gf;
-        o.facet_upload(slot);
+        if (!o.facet_upload(slot)) return false;
     }
     return true;
 }
@@ -632,7 +649,7 @@ bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
                     return false;
                 const bool ok = hip_ok(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0),
                                        "hipGraphInstantiate");
-                hipGraphDestroy(graph);
+                (void)hipGraphDestroy(graph);
                 if (!ok) return false;
                 o.batch_graphs.push_back({key, exec});
             }
@@ -655,7 +672,7 @@ bool obs_extract_batch(Obs &o, const std::vector<ExtractReq> &reqs,
             f.cid = gf ? sl.cid : nullptr;
             f.ctr = gf ? sl.ctr : nullptr;
             f.grid_facet = gf;
-            o.facet_upload(slot);
+            if (!o.facet_upload(slot)) return false;
         }
     }
     return hip_ok(hipGetLastError(), "batched extraction launch");
@@ -677,8 +694,10 @@ bool obs_subtract_batch(Obs &o, const std::vector<PendingSub> &subs)
         D.subs[b].fit = make_geom(p.l0, p.m0, p.flux, p.R_mu, p.e1, p.e2);
         D.subs[b].rnd = make_geom(p.l0, p.m0, p.flux, p.R_mu, 0., 0.);
     }
-    hipMemcpyAsync(o.d_desc->subs, D.subs, subs.size() * sizeof(BatchSub), hipMemcpyHostToDevice,
-                   g_stream);
+    if (!hip_ok(hipMemcpyAsync(o.d_desc->subs, D.subs, subs.size() * sizeof(BatchSub),
+                               hipMemcpyHostToDevice, g_stream),
+                "batched subtraction upload"))
+        return false;
     {
         KTimer t(RL2_K_SUBTRACT);
         launch_subtract_batch(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn,
@@ -732,14 +751,19 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
     for (int i = 0; i < n && tiled; i++)
         if (o.h_facets[items[i].slot].mod) tiled = false;  // given model: one group per (e, r)
     const bool want_detail = Lr || nit || (g_stats_on && !tiled);
+    // Every copy, fill and stream dependency queued below ANDs its status into q_ok, which
+    // is checked with the launches' before the results are read.
+    bool q_ok = true;
     // one copy back per evaluation: status, FLOP count, L_e and the error flags
     auto copy_out = [&]() {  // the results block itself is written to host memory
-        if (want_detail) {
-            hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double),
-                           hipMemcpyDeviceToHost, g_stream);
-            hipMemcpyAsync(b.h_nit, b.d_nit, (size_t)n * kMaxR * sizeof(int),
-                           hipMemcpyDeviceToHost, g_stream);
-        }
+        if (want_detail)
+            q_ok = hip_ok(hipMemcpyAsync(b.h_Lr, b.d_Lr, (size_t)n * kMaxR * sizeof(double),
+                                         hipMemcpyDeviceToHost, g_stream),
+                          "L_r copy") &&
+                   hip_ok(hipMemcpyAsync(b.h_nit, b.d_nit, (size_t)n * kMaxR * sizeof(int),
+                                         hipMemcpyDeviceToHost, g_stream),
+                          "step count copy") &&
+                   q_ok;
     };
     if (tiled) {
         // chunks per (item, r) from the host copy of each slot's cell count.  An item's chunk
@@ -798,8 +822,10 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         if (ntab > 0xffffffffu || tab_facet * 24u > kTileLdsBytes) ntab = 0;
         if (ntab && !b.ensure_tab(ntab)) return false;
         if (n > kInlItems)  // few items travel with the first kernel's arguments instead
-            hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
-                           g_stream);
+            q_ok = hip_ok(hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem),
+                                         hipMemcpyHostToDevice, g_stream),
+                          "items copy") &&
+                   q_ok;
         a.tasks = b.d_tasks;
         // the scalelength tasks in one or two groups: the second group's passes run on a
         // second stream, so one group's step kernels and launch gaps overlap the other's sums
@@ -828,10 +854,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // is (re)initialised by the pass-0 step kernel).  The second stream starts after
         // everything queued so far, and the r-marginalisation waits for both.
         auto passes_of = [&](int first, int count) {
-            if (ng == 2) {
-                hipEventRecord(g_fork, g_stream);
-                hipStreamWaitEvent(g_stream2, g_fork, 0);
-            }
+            if (ng == 2)
+                q_ok = hip_ok(hipEventRecord(g_fork, g_stream), "stream fork") &&
+                       hip_ok(hipStreamWaitEvent(g_stream2, g_fork, 0), "stream fork") && q_ok;
             for (int p = first; p < first + count; p++) {
                 const int pass = p == 0 ? 0 : 1;
                 if (ng == 2) {
@@ -847,10 +872,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 KTimer k(RL2_K_STEP);
                 launch_tile_step(g_stream, tg[0], pass);
             }
-            if (ng == 2) {
-                hipEventRecord(g_join, g_stream2);
-                hipStreamWaitEvent(g_stream, g_join, 0);
-            }
+            if (ng == 2)
+                q_ok = hip_ok(hipEventRecord(g_join, g_stream2), "stream join") &&
+                       hip_ok(hipStreamWaitEvent(g_stream, g_join, 0), "stream join") && q_ok;
         };
         // Fused passes (RL2_TUNE_FUSE_STEP): every pass after pass 0 takes the previous
         // pass's Newton step in its own work-groups, and one step kernel after the last
@@ -889,7 +913,10 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
             }
             if (ok) {
                 if (!b.ensure_ppart(total)) return false;
-                hipMemsetAsync(b.d_cnt, 0, (size_t)n * kMaxR * sizeof(unsigned), g_stream);
+                q_ok = hip_ok(hipMemsetAsync(b.d_cnt, 0, (size_t)n * kMaxR * sizeof(unsigned),
+                                             g_stream),
+                              "counter fill") &&
+                       q_ok;
                 PersistLaunch P;
                 P.t = t0;
                 P.t.tasks = b.d_tasks;
@@ -919,7 +946,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                     launch_rmarg(g_stream, ra);
                 }
                 copy_out();
-                if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+                if (!hip_ok(hipGetLastError(), "likelihood launch") || !q_ok) return false;
                 if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
                 if (b.h_status[0] != 0) {  // a task left unfinished: a wait gave up
                     set_error("likelihood: a device wait exceeded its bound");
@@ -980,7 +1007,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                     launch_rmarg(g_stream, ra);
                 }
                 copy_out();
-                if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+                if (!hip_ok(hipGetLastError(), "likelihood launch") || !q_ok) return false;
                 if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
                 if (b.h_status[0] == 0) break;
                 if (passes > 50) {  // the step count is capped at 50 (likelihood.c:284)
@@ -1015,7 +1042,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
                 launch_rmarg(g_stream, ra);
             }
             copy_out();
-            if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
+            if (!hip_ok(hipGetLastError(), "likelihood launch") || !q_ok) return false;
             if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
             if (b.h_status[0] == 0) break;
             if (passes > 50) {  // the step count is capped at 50 (likelihood.c:284)
@@ -1028,8 +1055,9 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         // FLOP-equivalents summed on the device: n * (19 + 18 + 25 * N_it) per (item, r)
         if (g_stats_on) stats_add_work(RL2_K_LIKELIHOOD, *b.h_flops, 0.);
     } else {
-        hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem), hipMemcpyHostToDevice,
-                       g_stream);
+        q_ok = hip_ok(hipMemcpyAsync(b.d_items, b.h_items, n * sizeof(LikItem),
+                                     hipMemcpyHostToDevice, g_stream),
+                      "items copy");
         {
             KTimer t(RL2_K_LIKELIHOOD);
             launch_likelihood(g_stream, a);
@@ -1040,7 +1068,7 @@ bool lik_eval(Obs &o, const std::vector<LikItem> &items, double *Le, int *err, d
         }
         if (!hip_ok(hipGetLastError(), "likelihood launch")) return false;
         copy_out();
-        if (!hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
+        if (!q_ok || !hip_ok(hipStreamSynchronize(g_stream), "likelihood sync")) return false;
         if (g_stats_on) {
             double fl = 0.;
             for (int i = 0; i < n; i++) {
@@ -1065,13 +1093,13 @@ static bool read_slot_n(Obs &o, int slot, unsigned *n)
            hip_ok(hipStreamSynchronize(g_stream), "sync");
 }
 
-static void prior_for(Obs &o, int slot, double flux)
+static bool prior_for(Obs &o, int slot, double flux)
 {
     const double mu = scale_mean(flux);
     FacetSlot &s = o.slots[slot];
     for (int k = 0; k < kMaxR; k++) s.rprior[k] = 0.;
     for (int nRo = 1; nRo < o.rg.numr; nRo++) s.rprior[nRo] = rfunc(mu, kRstd, o.rg.ro[nRo]);
-    o.facet_upload(slot);
+    return o.facet_upload(slot);
 }
 
 // One pass of data_processing.c:49-142 over `todo` (batch > 1: data_processing-mpi.c:65-216).
@@ -1101,15 +1129,22 @@ static bool proposal_inputs(Obs &o, const std::vector<unsigned> &todo, size_t b0
     }
     if (opt.features_out && pass == 0)
         for (size_t k = 0; k < nb; k++)
-            hipMemcpyAsync(opt.features_out + (size_t)todo[b0 + k] * kMlpIn, x + k * kMlpIn,
-                           kMlpIn * sizeof(float), hipMemcpyDeviceToHost, g_stream);
+            if (!hip_ok(hipMemcpyAsync(opt.features_out + (size_t)todo[b0 + k] * kMlpIn,
+                                       x + k * kMlpIn, kMlpIn * sizeof(float),
+                                       hipMemcpyDeviceToHost, g_stream),
+                        "features copy")) {
+                (void)hipStreamSynchronize(g_stream);  // the copies queued before it land first
+                return false;
+            }
     if (opt.proposal) {
         Mlp &net = opt.proposal->net;
         if (!net.forward(g_stream, (int)nb)) return false;
         std::vector<float> out(nb * kMlpOut);
-        hipMemcpyAsync(out.data(), net.act[kMlpLayers - 1], out.size() * sizeof(float),
-                       hipMemcpyDeviceToHost, g_stream);
-        if (!hip_ok(hipStreamSynchronize(g_stream), "proposal")) return false;
+        const bool copied = hip_ok(hipMemcpyAsync(out.data(), net.act[kMlpLayers - 1],
+                                                  out.size() * sizeof(float),
+                                                  hipMemcpyDeviceToHost, g_stream),
+                                   "proposal copy");
+        if (!hip_ok(hipStreamSynchronize(g_stream), "proposal") || !copied) return false;
         for (size_t k = 0; k < nb; k++) {
             const float *q = out.data() + k * kMlpOut;
             // covariance of (e1, e2, log r) = (L L^T)^-1 for the precision factor L
@@ -1208,7 +1243,7 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
             std::vector<ExtractReq> reqs(nb);
             for (size_t k = 0; k < nb; k++) {
                 const unsigned gal = todo[b0 + k];
-                prior_for(o, (int)k, flux[gal]);
+                if (!prior_for(o, (int)k, flux[gal])) return false;
                 const unsigned facet = facet_size(scale[gal], o.len);
                 reqs[k] = {(int)k, l[gal], m[gal], flux[gal], scale[gal], facet};
                 results[gal].facet = facet;
@@ -1218,7 +1253,7 @@ bool process_list(Obs &o, const std::vector<unsigned> &todo, int pass, const dou
         }
         for (size_t k = 0; k < nb && !batched; k++) {
             const unsigned gal = todo[b0 + k];
-            prior_for(o, (int)k, flux[gal]);
+            if (!prior_for(o, (int)k, flux[gal])) return false;
             const unsigned facet = facet_size(scale[gal], o.len);
             if (!obs_extract(o, (int)k, l[gal], m[gal], flux[gal], scale[gal], facet, false, 0.,
                              0., pending ? &pend : nullptr))
@@ -1432,31 +1467,34 @@ int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const do
               hip_ok(hipMalloc(&tmp, tmp_bytes ? tmp_bytes : 1), "hipMalloc");
     size_t res_bytes = o.nvis * (precision == RL2_PREC_FP64 ? 16 : 8);
     ok = ok && hip_ok(hipMalloc(&o.d_res, res_bytes), "hipMalloc residual");
+    auto up = [&](void *d, const void *hsrc, size_t bytes) {
+        return hip_ok(hipMemcpyAsync(d, hsrc, bytes, hipMemcpyHostToDevice, s), "obs upload");
+    };
+    ok = ok && up(tu, u, nrow * 8) && up(tv, v, nrow * 8) && up(tw, w, nrow * 8) &&
+         up(ts2, sigma2_vis, o.nvis * 4) && (!flag || up(tfl, flag, o.nvis)) &&
+         up(o.d_wn, o.wn.data(), nch * 8) && up(o.d_invlam, o.invlam.data(), nch * 8) &&
+         up(o.d_spec, o.spec.data(), nch * 8);
     if (ok) {
-        hipMemcpyAsync(tu, u, nrow * 8, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(tv, v, nrow * 8, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(tw, w, nrow * 8, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(ts2, sigma2_vis, o.nvis * 4, hipMemcpyHostToDevice, s);
-        if (flag) hipMemcpyAsync(tfl, flag, o.nvis, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(o.d_wn, o.wn.data(), nch * 8, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(o.d_invlam, o.invlam.data(), nch * 8, hipMemcpyHostToDevice, s);
-        hipMemcpyAsync(o.d_spec, o.spec.data(), nch * 8, hipMemcpyHostToDevice, s);
         // spatial sort of the rows (Morton order of (u, v) over [-len_m, len_m])
         double umax = 0.;
         for (unsigned c = 0; c < nch; c++) umax = fmax(umax, len * 2. * kPi / wavenumbers[c]);
         umax *= 1.0000001;
         launch_morton_keys(s, nrow, tu, tv, umax, keys, idx);
         uint32_t *sorted = nullptr;
-        sort_rows(s, nrow, keys, idx, keys2, idx2, tmp, tmp_bytes, &sorted);
-        hipMemcpyAsync(o.d_perm, sorted, nrow * 4, hipMemcpyDeviceToDevice, s);
-        launch_gather_rows(s, nrow, o.d_perm, tu, o.d_u);
-        launch_gather_rows(s, nrow, o.d_perm, tv, o.d_v);
-        launch_gather_rows(s, nrow, o.d_perm, tw, o.d_w);
-        launch_pack_sigma(s, nch, nrow, o.d_perm, ts2, tfl, o.d_s2f);
-        hipMemsetAsync(o.d_res, 0, res_bytes, s);
+        ok = sort_rows(s, nrow, keys, idx, keys2, idx2, tmp, tmp_bytes, &sorted) &&
+             hip_ok(hipMemcpyAsync(o.d_perm, sorted, nrow * 4, hipMemcpyDeviceToDevice, s),
+                    "row order");
+        if (ok) {
+            launch_gather_rows(s, nrow, o.d_perm, tu, o.d_u);
+            launch_gather_rows(s, nrow, o.d_perm, tv, o.d_v);
+            launch_gather_rows(s, nrow, o.d_perm, tw, o.d_w);
+            launch_pack_sigma(s, nch, nrow, o.d_perm, ts2, tfl, o.d_s2f);
+            ok = hip_ok(hipMemsetAsync(o.d_res, 0, res_bytes, s), "residual fill");
+        }
         ok = hip_ok(hipGetLastError(), "obs set-up") &&
-             hip_ok(hipStreamSynchronize(s), "obs set-up sync");
+             hip_ok(hipStreamSynchronize(s), "obs set-up sync") && ok;
     }
+    if (!ok) (void)hipStreamSynchronize(s);  // nothing queued may still read what is freed
     dfree(tu);
     dfree(tv);
     dfree(tw);
@@ -1466,7 +1504,7 @@ int rl2_obs_create(rl2_obs **out, unsigned int nch, unsigned long nrow, const do
     dfree(idx2);
     dfree(ts2);
     dfree(tfl);
-    if (tmp) hipFree(tmp);
+    if (tmp) (void)hipFree(tmp);
     if (!ok) {
         delete h;
         return RL2_ERR_HIP;
@@ -1487,12 +1525,17 @@ int rl2_obs_set_vis(rl2_obs *obs, const double *visData, const double *visSkyMod
     Obs &o = obs->o;
     double *dD = nullptr, *dS = nullptr;
     bool ok = dmalloc(&dD, 2 * o.nvis) && (!visSkyMod || dmalloc(&dS, 2 * o.nvis));
+    ok = ok &&
+         hip_ok(hipMemcpyAsync(dD, visData, o.nvis * 16, hipMemcpyHostToDevice, g_stream),
+                "visibility upload") &&
+         (!dS || hip_ok(hipMemcpyAsync(dS, visSkyMod, o.nvis * 16, hipMemcpyHostToDevice,
+                                       g_stream),
+                        "sky model upload"));
     if (ok) {
-        hipMemcpyAsync(dD, visData, o.nvis * 16, hipMemcpyHostToDevice, g_stream);
-        if (dS) hipMemcpyAsync(dS, visSkyMod, o.nvis * 16, hipMemcpyHostToDevice, g_stream);
         launch_residual_in(g_stream, o.prec, o.nch, o.nrow, o.d_perm, dD, dS, o.d_res);
         ok = hip_ok(hipGetLastError(), "set_vis") && hip_ok(hipStreamSynchronize(g_stream), "sync");
     }
+    if (!ok) (void)hipStreamSynchronize(g_stream);  // nothing queued may still read dD, dS
     dfree(dD);
     dfree(dS);
     ok = ok && o.measure_res_bound();
@@ -1507,9 +1550,10 @@ int rl2_obs_get_residual(rl2_obs *obs, double *residual)
     bool ok = dmalloc(&d, 2 * o.nvis);
     if (ok) {
         launch_residual_out(g_stream, o.prec, o.nch, o.nrow, o.d_perm, o.d_res, d);
-        hipMemcpyAsync(residual, d, o.nvis * 16, hipMemcpyDeviceToHost, g_stream);
         ok = hip_ok(hipGetLastError(), "get_residual") &&
-             hip_ok(hipStreamSynchronize(g_stream), "sync");
+             hip_ok(hipMemcpyAsync(residual, d, o.nvis * 16, hipMemcpyDeviceToHost, g_stream),
+                    "residual download");
+        ok = hip_ok(hipStreamSynchronize(g_stream), "sync") && ok;
     }
     dfree(d);
     return ok ? RL2_OK : RL2_ERR_HIP;
@@ -1534,14 +1578,16 @@ int rl2_obs_add_sources(rl2_obs *obs, unsigned int ngal, const double *flux, con
         g[i] = make_geom(l[i], m[i], flux[i], scale[i], e1 ? e1[i] : 0., e2 ? e2[i] : 0.);
     GalaxyGeom *dg = nullptr;
     if (!dmalloc(&dg, ngal)) return RL2_ERR_HIP;
-    hipMemcpyAsync(dg, g.data(), ngal * sizeof(GalaxyGeom), hipMemcpyHostToDevice, g_stream);
-    {
+    bool ok = hip_ok(hipMemcpyAsync(dg, g.data(), ngal * sizeof(GalaxyGeom),
+                                    hipMemcpyHostToDevice, g_stream),
+                     "source upload");
+    if (ok) {
         KTimer t(RL2_K_SOURCES);
         launch_add_sources(g_stream, o.prec, o.nch, o.nrow, o.d_u, o.d_v, o.d_w, o.d_wn, o.d_spec,
                            dg, ngal, sign, o.d_res);
     }
-    bool ok = hip_ok(hipGetLastError(), "add_sources") &&
-              hip_ok(hipStreamSynchronize(g_stream), "sync");
+    ok = ok && hip_ok(hipGetLastError(), "add_sources");
+    ok = hip_ok(hipStreamSynchronize(g_stream), "sync") && ok;
     dfree(dg);
     ok = ok && o.measure_res_bound();
     return ok ? RL2_OK : RL2_ERR_HIP;
@@ -1601,18 +1647,21 @@ int rl2_obs_extract(rl2_obs *obs, int slot, double l0, double m0, double flux, d
     s.n = n;
     if (ncoords) *ncoords = n;
     std::vector<double> re, im;
+    auto down = [](void *h, const void *d, size_t bytes) {
+        return hip_ok(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, g_stream), "extract copy");
+    };
+    bool ok = true;
     if (facet_vis) {
         re.resize(n);
         im.resize(n);
-        hipMemcpyAsync(re.data(), s.dre, n * 8, hipMemcpyDeviceToHost, g_stream);
-        hipMemcpyAsync(im.data(), s.dim, n * 8, hipMemcpyDeviceToHost, g_stream);
+        ok = down(re.data(), s.dre, n * 8) && down(im.data(), s.dim, n * 8);
     }
-    if (facet_sigma2) hipMemcpyAsync(facet_sigma2, s.s2, n * 8, hipMemcpyDeviceToHost, g_stream);
-    if (uu) hipMemcpyAsync(uu, s.uu, n * 8, hipMemcpyDeviceToHost, g_stream);
-    if (vv) hipMemcpyAsync(vv, s.vv, n * 8, hipMemcpyDeviceToHost, g_stream);
-    if (sum_w)
-        hipMemcpyAsync(sum_w, s.sumw, (size_t)facet * facet * 8, hipMemcpyDeviceToHost, g_stream);
-    if (!hip_ok(hipStreamSynchronize(g_stream), "extract copy")) return RL2_ERR_HIP;
+    if (facet_sigma2) ok = ok && down(facet_sigma2, s.s2, n * 8);
+    if (uu) ok = ok && down(uu, s.uu, n * 8);
+    if (vv) ok = ok && down(vv, s.vv, n * 8);
+    if (sum_w) ok = ok && down(sum_w, s.sumw, (size_t)facet * facet * 8);
+    ok = hip_ok(hipStreamSynchronize(g_stream), "extract copy") && ok;
+    if (!ok) return RL2_ERR_HIP;
     if (facet_vis)
         for (unsigned i = 0; i < n; i++) {
             facet_vis[2 * i] = re[i];
@@ -1637,23 +1686,26 @@ int rl2_obs_load_facet(rl2_obs *obs, int slot, unsigned int n, const double *fac
         re[i] = facet_vis[2 * i];
         im[i] = facet_vis[2 * i + 1];
     }
-    hipMemcpyAsync(s.dre, re.data(), n * 8, hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(s.dim, im.data(), n * 8, hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(s.s2, facet_sigma2, n * 8, hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(s.uu, uu, n * 8, hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(s.vv, vv, n * 8, hipMemcpyHostToDevice, g_stream);
-    hipMemcpyAsync(&o.d_facets[slot].n, &n, sizeof n, hipMemcpyHostToDevice, g_stream);
-    launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
-                      s.invs2, s.invs2_64, nullptr, 0, s.dc, s.loff);
+    auto up = [](void *d, const void *h, size_t bytes) {
+        return hip_ok(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_stream), "load_facet");
+    };
+    bool ok = up(s.dre, re.data(), n * 8) && up(s.dim, im.data(), n * 8) &&
+              up(s.s2, facet_sigma2, n * 8) && up(s.uu, uu, n * 8) && up(s.vv, vv, n * 8) &&
+              up(&o.d_facets[slot].n, &n, sizeof n);
+    if (ok)
+        launch_pack_facet(g_stream, &o.d_facets[slot], n, s.uu, s.vv, s.dre, s.dim, s.s2, s.pk,
+                          s.invs2, s.invs2_64, nullptr, 0, s.dc, s.loff);
     FacetDev &f = o.h_facets[slot];  // caller-given cells: no grid structure
-    if (f.cid || f.grid_facet) {
+    if (ok && (f.cid || f.grid_facet)) {
         f.cid = nullptr;
         f.ctr = nullptr;
         f.grid_facet = 0;
-        o.facet_upload(slot);
+        ok = o.facet_upload(slot);
     }
+    ok = hip_ok(hipStreamSynchronize(g_stream), "load_facet") && ok;  // before n and re, im go
+    if (!ok) return RL2_ERR_HIP;
     s.n = n;
-    return hip_ok(hipStreamSynchronize(g_stream), "load_facet") ? RL2_OK : RL2_ERR_HIP;
+    return RL2_OK;
 }
 
 int rl2_rmarg_eval(int nitems, int numr, const double *ro, const double *rprior,
@@ -1711,13 +1763,37 @@ int rl2_rmarg_eval(int nitems, int numr, const double *ro, const double *rprior,
     return ok ? RL2_OK : RL2_ERR_HIP;
 }
 
+int rl2_marg_position_eval(int n, const double *x, double *out)
+{
+    if (n <= 0 || !x || !out) {
+        set_error("rl2_marg_position_eval: n > 0 and both arrays required");
+        return RL2_ERR_ARG;
+    }
+    if (!ensure_init()) return RL2_ERR_NO_DEVICE;
+    double *d_x = nullptr, *d_out = nullptr;
+    bool ok = dmalloc(&d_x, n) && dmalloc(&d_out, n) &&
+              hip_ok(hipMemcpyAsync(d_x, x, n * sizeof(double), hipMemcpyHostToDevice,
+                                    g_stream), "marg_position x");
+    if (ok) {
+        launch_marg_position(g_stream, n, d_x, d_out);
+        ok = hip_ok(hipGetLastError(), "marg_position launch") &&
+             hip_ok(hipMemcpyAsync(out, d_out, n * sizeof(double), hipMemcpyDeviceToHost,
+                                   g_stream), "marg_position out") &&
+             hip_ok(hipStreamSynchronize(g_stream), "marg_position sync");
+    }
+    if (!ok) (void)hipStreamSynchronize(g_stream);
+    dfree(d_x);
+    dfree(d_out);
+    return ok ? RL2_OK : RL2_ERR_HIP;
+}
+
 int rl2_obs_set_prior(rl2_obs *obs, int slot, const double *rprior)
 {
     if (!obs || slot < 0 || !rprior) return RL2_ERR_ARG;
     Obs &o = obs->o;
     if (!o.ensure_slots(slot + 1)) return RL2_ERR_HIP;
     for (int k = 0; k < kMaxR; k++) o.slots[slot].rprior[k] = (k < o.rg.numr) ? rprior[k] : 0.;
-    o.facet_upload(slot);
+    if (!o.facet_upload(slot)) return RL2_ERR_HIP;
     return hip_ok(hipStreamSynchronize(g_stream), "set_prior") ? RL2_OK : RL2_ERR_HIP;
 }
 
@@ -1879,7 +1955,7 @@ int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double 
                       "autotune restore") &&
          hip_ok(hipStreamSynchronize(g_stream), "sync");
     o.res_bound = bound0;
-    hipFree(saved);
+    (void)hipFree(saved);
     if (ntrials) *ntrials = nt;
     if (lik_block) *lik_block = op.lik_block;
     return ok ? RL2_OK : RL2_ERR_HIP;
@@ -1888,7 +1964,7 @@ int rl2_autotune(rl2_obs *obs, unsigned int ngal, const double *l, const double 
 int rl2_stats_enable(int on)
 {
     if (!ensure_init()) return RL2_ERR_NO_DEVICE;
-    hipStreamSynchronize(g_stream);
+    if (!hip_ok(hipStreamSynchronize(g_stream), "stats sync")) return RL2_ERR_HIP;
     drain_stats();
     g_stats_on = on != 0;
     return RL2_OK;
@@ -1896,7 +1972,7 @@ int rl2_stats_enable(int on)
 
 int rl2_stats_reset(void)
 {
-    if (g_stream) hipStreamSynchronize(g_stream);
+    if (g_stream && !hip_ok(hipStreamSynchronize(g_stream), "stats sync")) return RL2_ERR_HIP;
     drain_stats();
     for (auto &k : g_stats) {
         k.ms = 0.;
@@ -1910,7 +1986,7 @@ int rl2_stats_reset(void)
 int rl2_stats_get(int kernel, rl2_kernel_stat *out)
 {
     if (kernel < 0 || kernel >= RL2_K_COUNT || !out) return RL2_ERR_ARG;
-    if (g_stream) hipStreamSynchronize(g_stream);
+    if (g_stream && !hip_ok(hipStreamSynchronize(g_stream), "stats sync")) return RL2_ERR_HIP;
     drain_stats();
     out->ms = g_stats[kernel].ms;
     out->launches = g_stats[kernel].launches;
