@@ -3,7 +3,8 @@ against the oracle's set_posterior_values + marginalise_posterior_r (likelihood.
 marginalise_r.c:29-223) on caller-given L_r rows, including the cases the fits reach only
 rarely: points at kBad or NaN, fewer than 5 points, all values below the -10 floor of ymax,
 a flat posterior, a rough one, one whose cubic spline overflows exp (the retry on the linear
-interpolant).  Tolerance: L_e within 1e-8 absolute (the wave's Gauss-Kronrod
+interpolant), one whose single sharp peak the first 41-point rule misses (L_e = -inf, as
+QUADPACK's acceptance of a zero first estimate gives in the reference).  Tolerance: L_e within 1e-8 absolute (the wave's Gauss-Kronrod
 sums differ from QUADPACK's sequential order only by rounding), error flags equal."""
 import ctypes as C
 
@@ -53,6 +54,9 @@ def _rows(numr, ro, rng):
     # alternating 0 / -40000: the natural cubic overshoots beyond exp's range, QAG on it fails
     # and the integral is retried on the linear interpolant (marginalise_r.c:171-210)
     rows.append(np.where(np.arange(numr - 1) % 2 == 0, 0.0, -40000.0))
+    # one point at 0, the rest at -5000: every node of the first 41-point rule sees exp(y) = 0,
+    # so QAG accepts an integral of 0 and L_e is -inf with no error flag, as in the reference
+    r = np.full(numr - 1, -5000.0); r[10] = 0.0; rows.append(r)
     base = 200.0 - 30.0 * (lro - lro.mean()) ** 2
     r = base.copy(); r[[0, 3, 7]] = KBAD; rows.append(r)               # scattered kBad
     r = base.copy(); r[:4] = KBAD; r[-3:] = KBAD; rows.append(r)        # ends at kBad
@@ -88,8 +92,11 @@ def test_rmarg_kernel_matches_the_oracle(gpu):
         assert err[i] == oerr, (i, err[i], oerr)
         if np.isnan(ole):
             assert np.isnan(le[i]), (i, le[i])
+        elif np.isinf(ole):
+            assert le[i] == ole, (i, le[i], ole)
         else:
             assert abs(le[i] - ole) <= 1e-8, (i, le[i], ole)
+    assert np.isneginf(le[25]) and err[25] == 0  # the peak the first rule misses
     # the cases whose outcome the reference fixes without quadrature
     assert err[-6] == 1 and le[-6] == KBAD      # 4 points: no marginalisation
     assert err[-4] == 1 and le[-4] == KBAD      # no point at all
